@@ -1,0 +1,180 @@
+"""Benchmark: SPH + self-gravity giant-impact KDK steps at 1,048,576 particles on one MI355X.
+
+One "step" is the reference's leapfrog step: two full force evaluations (Morton sort, LBVH, neighbour lists,
+smoothing-length solve, density/EOS, SPH forces, Barnes-Hut gravity) with a half-kick + drift between them
+and a half-kick after (ie/shaders/sph_step.wgsl:213-237, ie/crates/engine/src/hydrostatic.rs:345-359), with
+the Courant dt formed on the device each step. Inputs: the two-body impact (0.9 + 0.1 Earth masses, 45 deg,
+1.1 v_esc, 30% iron cores, granite mantles, seeded jittered lattices) at N = 1,048,576, already resident in
+HBM when the timed region starts.
+
+Prints one JSON line. `value` is the particle-steps/s of the headline variant (learned-h seed + split
+precision when a trained network is available, otherwise the plain Newton FP32 step, named in config).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+METRIC = "SPH+gravity particle-steps/s at 1M particles, learned-h vs plain, equal tolerance"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+FLOPS_PER_INTERACTION = 20.0  # softened monopole / particle-particle interaction (standard count)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1048576)
+    ap.add_argument("--leaf", type=int, default=16)
+    ap.add_argument("--profile-steps", type=int, default=3)
+    ap.add_argument("--variants", default="plain,learned")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=65536)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def run_variant(ic, precision, h_mode, args, weights=None):
+    from botheadstudios_amd.gpu_sph import GpuSph
+
+    sph = GpuSph(ic.n, leaf_size=args.leaf, precision=precision, h_mode=h_mode)
+    try:
+        if weights is not None:
+            sph.load_mlp(weights)
+        sph.upload(ic.particles48(), ic.eos, ic.softening)
+        sph.set_dt(0.0)      # adaptive Courant dt on the device
+        sph.set_av(1.0, 2.0)
+        sph.encode_kdk(args.warmup)
+        sph.sync()
+        sph.stats(reset=True)
+        t0 = time.perf_counter()
+        sph.encode_kdk(args.steps)
+        sph.sync()
+        t1 = time.perf_counter()
+        st = sph.stats(reset=True)
+        # Per-stage HIP-event timing over a separate profiled pass (direct launches, same kernels).
+        sph.profile(True)
+        sph.encode_kdk(args.profile_steps)
+        sph.sync()
+        prof = sph.profile_read(reset=True)
+        pst = sph.stats(reset=True)
+        sph.profile(False)
+        for k in ("nbr_overflow", "walk_overflow", "nonfinite"):
+            if st[k] or pst[k]:
+                raise RuntimeError(f"{precision}/{h_mode}: {k} = {st[k]}/{pst[k]}")
+        sec = t1 - t0
+        return dict(sec=sec, ms_per_step=1e3 * sec / args.steps, value=ic.n * args.steps / sec, stats=st,
+                    profile=prof, profile_stats=pst)
+    finally:
+        sph.close()
+
+
+def roofline(res, n, args):
+    """Dominant stage -> achieved vs peak. Gravity (FP32 VALU): 20 flops per evaluated interaction."""
+    prof = res["profile"]
+    stage = max(prof, key=lambda k: prof[k][0])
+    ms, calls = prof[stage]
+    avg_ms = ms / max(calls, 1)
+    pst = res["profile_stats"]
+    if stage == "gravity":
+        inter = pst["grav_cell_interactions"] + pst["grav_pp_interactions"]
+        flops = FLOPS_PER_INTERACTION * inter / max(calls, 1)
+        achieved = flops / (avg_ms * 1e-3) / 1e12
+        return dict(kernel="k_gravity_walk", bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS,
+                    unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=None, avg_ms=avg_ms,
+                    per_launch=dict(interactions=inter / max(calls, 1), flops=flops,
+                                    cell_per_particle=pst["grav_cell_interactions"] / max(calls, 1) / n,
+                                    pp_per_particle=pst["grav_pp_interactions"] / max(calls, 1) / n))
+    # HBM-bound stages: algorithmic bytes per particle per launch (SoA compulsory traffic).
+    bytes_per = {"sort": 32 + 64, "gather": 2 * 56, "tree": 16, "nbr": 20, "hsolve": 28, "density": 60,
+                 "forces": 60, "kick": 104}.get(stage, 64)
+    achieved = bytes_per * n / (avg_ms * 1e-3) / 1e9
+    return dict(kernel=stage, bound="hbm", achieved=achieved, peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=achieved / HBM_PEAK_GBS, traffic=None, avg_ms=avg_ms)
+
+
+def cpu_baseline(args):
+    """The oracle's FP64 restatement of HydroBody::step (hydrostatic.rs:345-359: grid neighbours, octree
+    Barnes-Hut theta 0.5 with leaf bucket 8, fixed h = 2 (m/rho0)^(1/3), Courant dt), 1 thread, on the same
+    impact geometry at a bounded particle count."""
+    from botheadstudios_amd import ics
+    from oracle import oracle
+
+    ic = ics.config_ic(4, n=args.cpu_n)
+    o = oracle.Hydro(ic.pos, ic.vel, ic.u, ic.h * (2.0 / ics.ETA_48), ic.m, ic.mat.astype(np.int32), ic.eos,
+                     ic.softening, theta=0.5, gravity_mode=0, cs_fd=0, brute=False, leaf_cap=8, nthreads=1)
+    t0 = time.perf_counter()
+    o.kdk(args.cpu_steps, dt=0.0, cfl=0.1)
+    sec = time.perf_counter() - t0
+    model = ""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        model = next((l.split(":", 1)[1].strip() for l in out.splitlines() if l.startswith("Model name")), "")
+    except Exception:
+        pass
+    return dict(value=args.cpu_n * args.cpu_steps / sec, unit="particle-steps/s", cores=1, kind="port",
+                sample=f"oracle FP64 HydroBody KDK step (fixed h, octree leaf 8, theta 0.5) on the same impact "
+                       f"geometry at N={args.cpu_n}, {args.cpu_steps} steps, {sec:.1f} s, 1 thread "
+                       f"({model or platform.processor()}, nproc={os.cpu_count()})")
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+                                   "design-only, see DESIGN.md)"}))
+        sys.exit(2)
+    from botheadstudios_amd import ics
+    from botheadstudios_amd import mlp as mlp_mod
+
+    ic = ics.config_ic(4, n=args.n)
+    variants = {}
+    wanted = args.variants.split(",")
+    if "plain" in wanted:
+        log("plain: Newton h, FP32 ...")
+        variants["plain_fp32"] = run_variant(ic, "fp32", "newton", args)
+        log(f"  {variants['plain_fp32']['ms_per_step']:.3f} ms/step")
+    weights = mlp_mod.default_weights_path()
+    if "learned" in wanted and weights is not None:
+        log("learned: MLP-seeded Newton h, split precision ...")
+        variants["learned_split"] = run_variant(ic, "split", "learned", args, weights=weights)
+        log(f"  {variants['learned_split']['ms_per_step']:.3f} ms/step")
+    head = "learned_split" if "learned_split" in variants else "plain_fp32"
+    res = variants[head]
+    line = {
+        "metric": METRIC, "value": res["value"], "unit": "particle-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": res["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32" if head == "plain_fp32" else "fp32+fp64",
+        "data": "synthetic (seeded jittered-lattice two-body impact)",
+        "config": {"workload": f"two-body giant impact, N={ic.n}, 0.9+0.1 Earth masses, 45 deg, 1.1 v_esc; "
+                               f"KDK step = 2 force evaluations; variant={head}",
+                   "n": ic.n, "leaf": args.leaf, "theta": 0.5, "n_ngb": 48, "h_tol": 1e-4, "variant": head},
+        "roofline": roofline(res, ic.n, args),
+        "variants": {k: {"value": v["value"], "ms_per_step": v["ms_per_step"],
+                         "mean_h_evals": v["stats"]["mean_h_evals"], "mean_h_iters": v["stats"]["mean_h_iters"],
+                         "mean_neighbours": v["stats"]["mean_neighbours"],
+                         "stage_ms_per_step": {s: round(ms / max(args.profile_steps, 1), 4)
+                                               for s, (ms, c) in v["profile"].items()}}
+                     for k, v in variants.items()},
+    }
+    if "plain_fp32" in variants and "learned_split" in variants:
+        line["learned_vs_plain"] = variants["learned_split"]["value"] / variants["plain_fp32"]["value"]
+    line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -132,3 +132,13 @@ struct Kahan3 {
     s[q] = t;
   }
 };
+
+// Blocks for a persistent launch: every CU filled to the kernel's occupancy (a multiple of the 8 XCDs).
+inline unsigned persistent_blocks(const void* kernel, int block) {
+  int dev = 0, cus = 0, occ = 0;
+  hipGetDevice(&dev);
+  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, 0);
+  const unsigned nb = (unsigned)((cus > 0 ? cus : 256) * (occ > 0 ? occ : 1));
+  return (nb + 7u) & ~7u;
+}

@@ -1,0 +1,704 @@
+// C ABI of libihs_sph: handle lifetime, upload/readback, the force-evaluation pipeline, the KDK step
+// (captured in a hipGraph), relaxation, standalone gravity and inspection. Mirrors GpuSph
+// (ie/crates/engine/src/gpu_sph.rs:742-922) and GravityField (ie/crates/engine/src/gpu_gravity.rs:532-626).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ihs_sph.h"
+#include "launch.h"
+#include "mlp.h"
+
+namespace {
+
+constexpr float kPi = 3.14159265358979f;
+
+struct Scratch {
+  // sorted-order compute arrays
+  float4* xm = nullptr;
+  float* hs = nullptr;
+  float4* rpc = nullptr;
+  float4* vh = nullptr;
+  float4* asph = nullptr;
+  float4* accg = nullptr;
+  uint32_t* nbr = nullptr;
+  uint32_t* ncount = nullptr;
+  uint32_t* flags = nullptr;
+  unsigned long long* counters = nullptr;
+  double* epart = nullptr;
+  double* eout = nullptr;
+  StepCtl* ctl = nullptr;
+  void* eos = nullptr;
+  ihs_particle* staging = nullptr;  // device AoS for upload/readback
+  // standalone gravity
+  double* gpos = nullptr;
+  double* gpx = nullptr;
+  double* gpy = nullptr;
+  double* gpz = nullptr;
+  float* gm = nullptr;
+  double* gmass = nullptr;
+  double* gacc = nullptr;
+  double* gpot = nullptr;
+};
+
+}  // namespace
+
+struct ihs_handle {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  ihs_config cfg{};
+  int64_t cap = 0, n = 0;
+  int split = 0;
+  int cur = 0;
+  MasterBufs st[2];
+  TreeBufs tb;
+  Scratch sc;
+  MlpDev mlp;
+  float softening = 0.f;
+  float h_cap = 0.f;
+  int64_t force_evals = 0;
+  bool have_fe = false;
+  // One captured KDK step per starting parity of the ping-pong state buffers.
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  // stage profiling
+  bool prof_on = false;
+  std::vector<hipEvent_t> prof_ev;
+  std::vector<int> prof_stage;
+  size_t prof_used = 0;
+  double prof_ms[IHS_STAGE_COUNT] = {0};
+  int64_t prof_calls[IHS_STAGE_COUNT] = {0};
+  std::vector<void*> allocs;
+  std::string err;
+};
+
+namespace {
+
+int fail(ihs_handle* h, int code, const char* fmt, ...) {
+  if (h) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    h->err = buf;
+  }
+  return code;
+}
+
+#define HIP_TRY(h, expr)                                                                    \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) return fail((h), IHS_ERR_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+hipError_t dalloc(ihs_handle* h, T** p, size_t count) {
+  void* q = nullptr;
+  hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
+  if (e == hipSuccess) {
+    h->allocs.push_back(q);
+    *p = (T*)q;
+  }
+  return e;
+}
+
+hipError_t alloc_raw(ihs_handle* h, void** p, size_t bytes) {
+  hipError_t e = hipMalloc(p, std::max<size_t>(bytes, 1));
+  if (e == hipSuccess) h->allocs.push_back(*p);
+  return e;
+}
+
+int alloc_all(ihs_handle* h) {
+  const int64_t C = h->cap;
+  const int K = h->cfg.leaf_size;
+  const int64_t Lc = (C + K - 1) / K + 1;
+  for (int b = 0; b < 2; ++b) {
+    MasterBufs& m = h->st[b];
+    HIP_TRY(h, alloc_raw(h, &m.px, 8 * C));
+    HIP_TRY(h, alloc_raw(h, &m.py, 8 * C));
+    HIP_TRY(h, alloc_raw(h, &m.pz, 8 * C));
+    HIP_TRY(h, alloc_raw(h, &m.u, 8 * C));
+    HIP_TRY(h, dalloc(h, &m.vx, C));
+    HIP_TRY(h, dalloc(h, &m.vy, C));
+    HIP_TRY(h, dalloc(h, &m.vz, C));
+    HIP_TRY(h, dalloc(h, &m.h, C));
+    HIP_TRY(h, dalloc(h, &m.m, C));
+    HIP_TRY(h, dalloc(h, &m.mat, C));
+    HIP_TRY(h, dalloc(h, &m.id, C));
+    HIP_TRY(h, dalloc(h, &m.prov, C));
+    HIP_TRY(h, dalloc(h, &m.rho, C));
+    HIP_TRY(h, dalloc(h, &m.feat, C));
+    HIP_TRY(h, dalloc(h, &m.hiter, C));
+  }
+  TreeBufs& t = h->tb;
+  t.cap = C;
+  t.lcap = Lc;
+  t.nb_cap = radix_tiles(C);
+  HIP_TRY(h, dalloc(h, &t.bbox, 8));
+  for (int b = 0; b < 2; ++b) {
+    HIP_TRY(h, dalloc(h, &t.keys[b], C));
+    HIP_TRY(h, dalloc(h, &t.vals[b], C));
+  }
+  HIP_TRY(h, dalloc(h, &t.hist, (size_t)256 * t.nb_cap));
+  HIP_TRY(h, dalloc(h, &t.dtot, 256));
+  HIP_TRY(h, dalloc(h, &t.ccode, Lc));
+  HIP_TRY(h, dalloc(h, &t.child, Lc));
+  HIP_TRY(h, dalloc(h, &t.range, Lc));
+  HIP_TRY(h, dalloc(h, &t.parent, 2 * Lc));
+  HIP_TRY(h, dalloc(h, &t.ready, Lc));
+  HIP_TRY(h, dalloc(h, &t.ncom, 2 * Lc));
+  HIP_TRY(h, dalloc(h, &t.nbmin, 2 * Lc));
+  HIP_TRY(h, dalloc(h, &t.nbmax, 2 * Lc));
+  HIP_TRY(h, dalloc(h, &t.gstart, Lc));
+  HIP_TRY(h, dalloc(h, &t.gblk, Lc / 1024 + 2));
+  HIP_TRY(h, dalloc(h, &t.groups, C + 1));
+  HIP_TRY(h, dalloc(h, &t.ngroups, 1));
+  HIP_TRY(h, dalloc(h, &t.wq, 16));
+  Scratch& s = h->sc;
+  HIP_TRY(h, dalloc(h, &s.xm, C));
+  HIP_TRY(h, dalloc(h, &s.hs, C));
+  HIP_TRY(h, dalloc(h, &s.rpc, C));
+  HIP_TRY(h, dalloc(h, &s.vh, C));
+  HIP_TRY(h, dalloc(h, &s.asph, C));
+  HIP_TRY(h, dalloc(h, &s.accg, C));
+  HIP_TRY(h, dalloc(h, &s.nbr, (size_t)C * h->cfg.nbr_capacity));
+  HIP_TRY(h, dalloc(h, &s.ncount, C));
+  HIP_TRY(h, dalloc(h, &s.flags, FLAG_COUNT));
+  HIP_TRY(h, dalloc(h, &s.counters, CNT_COUNT));
+  HIP_TRY(h, dalloc(h, &s.epart, 9 * 1024));
+  HIP_TRY(h, dalloc(h, &s.eout, 16));
+  HIP_TRY(h, dalloc(h, &s.ctl, 1));
+  HIP_TRY(h, alloc_raw(h, &s.eos, 16 * sizeof(ihs_eos)));
+  HIP_TRY(h, dalloc(h, &s.staging, C));
+  HIP_TRY(h, hipMemset(s.flags, 0, FLAG_COUNT * sizeof(uint32_t)));
+  HIP_TRY(h, hipMemset(s.counters, 0, CNT_COUNT * sizeof(unsigned long long)));
+  HIP_TRY(h, hipMemset(s.nbr, 0, (size_t)C * h->cfg.nbr_capacity * sizeof(uint32_t)));
+  HIP_TRY(h, hipMemset(s.ncount, 0, C * sizeof(uint32_t)));
+  StepCtl ctl{};
+  ctl.dt_fixed = 0.0;
+  ctl.cfl = h->cfg.cfl;
+  ctl.av[0] = 1.0f;
+  ctl.av[1] = 2.0f;
+  ctl.damp = 1.0f;
+  HIP_TRY(h, hipMemcpy(s.ctl, &ctl, sizeof ctl, hipMemcpyHostToDevice));
+  if (mlp_alloc(h->mlp, C) != 0) return fail(h, IHS_ERR_HIP, "MLP buffer allocation failed");
+  return IHS_OK;
+}
+
+// Records an event ending `stage` (stage < 0: a start marker) when profiling is on.
+void prof_mark(ihs_handle* h, int stage) {
+  if (!h->prof_on) return;
+  if (h->prof_used == h->prof_ev.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    h->prof_ev.push_back(e);
+    h->prof_stage.push_back(-1);
+  }
+  hipEventRecord(h->prof_ev[h->prof_used], h->stream);
+  h->prof_stage[h->prof_used] = stage;
+  h->prof_used++;
+}
+
+void prof_flush(ihs_handle* h) {
+  if (!h->prof_on || h->prof_used == 0) return;
+  hipStreamSynchronize(h->stream);
+  for (size_t i = 1; i < h->prof_used; ++i) {
+    const int st = h->prof_stage[i];
+    if (st < 0) continue;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, h->prof_ev[i - 1], h->prof_ev[i]) == hipSuccess) {
+      h->prof_ms[st] += ms;
+      h->prof_calls[st] += 1;
+    }
+  }
+  h->prof_used = 0;
+}
+
+bool valid_leaf(int k) { return k >= 1 && k <= 64 && (k & (k - 1)) == 0; }
+
+// One force evaluation: sort, permute, tree, neighbour lists, h-solve, density/EOS, forces, gravity.
+void force_eval(ihs_handle* h) {
+  hipStream_t s = h->stream;
+  const int64_t n = h->n;
+  const int K = h->cfg.leaf_size;
+  MasterBufs& S = h->st[h->cur];
+  MasterBufs& D = h->st[h->cur ^ 1];
+  Scratch& sc = h->sc;
+  const bool kahan = h->split != 0;
+  prof_mark(h, -1);
+  launch_fe_init(s, sc.flags);
+  launch_bbox_morton(s, h->split, S.px, S.py, S.pz, n, h->tb);
+  launch_radix_sort(s, n, 30, h->tb);
+  launch_fix_ties(s, n, S.id, h->tb);
+  prof_mark(h, IHS_STAGE_SORT);
+  const float margin = h->cfg.h_mode == IHS_H_FIXED ? 1.0f : h->cfg.h_margin;
+  launch_gather(s, h->split, n, h->tb.perm, S, D, sc.xm, sc.hs, margin, h->h_cap);
+  h->cur ^= 1;
+  prof_mark(h, IHS_STAGE_GATHER);
+  launch_karras(s, n, K, h->tb);
+  launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb);
+  launch_groups(s, n, K, h->tb);
+  prof_mark(h, IHS_STAGE_TREE);
+  launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags, sc.counters);
+  prof_mark(h, IHS_STAGE_NBR);
+  const float eta = std::cbrt(3.0f * h->cfg.n_ngb / (4.0f * kPi));
+  if (h->cfg.h_mode != IHS_H_FIXED) {
+    if (h->cfg.h_mode == IHS_H_LEARNED) mlp_seed_h(s, h->mlp, n, D, sc.hs);
+    launch_hsolve(s, n, sc.xm, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta * eta * eta, h->cfg.h_tol,
+                  h->cfg.h_max_iter, sc.flags, sc.counters);
+    prof_mark(h, IHS_STAGE_HSOLVE);
+  }
+  const int feat = h->cfg.h_mode == IHS_H_LEARNED || h->mlp.record;
+  launch_density(s, h->split, feat, n, sc.xm, D.h, D.vx, D.vy, D.vz, D.u, D.mat, sc.eos, sc.nbr, sc.ncount, n, sc.rpc,
+                 sc.vh, D.rho, D.feat, sc.flags);
+  prof_mark(h, IHS_STAGE_DENSITY);
+  launch_forces(s, kahan, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, n,
+                (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph);
+  prof_mark(h, IHS_STAGE_FORCES);
+  launch_gravity_walk(s, n, K, kahan, h->softening * h->softening, sc.xm, h->tb, sc.accg, sc.flags, sc.counters);
+  prof_mark(h, IHS_STAGE_GRAVITY);
+  if (h->mlp.record) mlp_record(s, h->mlp, n, D);
+  h->force_evals++;
+  h->have_fe = true;
+}
+
+void kdk_step(ihs_handle* h) {
+  force_eval(h);
+  launch_kick_drift(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, h->sc.flags);
+  prof_mark(h, IHS_STAGE_KICK);
+  force_eval(h);
+  launch_kick(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl);
+  prof_mark(h, IHS_STAGE_KICK);
+  prof_flush(h);
+}
+
+void drop_graph(ihs_handle* h) {
+  for (int p = 0; p < 2; ++p) {
+    if (h->graph_exec[p]) hipGraphExecDestroy(h->graph_exec[p]);
+    if (h->graph[p]) hipGraphDestroy(h->graph[p]);
+    h->graph_exec[p] = nullptr;
+    h->graph[p] = nullptr;
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ihs_abi_version(void) { return IHS_ABI_VERSION; }
+
+void ihs_default_config(ihs_config* c) {
+  if (!c) return;
+  std::memset(c, 0, sizeof *c);
+  c->capacity = 1 << 20;
+  c->leaf_size = 16;
+  c->precision = IHS_PREC_FP32;
+  c->h_mode = IHS_H_NEWTON;
+  c->theta = 0.5f;
+  c->cfl = 0.1f;
+  c->n_ngb = 48.0f;
+  c->h_tol = 1.0e-4f;
+  c->h_max_iter = 40;
+  c->h_margin = 1.25f;
+  c->h_cap_factor = 4.0f;
+  c->nbr_capacity = 192;
+  c->use_graph = 1;
+}
+
+int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
+  if (!out) return IHS_ERR_ARG;
+  *out = nullptr;
+  ihs_config c;
+  if (cfg) c = *cfg;
+  else ihs_default_config(&c);
+  if (c.capacity < 1 || !valid_leaf(c.leaf_size) || c.nbr_capacity < 1 || c.h_mode < 0 || c.h_mode > 2 ||
+      c.precision < 0 || c.precision > 1 || !(c.theta >= 0.f) || !(c.h_margin >= 1.0f))
+    return IHS_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IHS_ERR_HIP;
+  if (device < 0 || device >= ndev) return IHS_ERR_ARG;
+  c.nbr_capacity = (c.nbr_capacity + 3) & ~3;  // lists are stored in 16-byte chunks of four
+  ihs_handle* h = new ihs_handle();
+  h->device = device;
+  h->cfg = c;
+  h->cap = c.capacity;
+  h->split = c.precision == IHS_PREC_SPLIT;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete h;
+    return IHS_ERR_HIP;
+  }
+  if (alloc_all(h) != IHS_OK) {
+    ihs_destroy(h);
+    return IHS_ERR_HIP;
+  }
+  *out = h;
+  return IHS_OK;
+}
+
+int ihs_destroy(ihs_handle* h) {
+  if (!h) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  if (h->stream) hipStreamSynchronize(h->stream);
+  drop_graph(h);
+  mlp_free(h->mlp);
+  for (hipEvent_t e : h->prof_ev) hipEventDestroy(e);
+  for (void* p : h->allocs) hipFree(p);
+  if (h->stream) hipStreamDestroy(h->stream);
+  delete h;
+  return IHS_OK;
+}
+
+const char* ihs_last_error(const ihs_handle* h) { return h ? h->err.c_str() : "null handle"; }
+
+int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* eos, int32_t n_mat, float softening) {
+  if (!h) return IHS_ERR_ARG;
+  if (n < 1 || !p) return fail(h, IHS_ERR_ARG, "upload needs at least one particle");
+  if (n > h->cap) return fail(h, IHS_ERR_CAPACITY, "upload of %lld particles exceeds capacity %lld", (long long)n,
+                              (long long)h->cap);
+  if (!eos || n_mat < 1 || n_mat > 16) return fail(h, IHS_ERR_ARG, "need 1..16 EOS records, got %d", n_mat);
+  float hmax = 0.f;
+  for (int64_t i = 0; i < n; ++i) {
+    if (p[i].mat >= (uint32_t)n_mat) return fail(h, IHS_ERR_ARG, "particle %lld has mat %u >= n_mat %d", (long long)i,
+                                                 p[i].mat, n_mat);
+    if (!(p[i].h > 0.f) || !std::isfinite(p[i].h)) return fail(h, IHS_ERR_ARG, "particle %lld has h <= 0", (long long)i);
+    hmax = std::max(hmax, p[i].h);
+  }
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  drop_graph(h);
+  h->n = n;
+  h->cur = 0;
+  h->softening = softening;
+  h->h_cap = h->cfg.h_cap_factor * hmax;
+  h->have_fe = false;
+  HIP_TRY(h, hipMemcpy(h->sc.eos, eos, n_mat * sizeof(ihs_eos), hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(h->sc.staging, p, n * sizeof(ihs_particle), hipMemcpyHostToDevice));
+  launch_unpack(h->stream, h->split, n, (const SphParticle48*)h->sc.staging, h->st[0]);
+  StepCtl ctl;
+  HIP_TRY(h, hipMemcpy(&ctl, h->sc.ctl, sizeof ctl, hipMemcpyDeviceToHost));
+  ctl.t = 0.0;
+  ctl.dt = 0.0;
+  ctl.steps = 0;
+  HIP_TRY(h, hipMemcpy(h->sc.ctl, &ctl, sizeof ctl, hipMemcpyHostToDevice));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipGetLastError());
+  if (h->cfg.h_mode != IHS_H_FIXED) {
+    // The uploaded h is a guess: converge it before stepping. Each pass can grow h by the search
+    // margin, so repeat while the number of solves stopped at the searched radius keeps falling.
+    uint32_t prev = 0xffffffffu;
+    for (int pass = 0; pass < 16; ++pass) {
+      uint32_t before = 0, after = 0;
+      HIP_TRY(h, hipMemcpy(&before, h->sc.flags + FLAG_H_CAPPED, 4, hipMemcpyDeviceToHost));
+      force_eval(h);
+      HIP_TRY(h, hipStreamSynchronize(h->stream));
+      HIP_TRY(h, hipMemcpy(&after, h->sc.flags + FLAG_H_CAPPED, 4, hipMemcpyDeviceToHost));
+      const uint32_t capped = after - before;
+      if (capped == 0 || capped >= prev) break;
+      prev = capped;
+    }
+    HIP_TRY(h, hipGetLastError());
+  }
+  // Counters describe stepping only.
+  HIP_TRY(h, hipMemset(h->sc.counters, 0, CNT_COUNT * sizeof(unsigned long long)));
+  {
+    uint32_t fl[FLAG_COUNT] = {0};
+    fl[FLAG_SIGNAL_MIN] = 0xffffffffu;
+    HIP_TRY(h, hipMemcpy(h->sc.flags, fl, sizeof fl, hipMemcpyHostToDevice));
+  }
+  h->force_evals = 0;
+  return IHS_OK;
+}
+
+static int update_ctl(ihs_handle* h, void (*fn)(StepCtl&, const void*), const void* arg) {
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  StepCtl ctl;
+  HIP_TRY(h, hipMemcpy(&ctl, h->sc.ctl, sizeof ctl, hipMemcpyDeviceToHost));
+  fn(ctl, arg);
+  HIP_TRY(h, hipMemcpy(h->sc.ctl, &ctl, sizeof ctl, hipMemcpyHostToDevice));
+  return IHS_OK;
+}
+
+int ihs_set_dt(ihs_handle* h, double dt, float damp) {
+  if (!h) return IHS_ERR_ARG;
+  struct A {
+    double dt;
+    float damp;
+  } a{dt, damp};
+  return update_ctl(h, [](StepCtl& c, const void* p) {
+    const A* q = (const A*)p;
+    c.dt_fixed = q->dt > 0.0 ? q->dt : 0.0;
+    c.damp = q->damp;
+  }, &a);
+}
+
+int ihs_set_av(ihs_handle* h, float alpha, float beta) {
+  if (!h) return IHS_ERR_ARG;
+  float a[2] = {alpha, beta};
+  return update_ctl(h, [](StepCtl& c, const void* p) {
+    c.av[0] = ((const float*)p)[0];
+    c.av[1] = ((const float*)p)[1];
+  }, a);
+}
+
+int ihs_set_omega(ihs_handle* h, float omega) {
+  if (!h) return IHS_ERR_ARG;
+  return update_ctl(h, [](StepCtl& c, const void* p) { c.omega = *(const float*)p; }, &omega);
+}
+
+int ihs_force_eval(ihs_handle* h) {
+  if (!h) return IHS_ERR_ARG;
+  if (h->n < 1) return fail(h, IHS_ERR_STATE, "no particles uploaded");
+  hipSetDevice(h->device);
+  force_eval(h);
+  HIP_TRY(h, hipGetLastError());
+  return IHS_OK;
+}
+
+int ihs_step_kdk(ihs_handle* h, int32_t substeps) {
+  if (!h) return IHS_ERR_ARG;
+  if (h->n < 1) return fail(h, IHS_ERR_STATE, "no particles uploaded");
+  if (substeps < 0) return fail(h, IHS_ERR_ARG, "negative substeps");
+  hipSetDevice(h->device);
+  const bool graph = h->cfg.use_graph && !h->mlp.record && !h->prof_on;
+  if (!graph) {
+    for (int i = 0; i < substeps; ++i) kdk_step(h);
+    HIP_TRY(h, hipGetLastError());
+    return IHS_OK;
+  }
+  const int par = h->cur;  // a KDK step flips the buffers twice, so the parity is preserved
+  if (!h->graph_exec[par]) {
+    const int64_t fe0 = h->force_evals;
+    HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    kdk_step(h);
+    hipError_t e = hipStreamEndCapture(h->stream, &h->graph[par]);
+    h->force_evals = fe0;
+    if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+    HIP_TRY(h, hipGraphInstantiate(&h->graph_exec[par], h->graph[par], nullptr, nullptr, 0));
+  }
+  for (int i = 0; i < substeps; ++i) {
+    HIP_TRY(h, hipGraphLaunch(h->graph_exec[par], h->stream));
+    h->force_evals += 2;
+    h->have_fe = true;
+  }
+  return IHS_OK;
+}
+
+int ihs_relax(ihs_handle* h, int32_t steps) {
+  if (!h) return IHS_ERR_ARG;
+  if (h->n < 1) return fail(h, IHS_ERR_STATE, "no particles uploaded");
+  hipSetDevice(h->device);
+  for (int i = 0; i < steps; ++i) {
+    force_eval(h);
+    launch_relax(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl);
+  }
+  HIP_TRY(h, hipGetLastError());
+  return IHS_OK;
+}
+
+int64_t ihs_readback(ihs_handle* h, ihs_particle* out, int64_t cap, int32_t blocking) {
+  if (!h || !out) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  if (!blocking) {
+    const hipError_t q = hipStreamQuery(h->stream);
+    if (q == hipErrorNotReady) return IHS_EAGAIN;
+  }
+  const int64_t n = std::min(cap, h->n);
+  launch_pack(h->stream, h->split, h->n, h->st[h->cur], (SphParticle48*)h->sc.staging);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipMemcpy(out, h->sc.staging, n * sizeof(ihs_particle), hipMemcpyDeviceToHost));
+  return n;
+}
+
+int64_t ihs_count(const ihs_handle* h) { return h ? h->n : IHS_ERR_ARG; }
+
+int ihs_sync(ihs_handle* h) {
+  if (!h) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipGetLastError());
+  return IHS_OK;
+}
+
+int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset) {
+  if (!h || !out) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  std::memset(out, 0, sizeof *out);
+  if (h->have_fe) launch_energy(h->stream, h->split, h->n, h->st[h->cur], h->sc.accg, h->sc.epart, h->sc.eout);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  double e[9] = {0};
+  if (h->have_fe) HIP_TRY(h, hipMemcpy(e, h->sc.eout, sizeof e, hipMemcpyDeviceToHost));
+  StepCtl ctl;
+  uint32_t fl[FLAG_COUNT];
+  unsigned long long cn[CNT_COUNT];
+  HIP_TRY(h, hipMemcpy(&ctl, h->sc.ctl, sizeof ctl, hipMemcpyDeviceToHost));
+  HIP_TRY(h, hipMemcpy(fl, h->sc.flags, sizeof fl, hipMemcpyDeviceToHost));
+  HIP_TRY(h, hipMemcpy(cn, h->sc.counters, sizeof cn, hipMemcpyDeviceToHost));
+  out->ke = e[0];
+  out->ie = e[1];
+  out->pe = e[2];
+  for (int q = 0; q < 3; ++q) {
+    out->p[q] = e[3 + q];
+    out->L[q] = e[6 + q];
+  }
+  out->t = ctl.t;
+  out->dt = ctl.dt;
+  out->steps = (int64_t)ctl.steps;
+  out->force_evals = h->force_evals;
+  const double solved = (double)cn[CNT_H_SOLVED];
+  out->mean_h_iters = solved > 0 ? (double)cn[CNT_H_ITERS] / solved : 0.0;
+  out->mean_h_evals = solved > 0 ? (double)cn[CNT_H_EVALS] / solved : 0.0;
+  out->mean_neighbours =
+      h->force_evals > 0 && h->n > 0 ? (double)cn[CNT_PAIRS] / ((double)h->force_evals * (double)h->n) : 0.0;
+  out->nbr_overflow = fl[FLAG_NBR_OVERFLOW];
+  out->walk_overflow = fl[FLAG_WALK_OVERFLOW];
+  out->h_capped = fl[FLAG_H_CAPPED];
+  out->h_unconverged = fl[FLAG_H_UNCONVERGED];
+  out->nonfinite = fl[FLAG_NONFINITE];
+  out->grav_cell_interactions = (int64_t)cn[CNT_GRAV_CELL];
+  out->grav_pp_interactions = (int64_t)cn[CNT_GRAV_PP];
+  out->nbr_candidates = (int64_t)cn[CNT_NBR_CAND];
+  out->walk_nodes = (int64_t)cn[CNT_WALK_NODES];
+  if (reset) {
+    HIP_TRY(h, hipMemset(h->sc.counters, 0, sizeof cn));
+    fl[FLAG_NBR_OVERFLOW] = fl[FLAG_WALK_OVERFLOW] = fl[FLAG_H_CAPPED] = fl[FLAG_H_UNCONVERGED] = fl[FLAG_NONFINITE] = 0;
+    HIP_TRY(h, hipMemcpy(h->sc.flags, fl, sizeof fl, hipMemcpyHostToDevice));
+    h->force_evals = 0;
+  }
+  return IHS_OK;
+}
+
+int ihs_profile(ihs_handle* h, int32_t enable) {
+  if (!h) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  h->prof_on = enable != 0;
+  h->prof_used = 0;
+  return IHS_OK;
+}
+
+int ihs_profile_read(ihs_handle* h, double* ms, int64_t* calls, int32_t reset) {
+  if (!h) return IHS_ERR_ARG;
+  prof_flush(h);
+  for (int i = 0; i < IHS_STAGE_COUNT; ++i) {
+    if (ms) ms[i] = h->prof_ms[i];
+    if (calls) calls[i] = h->prof_calls[i];
+    if (reset) {
+      h->prof_ms[i] = 0.0;
+      h->prof_calls[i] = 0;
+    }
+  }
+  return IHS_OK;
+}
+
+int ihs_gravity(ihs_handle* h, const double* pos3, const double* mass, int64_t n, double softening, double theta,
+                int32_t mode, double* acc3, double* pot) {
+  if (!h) return IHS_ERR_ARG;
+  if (!pos3 || !mass || !acc3 || n < 1) return fail(h, IHS_ERR_ARG, "gravity needs n >= 1 bodies and buffers");
+  if (n > h->cap) return fail(h, IHS_ERR_CAPACITY, "gravity on %lld bodies exceeds capacity", (long long)n);
+  if (mode != IHS_GRAVITY_DIRECT && mode != IHS_GRAVITY_TREE) return fail(h, IHS_ERR_ARG, "bad gravity mode");
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  Scratch& sc = h->sc;
+  if (!sc.gpos) {
+    HIP_TRY(h, dalloc(h, &sc.gpos, 3 * h->cap));
+    HIP_TRY(h, dalloc(h, &sc.gpx, h->cap));
+    HIP_TRY(h, dalloc(h, &sc.gpy, h->cap));
+    HIP_TRY(h, dalloc(h, &sc.gpz, h->cap));
+    HIP_TRY(h, dalloc(h, &sc.gm, h->cap));
+    HIP_TRY(h, dalloc(h, &sc.gmass, h->cap));
+    HIP_TRY(h, dalloc(h, &sc.gacc, 3 * h->cap));
+    HIP_TRY(h, dalloc(h, &sc.gpot, h->cap));
+  }
+  hipStream_t s = h->stream;
+  HIP_TRY(h, hipMemcpyAsync(sc.gpos, pos3, 3 * n * sizeof(double), hipMemcpyHostToDevice, s));
+  HIP_TRY(h, hipMemcpyAsync(sc.gmass, mass, n * sizeof(double), hipMemcpyHostToDevice, s));
+  launch_bodies_in(s, sc.gpos, sc.gmass, n, sc.gpx, sc.gpy, sc.gpz, sc.gm);
+  const float soft2 = (float)(softening * softening);
+  if (mode == IHS_GRAVITY_DIRECT) {
+    launch_gather_xm(s, nullptr, n, sc.gpx, sc.gpy, sc.gpz, sc.gm, sc.xm);
+    launch_gravity_direct(s, n, soft2, sc.xm, sc.accg);
+    launch_acc_out(s, nullptr, n, sc.accg, sc.gacc, sc.gpot);
+  } else {
+    const int K = h->cfg.leaf_size;
+    launch_fe_init(s, sc.flags);
+    launch_bbox_morton(s, 1, sc.gpx, sc.gpy, sc.gpz, n, h->tb);
+    launch_radix_sort(s, n, 30, h->tb);  // identity payload: ties already ordered by index
+    launch_gather_xm(s, h->tb.perm, n, sc.gpx, sc.gpy, sc.gpz, sc.gm, sc.xm);
+    launch_karras(s, n, K, h->tb);
+    launch_upsweep(s, n, K, (float)theta, sc.xm, nullptr, h->tb);
+    launch_groups(s, n, K, h->tb);
+    launch_gravity_walk(s, n, K, 0, soft2, sc.xm, h->tb, sc.accg, sc.flags, sc.counters);
+    launch_acc_out(s, h->tb.perm, n, sc.accg, sc.gacc, sc.gpot);
+    h->have_fe = false;  // the tree buffers now describe these bodies, not the SPH state
+  }
+  HIP_TRY(h, hipMemcpyAsync(acc3, sc.gacc, 3 * n * sizeof(double), hipMemcpyDeviceToHost, s));
+  if (pot) HIP_TRY(h, hipMemcpyAsync(pot, sc.gpot, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIP_TRY(h, hipStreamSynchronize(s));
+  HIP_TRY(h, hipGetLastError());
+  return IHS_OK;
+}
+
+int64_t ihs_debug_tree(ihs_handle* h, uint32_t* sorted_codes, uint32_t* order, uint32_t* ccodes, uint32_t* left,
+                       uint32_t* right, uint32_t* parent) {
+  if (!h) return IHS_ERR_ARG;
+  if (!h->have_fe) return fail(h, IHS_ERR_STATE, "no force evaluation has built a tree for the current state");
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  const int64_t n = h->n, K = h->cfg.leaf_size, L = (n + K - 1) / K;
+  if (sorted_codes) HIP_TRY(h, hipMemcpy(sorted_codes, h->tb.skeys, n * 4, hipMemcpyDeviceToHost));
+  if (order) HIP_TRY(h, hipMemcpy(order, h->st[h->cur].id, n * 4, hipMemcpyDeviceToHost));
+  if (ccodes) HIP_TRY(h, hipMemcpy(ccodes, h->tb.ccode, L * 4, hipMemcpyDeviceToHost));
+  std::vector<int2> ch(L > 1 ? L - 1 : 1);
+  if (L > 1) HIP_TRY(h, hipMemcpy(ch.data(), h->tb.child, (L - 1) * sizeof(int2), hipMemcpyDeviceToHost));
+  for (int64_t i = 0; i < 2 * L - 1; ++i) {
+    if (left) left[i] = i < L - 1 ? (uint32_t)ch[i].x : 0u;
+    if (right) right[i] = i < L - 1 ? (uint32_t)ch[i].y : 0u;
+  }
+  if (parent) HIP_TRY(h, hipMemcpy(parent, h->tb.parent, (2 * L - 1) * 4, hipMemcpyDeviceToHost));
+  return L;
+}
+
+int ihs_debug_field(ihs_handle* h, int32_t field, void* out) {
+  if (!h || !out) return IHS_ERR_ARG;
+  if (h->n < 1) return fail(h, IHS_ERR_STATE, "no particles uploaded");
+  hipSetDevice(h->device);
+  const int64_t n = h->n;
+  hipStream_t s = h->stream;
+  MasterBufs& S = h->st[h->cur];
+  void* tmp = h->sc.staging;  // >= 48 bytes per particle
+  size_t bytes = 0;
+  switch (field) {
+    case IHS_FIELD_H: launch_scatter_by_id(s, n, S.id, S.h, tmp, 4); bytes = 4 * n; break;
+    case IHS_FIELD_RHO: launch_scatter_by_id(s, n, S.id, S.rho, tmp, 4); bytes = 4 * n; break;
+    case IHS_FIELD_RPC: launch_scatter_by_id(s, n, S.id, h->sc.rpc, tmp, 16); bytes = 16 * n; break;
+    case IHS_FIELD_ASPH: launch_scatter_by_id(s, n, S.id, h->sc.asph, tmp, 16); bytes = 16 * n; break;
+    case IHS_FIELD_AGRAV: launch_scatter_by_id(s, n, S.id, h->sc.accg, tmp, 16); bytes = 16 * n; break;
+    case IHS_FIELD_FEAT: launch_scatter_by_id(s, n, S.id, S.feat, tmp, 16); bytes = 16 * n; break;
+    case IHS_FIELD_NCOUNT: launch_scatter_by_id(s, n, S.id, h->sc.ncount, tmp, 4); bytes = 4 * n; break;
+    case IHS_FIELD_HS: launch_scatter_by_id(s, n, S.id, h->sc.hs, tmp, 4); bytes = 4 * n; break;
+    case IHS_FIELD_HITER: {
+      std::vector<uint8_t> it(n);
+      std::vector<uint32_t> id(n);
+      HIP_TRY(h, hipStreamSynchronize(s));
+      HIP_TRY(h, hipMemcpy(it.data(), S.hiter, n, hipMemcpyDeviceToHost));
+      HIP_TRY(h, hipMemcpy(id.data(), S.id, 4 * n, hipMemcpyDeviceToHost));
+      for (int64_t t = 0; t < n; ++t) ((uint32_t*)out)[id[t]] = it[t];
+      return IHS_OK;
+    }
+    case IHS_FIELD_POS64: launch_pos_f64(s, h->split, n, S, (double*)tmp); bytes = 24 * n; break;
+    default: return fail(h, IHS_ERR_ARG, "unknown field %d", field);
+  }
+  HIP_TRY(h, hipStreamSynchronize(s));
+  HIP_TRY(h, hipMemcpy(out, tmp, bytes, hipMemcpyDeviceToHost));
+  return IHS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,331 @@
+// Barnes-Hut self-gravity over the LBVH: a wavefront-coherent group walk (gfx950, wave64).
+//
+// One wave owns a walk group: a subtree of at most 64 consecutive Morton-sorted particles (so the group's
+// box is bounded by its node's Morton cell, never stretched across a jump of the curve). It walks the tree once for the whole
+// group with a short LDS stack, popping up to 64 nodes per step (one per lane). A node is accepted as a
+// monopole when the distance from its centre of mass to the GROUP's bounding box (softened) is at least
+// the node's critical radius |bmax-bmin|/theta + |centre-com| — the reference's robust MAC
+// (ie/shaders/bh_gravity.wgsl:375-386) evaluated at the group's nearest point, so every particle of the
+// group satisfies the per-particle test and the walk is never less accurate than the reference's
+// per-particle walk (bh_gravity.wgsl:337-394). Leaves are never accepted: their particles are summed
+// exactly (bh_gravity.wgsl:356-369). Accepted nodes and opened leaves are staged in LDS and evaluated by
+// all 64 lanes against broadcast sources, so there is no divergence in the force loop.
+#include "common.h"
+#include "launch.h"
+
+namespace {
+
+constexpr int GW_WAVES = 4;
+constexpr int GW_STACK = 1024;
+constexpr int GW_RING = 128;
+
+struct WalkLds {
+  uint32_t stack[GW_STACK];
+  float4 cell[GW_RING];
+  uint32_t leaf[GW_RING];
+  float4 part[64];
+  uint32_t pidx[64];
+};
+
+struct Acc {
+  float a[3], pot;
+  Kahan3 k;
+  float kpot, cpot;
+};
+
+template <bool KAHAN>
+__device__ __forceinline__ void acc_add(Acc& A, float px, float py, float pz, float pp) {
+  if (KAHAN) {
+    A.k.add(px, 0);
+    A.k.add(py, 1);
+    A.k.add(pz, 2);
+    const float y = pp - A.cpot;
+    const float t = A.kpot + y;
+    A.cpot = (t - A.kpot) - y;
+    A.kpot = t;
+  } else {
+    A.a[0] += px;
+    A.a[1] += py;
+    A.a[2] += pz;
+    A.pot += pp;
+  }
+}
+
+__device__ __forceinline__ void interact(const float4 s, const float4 me, float soft2, bool self, float& ax, float& ay,
+                                         float& az, float& pot) {
+  const float dx = s.x - me.x, dy = s.y - me.y, dz = s.z - me.z;
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
+  float rinv = __builtin_amdgcn_rsqf(r2);
+  rinv = self ? 0.f : rinv;
+  const float mr = s.w * rinv;
+  const float mr3 = mr * rinv * rinv;
+  ax = fmaf(dx, mr3, ax);
+  ay = fmaf(dy, mr3, ay);
+  az = fmaf(dz, mr3, az);
+  pot -= mr;
+}
+
+// Walk of one group by one wave. SELF: compare source indices to skip the particle itself (needed only
+// without softening; with softening the self term contributes zero force and a known potential -m/eps that
+// is removed at the end).
+template <bool KAHAN, bool SELF>
+__device__ __forceinline__ void walk_group(WalkLds& S, const int4 grp, uint32_t lane, const float4* __restrict__ xm,
+                                           int64_t n, int K, int64_t L, const int2* __restrict__ child,
+                                           const float4* __restrict__ ncom, const float4* __restrict__ nbmax,
+                                           float soft2, float4* __restrict__ accg, uint32_t* __restrict__ flags,
+                                           uint64_t& n_cell, uint64_t& n_pp, uint64_t& n_nodes) {
+  const int64_t t0 = grp.x;
+  const int64_t t = t0 + lane;
+  const bool valid = lane < (uint32_t)grp.y;
+  const float4 me = xm[valid ? t : t0];
+  // The group's box (invalid lanes repeat the first particle).
+  const float bx0 = wave_min(me.x), by0 = wave_min(me.y), bz0 = wave_min(me.z);
+  const float bx1 = wave_max(me.x), by1 = wave_max(me.y), bz1 = wave_max(me.z);
+  const uint64_t nvalid = (uint64_t)__popcll(__ballot(valid));
+
+  Acc A;
+  A.a[0] = A.a[1] = A.a[2] = A.pot = 0.f;
+  A.k.zero();
+  A.kpot = A.cpot = 0.f;
+
+  const uint32_t leaf0 = (uint32_t)(L - 1);
+  const uint32_t lpc = 64u / (uint32_t)K;
+  uint32_t sp = 1, chead = 0, ctail = 0, lhead = 0, ltail = 0;
+  wave_sync();
+  if (lane == 0) S.stack[0] = 0u;
+  wave_sync();
+
+  // Accepted nodes: chead stays a multiple of 64 until the final flush, so a chunk is contiguous in the
+  // 128-entry ring; cnt is a multiple of 4 (the tail is padded with massless entries).
+  auto eval_cells = [&](uint32_t cnt, uint32_t real) {
+    const float4* src = &S.cell[chead & (GW_RING - 1)];
+    float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
+    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e) interact(src[k0 + e], me, soft2, false, ax, ay, az, pp);
+    }
+    acc_add<KAHAN>(A, ax, ay, az, pp);
+    chead += cnt;
+    n_cell += nvalid * real;
+  };
+  auto eval_leaves = [&](uint32_t nl) {
+    const uint32_t li = lane / (uint32_t)K;
+    const bool have = li < nl;
+    const uint32_t leaf = have ? S.leaf[(lhead + li) & (GW_RING - 1)] : 0u;
+    const int64_t src = (int64_t)leaf * K + (lane % (uint32_t)K);
+    const bool ok = have && src < n;
+    wave_sync();
+    S.part[lane] = ok ? xm[src] : make_float4(0.f, 0.f, 0.f, 0.f);
+    if (SELF) S.pidx[lane] = ok ? (uint32_t)src : IHS_NONE;
+    wave_sync();
+    float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
+    const uint32_t cnt = (min(64u, nl * (uint32_t)K) + 3u) & ~3u;  // padding slots are massless
+    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e)
+        interact(S.part[k0 + e], me, soft2, SELF && S.pidx[k0 + e] == (uint32_t)t, ax, ay, az, pp);
+    }
+    acc_add<KAHAN>(A, ax, ay, az, pp);
+    lhead += nl;
+    n_pp += nvalid * (uint64_t)__popcll(__ballot(ok));
+    wave_sync();
+  };
+
+  while (sp > 0) {
+    const uint32_t take = min(sp, 64u);
+    const uint32_t base = sp - take;
+    const uint32_t node = lane < take ? S.stack[base + lane] : IHS_NONE;
+    sp = base;
+    n_nodes += take;
+    wave_sync();
+    // Every node, leaves included, is tested with the MAC (the reference always sums leaves exactly,
+    // bh_gravity.wgsl:356-369; with buckets of K particles a far leaf is the same monopole test as any node).
+    const bool live = node != IHS_NONE;
+    bool acc = false;
+    float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (live) {
+      c = ncom[node];
+      const float rc = nbmax[node].w;
+      const float dx = fmaxf(fmaxf(bx0 - c.x, c.x - bx1), 0.f);
+      const float dy = fmaxf(fmaxf(by0 - c.y, c.y - by1), 0.f);
+      const float dz = fmaxf(fmaxf(bz0 - c.z, c.z - bz1), 0.f);
+      const float d2 = dx * dx + dy * dy + dz * dz + soft2;
+      acc = d2 >= rc * rc;
+    }
+    const bool isleaf = live && !acc && node >= leaf0;
+    const bool open = live && !acc && node < leaf0;
+    const uint64_t ma = __ballot(acc), mo = __ballot(open), ml = __ballot(isleaf);
+    if (acc) S.cell[(ctail + lanes_below(ma)) & (GW_RING - 1)] = c;
+    ctail += (uint32_t)__popcll(ma);
+    if (isleaf) S.leaf[(ltail + lanes_below(ml)) & (GW_RING - 1)] = node - leaf0;
+    ltail += (uint32_t)__popcll(ml);
+    const uint32_t nopen = (uint32_t)__popcll(mo);
+    if (sp + 2u * nopen > (uint32_t)GW_STACK) {
+      if (lane == 0) atomicAdd(&flags[FLAG_WALK_OVERFLOW], 1u);
+    } else {
+      if (open) {
+        const int2 ch = child[node];
+        const uint32_t pos = sp + 2u * lanes_below(mo);
+        S.stack[pos] = (uint32_t)ch.x;
+        S.stack[pos + 1] = (uint32_t)ch.y;
+      }
+      sp += 2u * nopen;
+    }
+    wave_sync();
+    while (ctail - chead >= 64u) eval_cells(64u, 64u);
+    while (ltail - lhead >= lpc) eval_leaves(lpc);
+  }
+  if (ctail != chead) {
+    const uint32_t real = ctail - chead;
+    const uint32_t pad = (4u - (real & 3u)) & 3u;
+    if (lane < pad) S.cell[(ctail + lane) & (GW_RING - 1)] = make_float4(0.f, 0.f, 0.f, 0.f);
+    wave_sync();
+    ctail += pad;
+    eval_cells(ctail - chead, real);
+  }
+  while (ltail != lhead) eval_leaves(min(lpc, ltail - lhead));
+
+  if (valid) {
+    float a0, a1, a2, p;
+    if (KAHAN) {
+      a0 = A.k.s[0]; a1 = A.k.s[1]; a2 = A.k.s[2]; p = A.kpot;
+    } else {
+      a0 = A.a[0]; a1 = A.a[1]; a2 = A.a[2]; p = A.pot;
+    }
+    if (!SELF) p += me.w * __builtin_amdgcn_rsqf(soft2);  // remove the self term -m/eps
+    accg[t] = make_float4(kG32 * a0, kG32 * a1, kG32 * a2, kG32 * p);
+  }
+}
+
+// Persistent walk: every wave pulls groups from its XCD's queue (a contiguous eighth of the groups, so that
+// XCD's L2 serves neighbouring groups), then helps the other queues; irregular groups therefore overlap
+// instead of leaving the machine idle behind the slowest one.
+template <bool KAHAN, bool SELF>
+__global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __restrict__ xm, int64_t n, int K,
+                                                                 int64_t L, const int2* __restrict__ child,
+                                                                 const float4* __restrict__ ncom,
+                                                                 const float4* __restrict__ nbmax,
+                                                                 const int4* __restrict__ groups,
+                                                                 const uint32_t* __restrict__ ngroups,
+                                                                 uint32_t* __restrict__ wq, float soft2,
+                                                                 float4* __restrict__ accg, uint32_t* __restrict__ flags,
+                                                                 unsigned long long* __restrict__ counters) {
+  __shared__ WalkLds lds[GW_WAVES];
+  const int w = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  WalkLds& S = lds[w];
+  const uint32_t ng = *ngroups;
+  const uint32_t home = blockIdx.x & 7u;
+  uint64_t n_cell = 0, n_pp = 0, n_nodes = 0;
+  for (uint32_t r = 0; r < 8; ++r) {
+    const uint32_t q = (home + r) & 7u;
+    const uint32_t lo = (uint32_t)((uint64_t)ng * q / 8u), hi = (uint32_t)((uint64_t)ng * (q + 1u) / 8u);
+    for (;;) {
+      uint32_t g = 0;
+      if (lane == 0) g = atomicAdd(&wq[q], 1u);
+      g = (uint32_t)__builtin_amdgcn_readfirstlane((int)g) + lo;
+      if (g >= hi) break;
+      walk_group<KAHAN, SELF>(S, groups[g], lane, xm, n, K, L, child, ncom, nbmax, soft2, accg, flags, n_cell, n_pp,
+                              n_nodes);
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(&counters[CNT_GRAV_CELL], n_cell);
+    atomicAdd(&counters[CNT_GRAV_PP], n_pp);
+    atomicAdd(&counters[CNT_WALK_NODES], n_nodes);
+  }
+}
+
+// Tiled exact FP32 sum: every thread one target, sources staged through LDS 256 at a time.
+__global__ __launch_bounds__(256) void k_gravity_direct(const float4* __restrict__ xm, int64_t n, float soft2,
+                                                        float4* __restrict__ acc4) {
+  __shared__ float4 tile[256];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const float4 me = xm[i < n ? i : 0];
+  float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
+  for (int64_t j0 = 0; j0 < n; j0 += 256) {
+    const int64_t j = j0 + threadIdx.x;
+    __syncthreads();
+    tile[threadIdx.x] = j < n ? xm[j] : make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    const int cnt = (int)min<int64_t>(256, n - j0);
+    for (int k = 0; k < cnt; ++k) interact(tile[k], me, soft2, j0 + k == i, ax, ay, az, pp);
+  }
+  if (i < n) acc4[i] = make_float4(kG32 * ax, kG32 * ay, kG32 * az, kG32 * pp);
+}
+
+// Standalone gravity (the GravityField backend): AoS doubles in, SoA out.
+__global__ __launch_bounds__(256) void k_bodies_in(const double* __restrict__ pos3, const double* __restrict__ mass,
+                                                   int64_t n, double* __restrict__ px, double* __restrict__ py,
+                                                   double* __restrict__ pz, float* __restrict__ m) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  px[i] = pos3[3 * i];
+  py[i] = pos3[3 * i + 1];
+  pz[i] = pos3[3 * i + 2];
+  m[i] = (float)mass[i];
+}
+
+__global__ __launch_bounds__(256) void k_gather_xm(const uint32_t* __restrict__ perm, int64_t n,
+                                                   const double* __restrict__ px, const double* __restrict__ py,
+                                                   const double* __restrict__ pz, const float* __restrict__ m,
+                                                   float4* __restrict__ xm) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t s = perm ? perm[t] : (uint32_t)t;
+  xm[t] = make_float4((float)px[s], (float)py[s], (float)pz[s], m[s]);
+}
+
+__global__ __launch_bounds__(256) void k_acc_out(const uint32_t* __restrict__ perm, int64_t n,
+                                                 const float4* __restrict__ acc4, double* __restrict__ acc3,
+                                                 double* __restrict__ pot) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t >= n) return;
+  const int64_t i = perm ? (int64_t)perm[t] : t;
+  const float4 a = acc4[t];
+  acc3[3 * i] = a.x;
+  acc3[3 * i + 1] = a.y;
+  acc3[3 * i + 2] = a.z;
+  if (pot) pot[i] = a.w;
+}
+
+}  // namespace
+
+void launch_bodies_in(hipStream_t s, const double* pos3, const double* mass, int64_t n, double* px, double* py,
+                      double* pz, float* m) {
+  k_bodies_in<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(pos3, mass, n, px, py, pz, m);
+}
+void launch_gather_xm(hipStream_t s, const uint32_t* perm, int64_t n, const double* px, const double* py,
+                      const double* pz, const float* m, float4* xm) {
+  k_gather_xm<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(perm, n, px, py, pz, m, xm);
+}
+void launch_acc_out(hipStream_t s, const uint32_t* perm, int64_t n, const float4* acc4, double* acc3, double* pot) {
+  k_acc_out<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(perm, n, acc4, acc3, pot);
+}
+
+void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, float soft2, const float4* xm, const TreeBufs& tb,
+                         float4* accg, uint32_t* flags, unsigned long long* counters) {
+  const int64_t L = (n + K - 1) / K;
+  static unsigned nblk[4] = {0, 0, 0, 0};
+  const bool self = !(soft2 > 0.f);
+  const int variant = (kahan ? 2 : 0) + (self ? 1 : 0);
+#define IHS_WALK(KA, SE)                                                                                          \
+  do {                                                                                                            \
+    if (!nblk[variant]) nblk[variant] = persistent_blocks((const void*)k_gravity_walk<KA, SE>, GW_WAVES * 64);    \
+    k_gravity_walk<KA, SE><<<nblk[variant], GW_WAVES * 64, 0, s>>>(xm, n, K, L, tb.child, tb.ncom, tb.nbmax,     \
+                                                                  tb.groups, tb.ngroups, tb.wq, soft2, accg,   \
+                                                                  flags, counters);                            \
+  } while (0)
+  if (kahan) {
+    if (self) IHS_WALK(true, true);
+    else IHS_WALK(true, false);
+  } else {
+    if (self) IHS_WALK(false, true);
+    else IHS_WALK(false, false);
+  }
+#undef IHS_WALK
+}
+
+void launch_gravity_direct(hipStream_t s, int64_t n, float soft2, const float4* xm, float4* acc4) {
+  k_gravity_direct<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(xm, n, soft2, acc4);
+}

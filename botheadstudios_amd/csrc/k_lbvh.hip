@@ -1,0 +1,568 @@
+// Morton keys, LDS-staged stable radix sort, Karras LBVH build and ordered multipole upsweep (gfx950).
+//
+// Bit-exactness contract with the reference's CPU Morton leg (ie/crates/engine/src/gpu_gravity.rs:334-368):
+//   - bbox: exact float min/max through order-preserving u32 keys (ie/shaders/bh_gravity.wgsl:49-78);
+//   - Morton: IEEE f32 (p-lo)/ext, *1024, floor, clamp; expand_bits (bh_gravity.wgsl:82-98);
+//   - order: lexicographic (code, particle id) (gpu_gravity.rs:358-363): stable LSD radix sort, then
+//     runs of equal codes reordered by id;
+//   - Karras topology: determineRange/findSplit with the index tie-break (bh_gravity.wgsl:114-184).
+// This file is compiled without fast-math so division and floor round exactly as IEEE requires.
+#include "common.h"
+#include "launch.h"
+
+namespace {
+
+__device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
+  v &= 0x000003ffu;
+  v = (v * 0x00010001u) & 0xff0000ffu;
+  v = (v * 0x00000101u) & 0x0f00f00fu;
+  v = (v * 0x00000011u) & 0xc30c30c3u;
+  v = (v * 0x00000005u) & 0x49249249u;
+  return v;
+}
+
+__global__ void k_bbox_init(uint32_t* bbox) {
+  const int i = threadIdx.x;
+  if (i < 3) bbox[i] = 0xffffffffu;
+  else if (i < 6) bbox[i] = 0u;
+}
+
+template <class PT>
+__global__ __launch_bounds__(256) void k_bbox(const PT* __restrict__ px, const PT* __restrict__ py,
+                                              const PT* __restrict__ pz, int64_t n, uint32_t* __restrict__ bbox) {
+  uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t k[3] = {f2key((float)px[i]), f2key((float)py[i]), f2key((float)pz[i])};
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      lo[q] = min(lo[q], k[q]);
+      hi[q] = max(hi[q], k[q]);
+    }
+  }
+  __shared__ uint32_t s[4][6];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    lo[q] = wave_min(lo[q]);
+    hi[q] = wave_max(hi[q]);
+  }
+  if (lane == 0)
+    for (int q = 0; q < 3; ++q) {
+      s[w][q] = lo[q];
+      s[w][3 + q] = hi[q];
+    }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int q = threadIdx.x;
+    uint32_t v = s[0][q];
+    for (int ww = 1; ww < (int)(blockDim.x >> 6); ++ww) v = q < 3 ? min(v, s[ww][q]) : max(v, s[ww][q]);
+    if (q < 3) atomicMin(&bbox[q], v);
+    else atomicMax(&bbox[q], v);
+  }
+}
+
+template <class PT>
+__global__ __launch_bounds__(256) void k_morton(const PT* __restrict__ px, const PT* __restrict__ py,
+                                                const PT* __restrict__ pz, int64_t n, const uint32_t* __restrict__ bbox,
+                                                uint32_t* __restrict__ codes, uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float lo[3], ext[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    lo[q] = key2f(bbox[q]);
+    ext[q] = fmaxf(key2f(bbox[3 + q]) - lo[q], 1.0e-30f);
+  }
+  const float p[3] = {(float)px[i], (float)py[i], (float)pz[i]};
+  uint32_t qq[3];
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    float u = __fdiv_rn(__fsub_rn(p[q], lo[q]), ext[q]);
+    u = fminf(fmaxf(u, 0.0f), 1.0f);
+    float f = floorf(__fmul_rn(u, 1024.0f));
+    f = fminf(fmaxf(f, 0.0f), 1023.0f);
+    qq[q] = (uint32_t)f;
+  }
+  codes[i] = expand_bits(qq[0]) * 4u + expand_bits(qq[1]) * 2u + expand_bits(qq[2]);
+  vals[i] = (uint32_t)i;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Radix sort, 8-bit digits. Tile t covers keys [t*TILE, (t+1)*TILE); item r of thread x is
+// t*TILE + r*256 + x, so (r, wave, lane) order is the key order inside the tile (stability).
+__global__ __launch_bounds__(RADIX_THREADS) void k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
+                                                              uint32_t* __restrict__ hist, int nb) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RADIX_TILE;
+#pragma unroll
+  for (int r = 0; r < RADIX_ITEMS; ++r) {
+    const int64_t i = base + r * RADIX_THREADS + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+  }
+  __syncthreads();
+  hist[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+// Exclusive scan of 256 values held one per thread (block of 256); returns this thread's prefix.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_tmp /*[4]*/, uint32_t* total) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_tmp[w] = x;
+  __syncthreads();
+  uint32_t wbase = 0, tot = 0;
+#pragma unroll
+  for (int ww = 0; ww < 4; ++ww) {
+    const uint32_t t = s_tmp[ww];
+    if (ww < w) wbase += t;
+    tot += t;
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return wbase + x - v;
+}
+
+// One block per digit: exclusive scan over the nb tile counts of that digit; dtot[d] = digit total.
+__global__ __launch_bounds__(256) void k_radix_scan(uint32_t* __restrict__ hist, int nb, uint32_t* __restrict__ dtot) {
+  __shared__ uint32_t s_tmp[4];
+  const int d = blockIdx.x;
+  uint32_t carry = 0;
+  for (int c0 = 0; c0 < nb; c0 += 256) {
+    const int c = c0 + threadIdx.x;
+    const uint32_t v = c < nb ? hist[(int64_t)d * nb + c] : 0u;
+    uint32_t tot;
+    const uint32_t ex = block_excl_scan_256(v, s_tmp, &tot);
+    if (c < nb) hist[(int64_t)d * nb + c] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) dtot[d] = carry;
+}
+
+__global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                                 const uint32_t* __restrict__ vin,
+                                                                 uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                                 int64_t n, int shift, const uint32_t* __restrict__ hist,
+                                                                 const uint32_t* __restrict__ dtot, int nb) {
+  __shared__ uint32_t s_base[256];
+  __shared__ uint32_t s_wc[4][256];
+  __shared__ uint32_t s_tmp[4];
+  const int tid = threadIdx.x, w = tid >> 6;
+  {
+    const uint32_t dbase = block_excl_scan_256(dtot[tid], s_tmp, nullptr);
+    s_base[tid] = dbase + hist[(int64_t)tid * nb + blockIdx.x];
+    for (int ww = 0; ww < 4; ++ww) s_wc[ww][tid] = 0;
+  }
+  __syncthreads();
+  const int64_t base = (int64_t)blockIdx.x * RADIX_TILE;
+  for (int r = 0; r < RADIX_ITEMS; ++r) {
+    const int64_t i = base + r * RADIX_THREADS + tid;
+    const bool valid = i < n;
+    const uint32_t k = valid ? kin[i] : 0u;
+    const uint32_t v = valid ? vin[i] : 0u;
+    const uint32_t d = (k >> shift) & 255u;
+    // Lanes of this wave holding the same digit.
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; ++b) {
+      const uint64_t bb = __ballot((d >> b) & 1u);
+      peers &= ((d >> b) & 1u) ? bb : ~bb;
+    }
+    const uint32_t rank = lanes_below(peers);
+    if (valid && rank == 0) s_wc[w][d] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t pos = s_base[d] + rank;
+      for (int ww = 0; ww < w; ++ww) pos += s_wc[ww][d];
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+    {
+      uint32_t add = 0;
+#pragma unroll
+      for (int ww = 0; ww < 4; ++ww) {
+        add += s_wc[ww][tid];
+        s_wc[ww][tid] = 0;
+      }
+      s_base[tid] += add;
+    }
+    __syncthreads();
+  }
+}
+
+// Runs of equal codes are ordered by storage position after the stable sort; reorder each run by
+// the particle id so the final order is (code, id). Runs are short (a handful) in practice.
+__global__ __launch_bounds__(256) void k_fix_ties(const uint32_t* __restrict__ skeys, uint32_t* __restrict__ perm,
+                                                  const uint32_t* __restrict__ ids, int64_t n) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t c = skeys[t];
+  if (t > 0 && skeys[t - 1] == c) return;          // not a run start
+  if (t + 1 >= n || skeys[t + 1] != c) return;     // run of one
+  int64_t e = t + 1;
+  while (e < n && skeys[e] == c) ++e;
+  for (int64_t a = t + 1; a < e; ++a) {
+    const uint32_t pv = perm[a];
+    const uint32_t idv = ids[pv];
+    int64_t b = a - 1;
+    while (b >= t && ids[perm[b]] > idv) {
+      perm[b + 1] = perm[b];
+      --b;
+    }
+    perm[b + 1] = pv;
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Karras 2012 over the L leaf cluster codes (first code of each bucket of K sorted particles).
+__global__ __launch_bounds__(256) void k_leaf_codes(const uint32_t* __restrict__ skeys, int64_t L, int K,
+                                                    uint32_t* __restrict__ ccode, uint32_t* __restrict__ parent,
+                                                    uint32_t* __restrict__ gstart) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= L) return;
+  ccode[c] = skeys[c * K];
+  gstart[c] = IHS_NONE;
+  if (c == 0) parent[0] = IHS_NONE;  // the root (an internal node, or the single leaf when L == 1)
+}
+
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ cc, int64_t L, int64_t i, int64_t j) {
+  if (j < 0 || j >= L) return -1;
+  const uint32_t a = cc[i], b = cc[j];
+  if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+  return __clz(a ^ b);
+}
+
+__global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cc, int64_t L, int2* __restrict__ child,
+                                                int2* __restrict__ range, uint32_t* __restrict__ parent,
+                                                uint32_t* __restrict__ ready) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= L - 1) return;
+  const int64_t d = kdelta(cc, L, i, i + 1) > kdelta(cc, L, i, i - 1) ? 1 : -1;
+  const int dmin = kdelta(cc, L, i, i - d);
+  int64_t lmax = 2;
+  while (kdelta(cc, L, i, i + lmax * d) > dmin) lmax *= 2;
+  int64_t l = 0;
+  for (int64_t t = lmax / 2; t >= 1; t /= 2)
+    if (kdelta(cc, L, i, i + (l + t) * d) > dmin) l += t;
+  const int64_t j = i + l * d;
+  const int64_t first = i < j ? i : j, last = i < j ? j : i;
+  int64_t split = first;
+  const uint32_t fc = cc[first], lc = cc[last];
+  if (fc == lc) {
+    // All codes in [first, last] are equal: split where the index-augmented key (code, index) flips its
+    // highest differing index bit, the split that is consistent with the ranges kdelta() assigns to the
+    // children. The reference's (first + last) >> 1 (bh_gravity.wgsl:159-160) agrees for runs of two but
+    // yields a tree with shared or orphaned nodes for longer runs of equal cluster codes.
+    const int p = 31 - __clz((uint32_t)first ^ (uint32_t)last);
+    split = (int64_t)((((uint32_t)first >> p) + 1u) << p) - 1;
+  } else {
+    const int cp = __clz(fc ^ lc);
+    int64_t step = last - first;
+    do {
+      step = (step + 1) >> 1;
+      const int64_t ns = split + step;
+      if (ns < last && __clz(fc ^ cc[ns]) > cp) split = ns;
+    } while (step > 1);
+  }
+  const uint32_t li = split == first ? (uint32_t)(L - 1 + split) : (uint32_t)split;
+  const uint32_t ri = split + 1 == last ? (uint32_t)(L - 1 + split + 1) : (uint32_t)(split + 1);
+  child[i] = make_int2((int)li, (int)ri);
+  range[i] = make_int2((int)first, (int)last);
+  parent[li] = (uint32_t)i;
+  parent[ri] = (uint32_t)i;
+  ready[i] = 0u;
+}
+
+// ---------------------------------------------------------------------------------------------
+// Upsweep. Phase 1: one thread per leaf sums its bucket, then climbs; a node is merged by the second
+// child to arrive (arrival counter with workgroup-scope acquire/release) as long as the node's leaf
+// range lies inside this block's leaves. Phase 2 (one workgroup): the remaining top nodes.
+constexpr int UPS_LEAVES = 1024;
+
+__device__ __forceinline__ void node_merge(uint32_t p, const int2* __restrict__ child, float4* ncom, float4* nbmin,
+                                           float4* nbmax, float inv_theta) {
+  const int2 ch = child[p];
+  const float4 cl = ncom[ch.x], cr = ncom[ch.y];
+  const float4 bl0 = nbmin[ch.x], br0 = nbmin[ch.y];
+  const float4 bl1 = nbmax[ch.x], br1 = nbmax[ch.y];
+  const float m = cl.w + cr.w;
+  float4 c = make_float4(0.f, 0.f, 0.f, m);
+  if (m > 0.f) {
+    const float im = 1.0f / m;
+    c.x = (cl.x * cl.w + cr.x * cr.w) * im;
+    c.y = (cl.y * cl.w + cr.y * cr.w) * im;
+    c.z = (cl.z * cl.w + cr.z * cr.w) * im;
+  }
+  const float4 b0 = make_float4(fminf(bl0.x, br0.x), fminf(bl0.y, br0.y), fminf(bl0.z, br0.z), fmaxf(bl0.w, br0.w));
+  float4 b1 = make_float4(fmaxf(bl1.x, br1.x), fmaxf(bl1.y, br1.y), fmaxf(bl1.z, br1.z), 0.f);
+  const float sx = b1.x - b0.x, sy = b1.y - b0.y, sz = b1.z - b0.z;
+  const float ex = 0.5f * (b0.x + b1.x) - c.x, ey = 0.5f * (b0.y + b1.y) - c.y, ez = 0.5f * (b0.z + b1.z) - c.z;
+  b1.w = sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta + sqrtf(ex * ex + ey * ey + ez * ez);
+  ncom[p] = c;
+  nbmin[p] = b0;
+  nbmax[p] = b1;
+}
+
+__global__ __launch_bounds__(UPS_LEAVES) void k_upsweep_local(const float4* __restrict__ xm, const float* __restrict__ hs,
+                                                              int64_t n, int K, int64_t L, const int2* __restrict__ child,
+                                                              const int2* __restrict__ range,
+                                                              const uint32_t* __restrict__ parent, uint32_t* ready,
+                                                              float4* ncom, float4* nbmin, float4* nbmax, float inv_theta) {
+  const int64_t c = (int64_t)blockIdx.x * UPS_LEAVES + threadIdx.x;
+  if (c >= L) return;
+  const int64_t blo = (int64_t)blockIdx.x * UPS_LEAVES, bhi = blo + UPS_LEAVES;
+  const int64_t start = c * K, end = min(start + (int64_t)K, n);
+  float m = 0.f, mx = 0.f, my = 0.f, mz = 0.f, hm = 0.f;
+  float lo[3] = {1.0e30f, 1.0e30f, 1.0e30f}, hi[3] = {-1.0e30f, -1.0e30f, -1.0e30f};
+  for (int64_t s = start; s < end; ++s) {
+    const float4 b = xm[s];
+    m += b.w;
+    mx += b.x * b.w;
+    my += b.y * b.w;
+    mz += b.z * b.w;
+    lo[0] = fminf(lo[0], b.x); lo[1] = fminf(lo[1], b.y); lo[2] = fminf(lo[2], b.z);
+    hi[0] = fmaxf(hi[0], b.x); hi[1] = fmaxf(hi[1], b.y); hi[2] = fmaxf(hi[2], b.z);
+    if (hs) hm = fmaxf(hm, hs[s]);
+  }
+  float4 cm = make_float4(0.f, 0.f, 0.f, m);
+  if (m > 0.f) {
+    cm.x = mx / m;
+    cm.y = my / m;
+    cm.z = mz / m;
+  }
+  const uint32_t li = (uint32_t)(L - 1 + c);
+  ncom[li] = cm;
+  nbmin[li] = make_float4(lo[0], lo[1], lo[2], hm);
+  {
+    const float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
+    const float ex = 0.5f * (lo[0] + hi[0]) - cm.x, ey = 0.5f * (lo[1] + hi[1]) - cm.y, ez = 0.5f * (lo[2] + hi[2]) - cm.z;
+    nbmax[li] = make_float4(hi[0], hi[1], hi[2],
+                            sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta + sqrtf(ex * ex + ey * ey + ez * ez));
+  }
+  uint32_t p = parent[li];
+  while (p != IHS_NONE) {
+    const int2 rg = range[p];
+    if (rg.x < blo || rg.y >= bhi) break;  // top node: phase 2
+    const uint32_t old = __hip_atomic_fetch_add(&ready[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0u) break;
+    node_merge(p, child, ncom, nbmin, nbmax, inv_theta);
+    p = parent[p];
+  }
+}
+
+__device__ __forceinline__ bool is_top(const int2* __restrict__ range, int64_t L, uint32_t node) {
+  if (node >= (uint32_t)(L - 1)) return false;  // leaves are complete after phase 1
+  const int2 rg = range[node];
+  return (rg.x / UPS_LEAVES) != (rg.y / UPS_LEAVES);
+}
+
+__global__ __launch_bounds__(1024) void k_upsweep_top(int64_t L, const int2* __restrict__ child,
+                                                      const int2* __restrict__ range, const uint32_t* __restrict__ parent,
+                                                      uint32_t* ready, float4* ncom, float4* nbmin, float4* nbmax,
+                                                      float inv_theta) {
+  for (int64_t p = threadIdx.x; p < L - 1; p += blockDim.x) {
+    if (!is_top(range, L, (uint32_t)p)) continue;
+    const int2 ch = child[p];
+    const uint32_t kids[2] = {(uint32_t)ch.x, (uint32_t)ch.y};
+    for (int k = 0; k < 2; ++k) {
+      if (is_top(range, L, kids[k])) continue;  // arrives through its own climb
+      uint32_t q = (uint32_t)p;
+      while (q != IHS_NONE) {
+        const uint32_t old = __hip_atomic_fetch_add(&ready[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0u) break;
+        node_merge(q, child, ncom, nbmin, nbmax, inv_theta);
+        q = parent[q];
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Walk groups: node x is a group root when it holds <= GROUP_MAX particles and its parent holds more.
+// Along any leaf-to-root path the particle count grows, so the roots partition the leaves.
+__device__ __forceinline__ int64_t node_count(const int2* __restrict__ range, int64_t L, int K, int64_t n, uint32_t x,
+                                              int64_t& first) {
+  int64_t last;
+  if (x >= (uint32_t)(L - 1)) {
+    first = last = (int64_t)x - (L - 1);
+  } else {
+    const int2 rg = range[x];
+    first = rg.x;
+    last = rg.y;
+  }
+  return min((last + 1) * K, n) - first * K;
+}
+
+__device__ __forceinline__ bool group_ok(const int2* __restrict__ range, int64_t L, int K, int64_t n, uint32_t x,
+                                         const float4* __restrict__ nbmin, const float4* __restrict__ nbmax) {
+  int64_t first;
+  if (node_count(range, L, K, n, x, first) > GROUP_MAX) return false;
+  const float4 b0 = nbmin[x], b1 = nbmax[x];
+  if (!(b0.w > 0.f)) return true;
+  const float dx = b1.x - b0.x, dy = b1.y - b0.y, dz = b1.z - b0.z;
+  return dx * dx + dy * dy + dz * dz <= (GROUP_DIAG_HS * b0.w) * (GROUP_DIAG_HS * b0.w);
+}
+
+// One thread per leaf: the topmost acceptable node on the leaf's root path is its group; the group's first
+// leaf records it. A leaf with no acceptable node is split into single-particle groups.
+__global__ __launch_bounds__(256) void k_mark_groups(int64_t n, int K, int64_t L, const int2* __restrict__ range,
+                                                     const uint32_t* __restrict__ parent,
+                                                     const float4* __restrict__ nbmin,
+                                                     const float4* __restrict__ nbmax,
+                                                     uint32_t* __restrict__ gstart) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= L) return;
+  uint32_t best = IHS_NONE;
+  for (uint32_t x = (uint32_t)(L - 1 + c); x != IHS_NONE; x = parent[x]) {
+    int64_t first;
+    if (node_count(range, L, K, n, x, first) > GROUP_MAX) break;  // counts only grow towards the root
+    if (group_ok(range, L, K, n, x, nbmin, nbmax)) best = x;
+  }
+  if (best == IHS_NONE) {
+    gstart[c] = GROUP_SPLIT;
+    return;
+  }
+  int64_t first;
+  node_count(range, L, K, n, best, first);
+  gstart[c] = first == c ? best : IHS_NONE;
+}
+
+constexpr int GRP_BLK = 1024;
+
+__device__ __forceinline__ uint32_t group_weight(const uint32_t* __restrict__ gstart, int64_t L, int K, int64_t n,
+                                                 int64_t c) {
+  if (c >= L) return 0u;
+  const uint32_t gs = gstart[c];
+  if (gs == IHS_NONE) return 0u;
+  if (gs == GROUP_SPLIT) return (uint32_t)(min((c + 1) * K, n) - c * K);
+  return 1u;
+}
+
+// Block-wide (1024) exclusive scan of per-leaf group counts.
+__device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t* s_w /*16*/, uint32_t* total) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_w[w] = x;
+  __syncthreads();
+  uint32_t base = 0, tot = 0;
+  for (int ww = 0; ww < GRP_BLK / 64; ++ww) {
+    const uint32_t t = s_w[ww];
+    if (ww < w) base += t;
+    tot += t;
+  }
+  if (total) *total = tot;
+  return base + x - v;
+}
+
+__global__ __launch_bounds__(GRP_BLK) void k_group_count(const uint32_t* __restrict__ gstart, int64_t L, int K,
+                                                         int64_t n, uint32_t* __restrict__ gblk) {
+  __shared__ uint32_t s_w[GRP_BLK / 64];
+  const int64_t c = (int64_t)blockIdx.x * GRP_BLK + threadIdx.x;
+  uint32_t tot;
+  block_excl_scan_1024(group_weight(gstart, L, K, n, c), s_w, &tot);
+  if (threadIdx.x == 0) gblk[blockIdx.x] = tot;
+}
+
+__global__ void k_group_offsets(uint32_t* __restrict__ gblk, int nblk, uint32_t* __restrict__ ngroups,
+                                uint32_t* __restrict__ wq) {
+  if (threadIdx.x < 16) wq[threadIdx.x] = 0u;
+  if (threadIdx.x != 0) return;
+  uint32_t run = 0;
+  for (int b = 0; b < nblk; ++b) {
+    const uint32_t c = gblk[b];
+    gblk[b] = run;
+    run += c;
+  }
+  *ngroups = run;
+}
+
+__global__ __launch_bounds__(GRP_BLK) void k_group_emit(const uint32_t* __restrict__ gstart, int64_t L, int K, int64_t n,
+                                                        const int2* __restrict__ range,
+                                                        const uint32_t* __restrict__ gblk, int4* __restrict__ groups) {
+  __shared__ uint32_t s_w[GRP_BLK / 64];
+  const int64_t c = (int64_t)blockIdx.x * GRP_BLK + threadIdx.x;
+  const uint32_t wgt = group_weight(gstart, L, K, n, c);
+  const uint32_t idx = gblk[blockIdx.x] + block_excl_scan_1024(wgt, s_w, nullptr);
+  if (wgt == 0) return;
+  const uint32_t gs = gstart[c];
+  if (gs == GROUP_SPLIT) {
+    for (uint32_t q = 0; q < wgt; ++q) groups[idx + q] = make_int4((int)(c * K + q), 1, (int)(L - 1 + c), 0);
+    return;
+  }
+  int64_t first;
+  const int64_t cnt = node_count(range, L, K, n, gs, first);
+  groups[idx] = make_int4((int)(first * K), (int)cnt, (int)gs, 0);
+}
+
+inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace
+
+void launch_bbox_morton(hipStream_t s, int split, const void* px, const void* py, const void* pz, int64_t n,
+                        TreeBufs& tb) {
+  k_bbox_init<<<1, 64, 0, s>>>(tb.bbox);
+  const unsigned nbk = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
+  if (split) {
+    k_bbox<double><<<nbk, 256, 0, s>>>((const double*)px, (const double*)py, (const double*)pz, n, tb.bbox);
+    k_morton<double><<<grid_for(n, 256), 256, 0, s>>>((const double*)px, (const double*)py, (const double*)pz, n,
+                                                      tb.bbox, tb.keys[0], tb.vals[0]);
+  } else {
+    k_bbox<float><<<nbk, 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, n, tb.bbox);
+    k_morton<float><<<grid_for(n, 256), 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, n,
+                                                     tb.bbox, tb.keys[0], tb.vals[0]);
+  }
+}
+
+void launch_radix_sort(hipStream_t s, int64_t n, int bits, TreeBufs& tb) {
+  const int nb = radix_tiles(n);
+  int cur = 0;
+  for (int shift = 0; shift < bits; shift += 8) {
+    k_radix_hist<<<nb, RADIX_THREADS, 0, s>>>(tb.keys[cur], n, shift, tb.hist, nb);
+    k_radix_scan<<<256, 256, 0, s>>>(tb.hist, nb, tb.dtot);
+    k_radix_scatter<<<nb, RADIX_THREADS, 0, s>>>(tb.keys[cur], tb.vals[cur], tb.keys[cur ^ 1], tb.vals[cur ^ 1], n,
+                                                 shift, tb.hist, tb.dtot, nb);
+    cur ^= 1;
+  }
+  tb.skeys = tb.keys[cur];
+  tb.perm = tb.vals[cur];
+}
+
+void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb) {
+  k_fix_ties<<<grid_for(n, 256), 256, 0, s>>>(tb.skeys, tb.perm, ids, n);
+}
+
+void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb) {
+  const int64_t L = (n + K - 1) / K;
+  k_leaf_codes<<<grid_for(L, 256), 256, 0, s>>>(tb.skeys, L, K, tb.ccode, tb.parent, tb.gstart);
+  if (L > 1) k_karras<<<grid_for(L - 1, 256), 256, 0, s>>>(tb.ccode, L, tb.child, tb.range, tb.parent, tb.ready);
+}
+
+void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb) {
+  const int64_t L = (n + K - 1) / K;
+  const int nblk = (int)grid_for(L, GRP_BLK);
+  k_mark_groups<<<grid_for(L, 256), 256, 0, s>>>(n, K, L, tb.range, tb.parent, tb.nbmin, tb.nbmax, tb.gstart);
+  k_group_count<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.gblk);
+  k_group_offsets<<<1, 64, 0, s>>>(tb.gblk, nblk, tb.ngroups, tb.wq);
+  k_group_emit<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.range, tb.gblk, tb.groups);
+}
+
+void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb) {
+  const int64_t L = (n + K - 1) / K;
+  const float inv_theta = theta > 0.f ? 1.0f / theta : 3.0e38f;
+  k_upsweep_local<<<grid_for(L, UPS_LEAVES), UPS_LEAVES, 0, s>>>(xm, hs, n, K, L, tb.child, tb.range, tb.parent,
+                                                                tb.ready, tb.ncom, tb.nbmin, tb.nbmax, inv_theta);
+  if (L > UPS_LEAVES)
+    k_upsweep_top<<<1, 1024, 0, s>>>(L, tb.child, tb.range, tb.parent, tb.ready, tb.ncom, tb.nbmin, tb.nbmax,
+                                     inv_theta);
+}

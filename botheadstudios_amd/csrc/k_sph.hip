@@ -1,0 +1,453 @@
+// SPH passes over Morton-sorted particles (gfx950, wave64):
+//   k_nbr_build  — per-group LBVH walk (box vs box + search radius), LDS-staged leaf tiles, per-particle
+//                  neighbour lists in column-major HBM layout (lane-coalesced reads later);
+//   k_hsolve     — Newton smoothing-length solve to a density-consistency tolerance (new: the reference
+//                  keeps h fixed, ie/crates/engine/src/hydrostatic.rs:58-61);
+//   k_density    — rho_i with the pairwise h_ij = (h_i+h_j)/2 (ie/shaders/sph_step.wgsl:118-149), then
+//                  the Tillotson P and c per particle (sph_step.wgsl:78-100) and the Courant signal
+//                  (sph_step.wgsl:262-269), fused;
+//   k_forces     — pressure + Monaghan AV + du/dt (ie/shaders/sph_step.wgsl:170-208), gather form.
+#include "common.h"
+#include "launch.h"
+
+namespace {
+
+constexpr int NB_WAVES = 4;
+constexpr int NB_STACK = 1024;
+constexpr int NB_RING = 128;
+
+struct NbrLds {
+  uint32_t stack[NB_STACK];
+  uint32_t leaf[NB_RING];
+  float4 part[64];
+  float phs[64];
+  uint32_t pidx[64];
+};
+
+// Neighbour list of one group (one wave): LBVH walk with box-vs-box tests against the larger search radius,
+// candidate leaves staged through LDS 64 particles at a time, each lane testing r < max(hs_i, hs_j).
+__device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t lane, const float4* __restrict__ xm,
+                                          const float* __restrict__ hs, int64_t n, int K, int64_t L,
+                                          const int2* __restrict__ child, const float4* __restrict__ nbmin,
+                                          const float4* __restrict__ nbmax, uint4* __restrict__ nbr4,
+                                          uint32_t* __restrict__ ncount, int cap, int64_t stride,
+                                          uint32_t* __restrict__ flags, uint64_t& ncand, uint64_t& nnodes,
+                                          uint64_t& npairs) {
+  const int64_t t0 = grp.x;
+  const int64_t t = t0 + lane;
+  const bool valid = lane < (uint32_t)grp.y;
+  const float4 me = xm[valid ? t : t0];
+  const float myhs = valid ? hs[t] : 0.f;
+  const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
+  const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
+  const float ghs = wave_max(myhs);
+  const uint64_t nvalid = (uint64_t)__popcll(__ballot(valid));
+  const uint32_t leaf0 = (uint32_t)(L - 1);
+  const uint32_t lpc = 64u / (uint32_t)K;
+  uint32_t sp = 1, lhead = 0, ltail = 0, cnt = 0;
+  wave_sync();
+  if (lane == 0) S.stack[0] = 0u;
+  wave_sync();
+
+  uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, nb = 0, nchunk = 0;
+  const uint32_t cap4 = (uint32_t)cap >> 2;
+  const uint32_t tt = (uint32_t)t;
+  auto flush = [&](uint32_t nl) {
+    const uint32_t li = lane / (uint32_t)K;
+    const bool have = li < nl;
+    const uint32_t leaf = have ? S.leaf[(lhead + li) & (NB_RING - 1)] : 0u;
+    const int64_t src = (int64_t)leaf * K + (lane % (uint32_t)K);
+    const bool ok = have && src < n;
+    wave_sync();
+    // Empty slots sit at 1e30 with a zero radius: they can never pass the distance test.
+    S.part[lane] = ok ? xm[src] : make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.f);
+    S.phs[lane] = ok ? hs[src] : 0.f;
+    S.pidx[lane] = ok ? (uint32_t)src : IHS_NONE;
+    wave_sync();
+    const uint32_t m = (min(64u, nl * (uint32_t)K) + 3u) & ~3u;
+    ncand += nvalid * m;
+    // Branch-free test of every candidate into a per-lane hit mask, then visit only the hits: a
+    // divergent branch inside the candidate loop would serialise the LDS loads.
+    uint64_t hits = 0;
+    for (uint32_t k0 = 0; k0 < m; k0 += 4) {
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e) {
+        const float4 p = S.part[k0 + e];
+        const uint32_t j = S.pidx[k0 + e];
+        const float R = fmaxf(myhs, S.phs[k0 + e]);
+        const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
+        const float r2 = dx * dx + dy * dy + dz * dz;
+        const uint64_t hit = (uint64_t)((uint32_t)(r2 < R * R) & (uint32_t)(j != tt));  // no short-circuit
+        hits |= hit << (k0 + e);
+      }
+    }
+    if (!valid) hits = 0;
+    while (hits) {
+      const uint32_t k = (uint32_t)__builtin_ctzll(hits);
+      hits &= hits - 1;
+      const uint32_t j = S.pidx[k];
+      // Neighbours are staged four at a time and written as one 16-byte chunk.
+      b0 = nb == 0 ? j : b0;
+      b1 = nb == 1 ? j : b1;
+      b2 = nb == 2 ? j : b2;
+      b3 = nb == 3 ? j : b3;
+      ++cnt;
+      if (++nb == 4) {
+        if (nchunk < cap4) nbr4[(int64_t)nchunk * stride + t] = make_uint4(b0, b1, b2, b3);
+        ++nchunk;
+        nb = 0;
+      }
+    }
+    lhead += nl;
+    wave_sync();
+  };
+
+  while (sp > 0) {
+    const uint32_t take = min(sp, 64u);
+    const uint32_t base = sp - take;
+    const uint32_t node = lane < take ? S.stack[base + lane] : IHS_NONE;
+    sp = base;
+    nnodes += take;
+    wave_sync();
+    bool pass = false;
+    if (node != IHS_NONE) {
+      const float4 c0 = nbmin[node], c1 = nbmax[node];
+      const float dx = fmaxf(fmaxf(c0.x - gx1, gx0 - c1.x), 0.f);
+      const float dy = fmaxf(fmaxf(c0.y - gy1, gy0 - c1.y), 0.f);
+      const float dz = fmaxf(fmaxf(c0.z - gz1, gz0 - c1.z), 0.f);
+      const float R = fmaxf(ghs, c0.w);
+      pass = dx * dx + dy * dy + dz * dz < R * R;
+    }
+    const bool isleaf = pass && node >= leaf0;
+    const bool open = pass && node < leaf0;
+    const uint64_t mo = __ballot(open), ml = __ballot(isleaf);
+    if (isleaf) S.leaf[(ltail + lanes_below(ml)) & (NB_RING - 1)] = node - leaf0;
+    ltail += (uint32_t)__popcll(ml);
+    const uint32_t nopen = (uint32_t)__popcll(mo);
+    if (sp + 2u * nopen > (uint32_t)NB_STACK) {
+      if (lane == 0) atomicAdd(&flags[FLAG_WALK_OVERFLOW], 1u);
+    } else {
+      if (open) {
+        const int2 ch = child[node];
+        const uint32_t pos = sp + 2u * lanes_below(mo);
+        S.stack[pos] = (uint32_t)ch.x;
+        S.stack[pos + 1] = (uint32_t)ch.y;
+      }
+      sp += 2u * nopen;
+    }
+    wave_sync();
+    while (ltail - lhead >= lpc) flush(lpc);
+  }
+  while (ltail != lhead) flush(min(lpc, ltail - lhead));
+  if (valid && nb > 0 && nchunk < cap4) nbr4[(int64_t)nchunk * stride + t] = make_uint4(b0, b1, b2, b3);
+  const uint32_t capn = cap4 * 4u;
+  if (valid) ncount[t] = min(cnt, capn);
+  const uint64_t over = __ballot(valid && cnt > capn);
+  npairs += wave_sum((unsigned long long)(valid ? min(cnt, capn) : 0u));
+  if (lane == 0 && over) atomicAdd(&flags[FLAG_NBR_OVERFLOW], (uint32_t)__popcll(over));
+}
+
+// Persistent: waves pull groups from per-XCD queues (wq[8..16)), then help the other queues.
+__global__ __launch_bounds__(NB_WAVES * 64) void k_nbr_build(const float4* __restrict__ xm, const float* __restrict__ hs,
+                                                             int64_t n, int K, int64_t L, const int2* __restrict__ child,
+                                                             const float4* __restrict__ nbmin,
+                                                             const float4* __restrict__ nbmax,
+                                                             const int4* __restrict__ groups,
+                                                             const uint32_t* __restrict__ ngroups,
+                                                             uint32_t* __restrict__ wq, uint4* __restrict__ nbr4,
+                                                             uint32_t* __restrict__ ncount, int cap, int64_t stride,
+                                                             uint32_t* __restrict__ flags,
+                                                             unsigned long long* __restrict__ counters) {
+  __shared__ NbrLds lds[NB_WAVES];
+  const int w = threadIdx.x >> 6;
+  const uint32_t lane = threadIdx.x & 63;
+  NbrLds& S = lds[w];
+  const uint32_t ng = *ngroups;
+  const uint32_t home = blockIdx.x & 7u;
+  uint64_t ncand = 0, nnodes = 0, npairs = 0;
+  for (uint32_t r = 0; r < 8; ++r) {
+    const uint32_t q = (home + r) & 7u;
+    const uint32_t lo = (uint32_t)((uint64_t)ng * q / 8u), hi = (uint32_t)((uint64_t)ng * (q + 1u) / 8u);
+    for (;;) {
+      uint32_t g = 0;
+      if (lane == 0) g = atomicAdd(&wq[8 + q], 1u);
+      g = (uint32_t)__builtin_amdgcn_readfirstlane((int)g) + lo;
+      if (g >= hi) break;
+      nbr_group(S, groups[g], lane, xm, hs, n, K, L, child, nbmin, nbmax, nbr4, ncount, cap, stride, flags, ncand,
+                nnodes, npairs);
+    }
+  }
+  if (lane == 0) {
+    atomicAdd(&counters[CNT_PAIRS], npairs);
+    atomicAdd(&counters[CNT_NBR_CAND], ncand);
+    atomicAdd(&counters[CNT_WALK_NODES], nnodes);
+  }
+}
+
+// Visits the neighbour list of particle t: 16-byte chunks of four indices, chunk c of particle t at
+// c * stride + t (lanes of a wave read consecutive chunks).
+template <class F>
+__device__ __forceinline__ void for_neighbours(const uint4* __restrict__ nbr4, int64_t stride, int64_t t, uint32_t cnt,
+                                               F&& f) {
+  for (uint32_t k = 0; k < cnt; k += 4) {
+    const uint4 q = nbr4[(int64_t)(k >> 2) * stride + t];
+    f(q.x);
+    if (k + 1 < cnt) f(q.y);
+    if (k + 2 < cnt) f(q.z);
+    if (k + 3 < cnt) f(q.w);
+  }
+}
+
+// Own-h gather density rho~(h) = sum_j m_j W(r_ij, h) (self included) and h*d(rho~)/dh, using
+// dW/dh = -(3W + r dW/dr)/h. Returns rho~, writes h*drho/dh.
+__device__ __forceinline__ float own_density(const float4 me, float hh, const uint4* __restrict__ nbr4, int64_t stride,
+                                             int64_t t, uint32_t cnt, const float4* __restrict__ xm, float& hdrho,
+                                             uint32_t& nngb) {
+  const float hinv = 1.0f / hh;
+  const float sig = (8.0f / kPI32) * hinv * hinv * hinv;
+  float rho = me.w * sig;
+  float hd = -3.0f * rho;
+  const float h2 = hh * hh;
+  uint32_t c = 0;
+  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) {
+    const float4 q = xm[j];
+    const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < h2) {
+      const float qq = sqrtf(r2) * hinv;
+      float wq, dq;
+      if (qq < 0.5f) {
+        wq = 1.0f - 6.0f * qq * qq + 6.0f * qq * qq * qq;
+        dq = -12.0f * qq + 18.0f * qq * qq;
+      } else {
+        const float tt = 1.0f - qq;
+        wq = 2.0f * tt * tt * tt;
+        dq = -6.0f * tt * tt;
+      }
+      const float wv = sig * wq;
+      rho += q.w * wv;
+      hd -= q.w * (3.0f * wv + qq * sig * dq);
+      ++c;
+    }
+  });
+  hdrho = hd;
+  nngb = c;
+  return rho;
+}
+
+__global__ __launch_bounds__(256) void k_hsolve(const float4* __restrict__ xm, const float* __restrict__ hs,
+                                                float* __restrict__ h, uint8_t* __restrict__ iters,
+                                                const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
+                                                int64_t stride, int64_t n, float eta3, float tol, int max_iter,
+                                                uint32_t* __restrict__ flags, unsigned long long* __restrict__ counters) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < n;
+  uint32_t it = 0, evals = 0;
+  bool capped = false, unconv = false;
+  if (valid) {
+    const float4 me = xm[t];
+    const float hsi = hs[t];
+    const uint32_t cnt = ncount[t];
+    float hh = fminf(h[t], hsi);
+    float lo = 0.f, hi = INFINITY;
+    for (;;) {
+      float hd;
+      uint32_t nn;
+      const float rho = own_density(me, hh, nbr4, stride, t, cnt, xm, hd, nn);
+      ++evals;
+      const float hinv = 1.0f / hh;
+      const float target = me.w * eta3 * hinv * hinv * hinv;
+      const float f = rho - target;
+      if (fabsf(f) <= tol * rho) break;
+      if ((int)it >= max_iter) {
+        unconv = true;
+        break;
+      }
+      if (f < 0.f) lo = hh;
+      else hi = hh;
+      if (f < 0.f && hh >= hsi) {
+        capped = true;
+        break;
+      }
+      const float hdf = hd + 3.0f * target;  // h * df/dh
+      float hn = hdf > 0.f ? hh - f * hh / hdf : -1.0f;
+      if (!(hn > lo && hn < hi)) hn = isinf(hi) ? 2.0f * hh : 0.5f * (lo + hi);
+      hh = fminf(hn, hsi);
+      ++it;
+    }
+    h[t] = hh;
+    iters[t] = (uint8_t)min(it, 255u);
+  }
+  const uint64_t mc = __ballot(capped), mu = __ballot(unconv);
+  const unsigned long long si = wave_sum((unsigned long long)it), se = wave_sum((unsigned long long)evals);
+  const unsigned long long sn = (unsigned long long)__popcll(__ballot(valid));
+  if ((threadIdx.x & 63) == 0 && sn) {
+    if (mc) atomicAdd(&flags[FLAG_H_CAPPED], (uint32_t)__popcll(mc));
+    if (mu) atomicAdd(&flags[FLAG_H_UNCONVERGED], (uint32_t)__popcll(mu));
+    atomicAdd(&counters[CNT_H_ITERS], si);
+    atomicAdd(&counters[CNT_H_EVALS], se);
+    atomicAdd(&counters[CNT_H_SOLVED], sn);
+  }
+}
+
+// Density with h_ij, EOS, Courant signal; optional MLP features (neighbour count within h_i,
+// div v and |grad rho|, both with the h_ij kernel gradient).
+template <class PT, bool FEAT>
+__global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, const float* __restrict__ h,
+                                                 const float* __restrict__ vx, const float* __restrict__ vy,
+                                                 const float* __restrict__ vz, const PT* __restrict__ u,
+                                                 const uint8_t* __restrict__ mat, const EosDev* __restrict__ eos,
+                                                 const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
+                                                 int64_t stride, int64_t n, float4* __restrict__ rpc,
+                                                 float4* __restrict__ vh, float* __restrict__ rho_out,
+                                                 float4* __restrict__ feat, uint32_t* __restrict__ flags) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < n;
+  uint32_t key = 0xffffffffu;
+  if (valid) {
+    const float4 me = xm[t];
+    const float hi = h[t];
+    const float vxi = vx[t], vyi = vy[t], vzi = vz[t];
+    const uint32_t cnt = ncount[t];
+    const float hinv_i = 1.0f / hi;
+    float rho = me.w * (8.0f / kPI32) * hinv_i * hinv_i * hinv_i;
+    float dv = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
+    uint32_t nn = 0;
+    for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) {
+      const float4 q = xm[j];
+      const float hj = h[j];
+      const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float hij = 0.5f * (hi + hj);
+      if (FEAT && r2 < hi * hi) ++nn;
+      if (r2 < hij * hij) {
+        const float r = sqrtf(r2);
+        rho += q.w * sph_w32(r, hij);
+        if (FEAT && r > 1.0e-9f) {
+          const float dw = sph_dw32(r, hij) / r;
+          const float ddx = dx * dw, ddy = dy * dw, ddz = dz * dw;
+          gx += q.w * ddx;
+          gy += q.w * ddy;
+          gz += q.w * ddz;
+          dv += q.w * ((vxi - vx[j]) * ddx + (vyi - vy[j]) * ddy + (vzi - vz[j]) * ddz);
+        }
+      }
+    });
+    const EosDev e = eos[mat[t]];
+    const float uu = (float)u[t];
+    const float p = till_pressure(e, rho, uu);
+    const float c = till_sound_speed(e, rho, uu, p);
+    rpc[t] = make_float4(rho, p / (rho * rho), c, p);
+    vh[t] = make_float4(vxi, vyi, vzi, hi);
+    rho_out[t] = rho;
+    if (FEAT) feat[t] = make_float4((float)nn, -dv / rho, sqrtf(gx * gx + gy * gy + gz * gz), 0.f);
+    const float sig = hi / fmaxf(c + sqrtf(vxi * vxi + vyi * vyi + vzi * vzi), 1.0f);
+    key = f2key(sig);
+    if (!(sig == sig) || !(rho == rho)) atomicAdd(&flags[FLAG_NONFINITE], 1u);
+  }
+  key = wave_min(key);
+  if ((threadIdx.x & 63) == 0 && key != 0xffffffffu) atomicMin(&flags[FLAG_SIGNAL_MIN], key);
+}
+
+template <bool KAHAN>
+__global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, const float4* __restrict__ vh,
+                                                const float4* __restrict__ rpc, const uint4* __restrict__ nbr4,
+                                                const uint32_t* __restrict__ ncount, int64_t stride, int64_t n,
+                                                const float* __restrict__ av, float4* __restrict__ asph) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const float alpha = av[0], beta = av[1];
+  const float4 me = xm[t];
+  const float4 vi = vh[t];
+  const float4 ri = rpc[t];
+  const uint32_t cnt = ncount[t];
+  float ax = 0.f, ay = 0.f, az = 0.f, de = 0.f;
+  Kahan3 ka;
+  ka.zero();
+  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) {
+    const float4 q = xm[j];
+    const float4 vj = vh[j];
+    const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    const float hij = 0.5f * (vi.w + vj.w);
+    if (!(r2 < hij * hij) || !(r2 > 1.0e-18f)) return;
+    const float4 rj = rpc[j];
+    const float r = sqrtf(r2);
+    const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
+    const float vr = dvx * dx + dvy * dy + dvz * dz;
+    float pij = 0.f;
+    if (vr < 0.f) {
+      const float mu = hij * vr / (r2 + 0.01f * hij * hij);
+      const float cbar = 0.5f * (ri.z + rj.z);
+      const float rbar = 0.5f * (ri.x + rj.x);
+      pij = (-alpha * cbar * mu + beta * mu * mu) / rbar;
+    }
+    const float coeff = ri.y + rj.y + pij;
+    const float gw = sph_dw32(r, hij) / r;
+    const float fx = dx * gw, fy = dy * gw, fz = dz * gw;
+    const float s = -coeff * q.w;
+    if (KAHAN) {
+      ka.add(fx * s, 0);
+      ka.add(fy * s, 1);
+      ka.add(fz * s, 2);
+    } else {
+      ax = fmaf(fx, s, ax);
+      ay = fmaf(fy, s, ay);
+      az = fmaf(fz, s, az);
+    }
+    de += 0.5f * q.w * coeff * (dvx * fx + dvy * fy + dvz * fz);
+  });
+  if (KAHAN) {
+    ax = ka.s[0];
+    ay = ka.s[1];
+    az = ka.s[2];
+  }
+  asph[t] = make_float4(ax, ay, az, de);
+}
+
+inline unsigned g256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+}  // namespace
+
+void launch_nbr_build(hipStream_t s, int64_t n, int K, const float4* xm, const float* hs, const TreeBufs& tb,
+                      uint32_t* nbr, uint32_t* ncount, int cap, int64_t stride, uint32_t* flags,
+                      unsigned long long* counters) {
+  uint4* nbr4 = (uint4*)nbr;
+  const int64_t L = (n + K - 1) / K;
+  static unsigned nblk = 0;
+  if (!nblk) nblk = persistent_blocks((const void*)k_nbr_build, NB_WAVES * 64);
+  k_nbr_build<<<nblk, NB_WAVES * 64, 0, s>>>(xm, hs, n, K, L, tb.child, tb.nbmin, tb.nbmax, tb.groups, tb.ngroups,
+                                             tb.wq, nbr4, ncount, cap, stride, flags, counters);
+}
+
+void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, float* h, uint8_t* iters,
+                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol, int max_iter,
+                   uint32_t* flags, unsigned long long* counters) {
+  k_hsolve<<<g256(n), 256, 0, s>>>(xm, hs, h, iters, (const uint4*)nbr, ncount, stride, n, eta3, tol, max_iter, flags,
+                                   counters);
+}
+
+void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const float* vx,
+                    const float* vy, const float* vz, const void* u, const uint8_t* mat, const void* eos,
+                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float4* rpc, float4* vh, float* rho,
+                    float4* featv, uint32_t* flags) {
+  const EosDev* e = (const EosDev*)eos;
+#define IHS_DENS(PT, F)                                                                                              \
+  k_density<PT, F><<<g256(n), 256, 0, s>>>(xm, h, vx, vy, vz, (const PT*)u, mat, e, (const uint4*)nbr, ncount, stride, n, rpc, vh, \
+                                           rho, featv, flags)
+  if (split) {
+    if (feat) IHS_DENS(double, true);
+    else IHS_DENS(double, false);
+  } else {
+    if (feat) IHS_DENS(float, true);
+    else IHS_DENS(float, false);
+  }
+#undef IHS_DENS
+}
+
+void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const float4* vh, const float4* rpc,
+                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, const float* av, float4* asph) {
+  const uint4* nbr4 = (const uint4*)nbr;
+  if (kahan) k_forces<true><<<g256(n), 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph);
+  else k_forces<false><<<g256(n), 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph);
+}

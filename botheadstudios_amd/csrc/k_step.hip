@@ -1,0 +1,296 @@
+// State permutation, leapfrog integration and FP64 reductions (gfx950).
+//
+// Integration follows the reference's KDK (ie/shaders/sph_step.wgsl:213-237; ie/crates/engine/src/
+// hydrostatic.rs:345-359): v += a dt/2; u = max(u + du dt/2, 0); x += v dt; ...; v += a dt/2; u = ....
+// In split precision the position and internal-energy updates run in FP64 (PT = double); otherwise
+// the whole update is FP32 as in the reference. The Courant dt (ie/crates/engine/src/hydrostatic.rs:
+// 366-374; impact-run:345-356) is formed on the device from the density pass's min-signal key, so a
+// step needs no host round trip.
+#include "common.h"
+#include "launch.h"
+
+namespace {
+
+inline unsigned g256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+__global__ void k_fe_init(uint32_t* flags) {
+  if (threadIdx.x == 0) flags[FLAG_SIGNAL_MIN] = 0xffffffffu;
+}
+
+template <class PT>
+__global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ perm, int64_t n, MasterBufs S, MasterBufs D,
+                                                float4* __restrict__ xm, float* __restrict__ hs, float margin,
+                                                float hcap) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t s = perm[t];
+  const PT x = ((const PT*)S.px)[s], y = ((const PT*)S.py)[s], z = ((const PT*)S.pz)[s];
+  ((PT*)D.px)[t] = x;
+  ((PT*)D.py)[t] = y;
+  ((PT*)D.pz)[t] = z;
+  D.vx[t] = S.vx[s];
+  D.vy[t] = S.vy[s];
+  D.vz[t] = S.vz[s];
+  ((PT*)D.u)[t] = ((const PT*)S.u)[s];
+  const float h = S.h[s];
+  D.h[t] = h;
+  const float m = S.m[s];
+  D.m[t] = m;
+  D.mat[t] = S.mat[s];
+  D.id[t] = S.id[s];
+  D.prov[t] = S.prov[s];
+  D.rho[t] = S.rho[s];
+  D.feat[t] = S.feat[s];
+  D.hiter[t] = S.hiter[s];
+  xm[t] = make_float4((float)x, (float)y, (float)z, m);
+  hs[t] = fminf(h * margin, fmaxf(hcap, h));
+}
+
+__device__ __forceinline__ double step_dt(const StepCtl* ctl, const uint32_t* flags) {
+  return ctl->dt_fixed > 0.0 ? ctl->dt_fixed : ctl->cfl * (double)key2f(flags[FLAG_SIGNAL_MIN]);
+}
+
+template <class PT>
+__device__ __forceinline__ PT advance(PT x, float v, double dt) {
+  return (PT)(x + (PT)v * (PT)dt);
+}
+
+template <class PT>
+__global__ __launch_bounds__(256) void k_kick_drift(int64_t n, MasterBufs S, const float4* __restrict__ asph,
+                                                    const float4* __restrict__ accg, StepCtl* ctl,
+                                                    const uint32_t* __restrict__ flags) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double dt = step_dt(ctl, flags);
+  if (t == 0) {
+    ctl->dt = dt;
+    ctl->t += dt;
+    ctl->steps += 1;
+  }
+  if (t >= n) return;
+  const float4 a = asph[t], g = accg[t];
+  const float hdt = (float)(0.5 * dt);
+  const float vx = S.vx[t] + (a.x + g.x) * hdt;
+  const float vy = S.vy[t] + (a.y + g.y) * hdt;
+  const float vz = S.vz[t] + (a.z + g.z) * hdt;
+  S.vx[t] = vx;
+  S.vy[t] = vy;
+  S.vz[t] = vz;
+  PT* u = (PT*)S.u;
+  const PT un = u[t] + (PT)a.w * (PT)(0.5 * dt);
+  u[t] = un > (PT)0 ? un : (PT)0;
+  ((PT*)S.px)[t] = advance<PT>(((PT*)S.px)[t], vx, dt);
+  ((PT*)S.py)[t] = advance<PT>(((PT*)S.py)[t], vy, dt);
+  ((PT*)S.pz)[t] = advance<PT>(((PT*)S.pz)[t], vz, dt);
+}
+
+template <class PT>
+__global__ __launch_bounds__(256) void k_kick(int64_t n, MasterBufs S, const float4* __restrict__ asph,
+                                              const float4* __restrict__ accg, const StepCtl* __restrict__ ctl) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const double dt = ctl->dt;
+  const float4 a = asph[t], g = accg[t];
+  const float hdt = (float)(0.5 * dt);
+  S.vx[t] += (a.x + g.x) * hdt;
+  S.vy[t] += (a.y + g.y) * hdt;
+  S.vz[t] += (a.z + g.z) * hdt;
+  PT* u = (PT*)S.u;
+  const PT un = u[t] + (PT)a.w * (PT)(0.5 * dt);
+  u[t] = un > (PT)0 ? un : (PT)0;
+}
+
+// ie/shaders/sph_step.wgsl:244-257 (damped relaxation in a frame rotating at omega about +z).
+template <class PT>
+__global__ __launch_bounds__(256) void k_relax(int64_t n, MasterBufs S, const float4* __restrict__ asph,
+                                               const float4* __restrict__ accg, StepCtl* ctl) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double dt = ctl->dt_fixed;
+  if (t == 0) {
+    ctl->dt = dt;
+    ctl->t += dt;
+    ctl->steps += 1;
+  }
+  if (t >= n) return;
+  const float4 a = asph[t], g = accg[t];
+  const float w2 = ctl->omega * ctl->omega;
+  const float fdt = (float)dt, damp = ctl->damp;
+  PT* px = (PT*)S.px;
+  PT* py = (PT*)S.py;
+  PT* pz = (PT*)S.pz;
+  const float x = (float)px[t], y = (float)py[t];
+  const float vx = (S.vx[t] + (a.x + g.x + w2 * x) * fdt) * damp;
+  const float vy = (S.vy[t] + (a.y + g.y + w2 * y) * fdt) * damp;
+  const float vz = (S.vz[t] + (a.z + g.z) * fdt) * damp;
+  S.vx[t] = vx;
+  S.vy[t] = vy;
+  S.vz[t] = vz;
+  px[t] = advance<PT>(px[t], vx, dt);
+  py[t] = advance<PT>(py[t], vy, dt);
+  pz[t] = advance<PT>(pz[t], vz, dt);
+}
+
+constexpr int EN_BLOCKS = 1024;
+
+// Per-block FP64 partials of KE, IE, PE (= 1/2 m phi), momentum and angular momentum about the origin.
+template <class PT>
+__global__ __launch_bounds__(256) void k_energy_part(int64_t n, MasterBufs S, const float4* __restrict__ accg,
+                                                     double* __restrict__ part) {
+  double q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const double m = S.m[t];
+    const double vx = S.vx[t], vy = S.vy[t], vz = S.vz[t];
+    const double x = ((const PT*)S.px)[t], y = ((const PT*)S.py)[t], z = ((const PT*)S.pz)[t];
+    q[0] += 0.5 * m * (vx * vx + vy * vy + vz * vz);
+    q[1] += m * (double)((const PT*)S.u)[t];
+    q[2] += 0.5 * m * (double)accg[t].w;
+    q[3] += m * vx;
+    q[4] += m * vy;
+    q[5] += m * vz;
+    q[6] += m * (y * vz - z * vy);
+    q[7] += m * (z * vx - x * vz);
+    q[8] += m * (x * vy - y * vx);
+  }
+  __shared__ double s[4][9];
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) q[k] = wave_sum(q[k]);
+  if (lane == 0)
+    for (int k = 0; k < 9; ++k) s[w][k] = q[k];
+  __syncthreads();
+  if (threadIdx.x < 9) part[(int64_t)blockIdx.x * 9 + threadIdx.x] = s[0][threadIdx.x] + s[1][threadIdx.x] +
+                                                                     s[2][threadIdx.x] + s[3][threadIdx.x];
+}
+
+// Fixed-order final sum (deterministic): 256 threads take strided block partials, then a fixed tree.
+__global__ __launch_bounds__(256) void k_energy_final(const double* __restrict__ part, int nb,
+                                                      double* __restrict__ out) {
+  __shared__ double s[9][256];
+  double q[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int b = threadIdx.x; b < nb; b += 256)
+    for (int k = 0; k < 9; ++k) q[k] += part[(int64_t)b * 9 + k];
+  for (int k = 0; k < 9; ++k) s[k][threadIdx.x] = q[k];
+  __syncthreads();
+  for (int w = 128; w >= 1; w >>= 1) {
+    if ((int)threadIdx.x < w)
+      for (int k = 0; k < 9; ++k) s[k][threadIdx.x] += s[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 9) out[threadIdx.x] = s[threadIdx.x][0];
+}
+
+template <class PT>
+__global__ __launch_bounds__(256) void k_unpack(int64_t n, const SphParticle48* __restrict__ in, MasterBufs S) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const SphParticle48 p = in[i];
+  ((PT*)S.px)[i] = (PT)p.pos[0];
+  ((PT*)S.py)[i] = (PT)p.pos[1];
+  ((PT*)S.pz)[i] = (PT)p.pos[2];
+  S.vx[i] = p.vel[0];
+  S.vy[i] = p.vel[1];
+  S.vz[i] = p.vel[2];
+  ((PT*)S.u)[i] = (PT)p.u;
+  S.h[i] = p.h;
+  S.m[i] = p.mass;
+  S.mat[i] = (uint8_t)min(p.mat, 15u);
+  S.id[i] = (uint32_t)i;
+  S.prov[i] = p.prov;
+  S.rho[i] = p.rho;
+  S.feat[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  S.hiter[i] = 0;
+}
+
+template <class PT>
+__global__ __launch_bounds__(256) void k_pack(int64_t n, MasterBufs S, SphParticle48* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  SphParticle48 p;
+  p.pos[0] = (float)((const PT*)S.px)[t];
+  p.pos[1] = (float)((const PT*)S.py)[t];
+  p.pos[2] = (float)((const PT*)S.pz)[t];
+  p.h = S.h[t];
+  p.vel[0] = S.vx[t];
+  p.vel[1] = S.vy[t];
+  p.vel[2] = S.vz[t];
+  p.u = (float)((const PT*)S.u)[t];
+  p.mass = S.m[t];
+  p.mat = S.mat[t];
+  p.rho = S.rho[t];
+  p.prov = S.prov[t];
+  out[S.id[t]] = p;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_scatter(int64_t n, const uint32_t* __restrict__ id, const T* __restrict__ src,
+                                                 T* __restrict__ dst) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) dst[id[t]] = src[t];
+}
+
+template <class PT>
+__global__ __launch_bounds__(256) void k_pos_f64(int64_t n, MasterBufs S, double* __restrict__ out) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const uint32_t i = S.id[t];
+  out[3 * (int64_t)i] = (double)((const PT*)S.px)[t];
+  out[3 * (int64_t)i + 1] = (double)((const PT*)S.py)[t];
+  out[3 * (int64_t)i + 2] = (double)((const PT*)S.pz)[t];
+}
+
+}  // namespace
+
+void launch_fe_init(hipStream_t s, uint32_t* flags) { k_fe_init<<<1, 64, 0, s>>>(flags); }
+
+void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,
+                   const MasterBufs& dst, float4* xm, float* hs, float h_margin, float h_cap) {
+  if (split) k_gather<double><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, hs, h_margin, h_cap);
+  else k_gather<float><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, hs, h_margin, h_cap);
+}
+
+void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph,
+                       const float4* accg, StepCtl* ctl, const uint32_t* flags) {
+  const unsigned g = std::max(1u, g256(n));
+  if (split) k_kick_drift<double><<<g, 256, 0, s>>>(n, st, asph, accg, ctl, flags);
+  else k_kick_drift<float><<<g, 256, 0, s>>>(n, st, asph, accg, ctl, flags);
+}
+
+void launch_kick(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
+                 const StepCtl* ctl) {
+  if (split) k_kick<double><<<g256(n), 256, 0, s>>>(n, st, asph, accg, ctl);
+  else k_kick<float><<<g256(n), 256, 0, s>>>(n, st, asph, accg, ctl);
+}
+
+void launch_relax(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
+                  StepCtl* ctl) {
+  const unsigned g = std::max(1u, g256(n));
+  if (split) k_relax<double><<<g, 256, 0, s>>>(n, st, asph, accg, ctl);
+  else k_relax<float><<<g, 256, 0, s>>>(n, st, asph, accg, ctl);
+}
+
+void launch_energy(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* accg, double* part,
+                   double* out) {
+  const int nb = (int)std::min<int64_t>(EN_BLOCKS, std::max<int64_t>(1, (n + 255) / 256));
+  if (split) k_energy_part<double><<<nb, 256, 0, s>>>(n, st, accg, part);
+  else k_energy_part<float><<<nb, 256, 0, s>>>(n, st, accg, part);
+  k_energy_final<<<1, 256, 0, s>>>(part, nb, out);
+}
+
+void launch_unpack(hipStream_t s, int split, int64_t n, const SphParticle48* in, const MasterBufs& st) {
+  if (split) k_unpack<double><<<g256(n), 256, 0, s>>>(n, in, st);
+  else k_unpack<float><<<g256(n), 256, 0, s>>>(n, in, st);
+}
+
+void launch_pack(hipStream_t s, int split, int64_t n, const MasterBufs& st, SphParticle48* out) {
+  if (split) k_pack<double><<<g256(n), 256, 0, s>>>(n, st, out);
+  else k_pack<float><<<g256(n), 256, 0, s>>>(n, st, out);
+}
+
+void launch_scatter_by_id(hipStream_t s, int64_t n, const uint32_t* id, const void* src, void* dst, int elem) {
+  if (elem == 16) k_scatter<float4><<<g256(n), 256, 0, s>>>(n, id, (const float4*)src, (float4*)dst);
+  else k_scatter<uint32_t><<<g256(n), 256, 0, s>>>(n, id, (const uint32_t*)src, (uint32_t*)dst);
+}
+
+void launch_pos_f64(hipStream_t s, int split, int64_t n, const MasterBufs& st, double* out3) {
+  if (split) k_pos_f64<double><<<g256(n), 256, 0, s>>>(n, st, out3);
+  else k_pos_f64<float><<<g256(n), 256, 0, s>>>(n, st, out3);
+}

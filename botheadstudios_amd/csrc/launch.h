@@ -1,0 +1,177 @@
+// Host-side launchers of the gfx950 kernels (each .hip file defines its kernels and launchers).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <algorithm>
+
+// Error/diagnostic counters kept on device (indices into Workspace::flags).
+enum FlagIndex : int {
+  FLAG_NBR_OVERFLOW = 0,   // particles whose neighbour list exceeded its capacity
+  FLAG_WALK_OVERFLOW = 1,  // tree-walk stack overflows (interactions dropped)
+  FLAG_H_CAPPED = 2,       // h-solves stopped at the searched radius (isolated particles)
+  FLAG_H_UNCONVERGED = 3,  // h-solves that hit the iteration limit
+  FLAG_SIGNAL_MIN = 4,     // order-preserving key of min_i h_i/max(c_i+|v_i|,1)
+  FLAG_NONFINITE = 5,      // particles with non-finite results
+  FLAG_COUNT = 8
+};
+// 64-bit counters (Workspace::counters).
+enum CounterIndex : int {
+  CNT_H_ITERS = 0,   // total Newton updates
+  CNT_H_EVALS = 1,   // total density evaluations in the h-solve
+  CNT_H_SOLVED = 2,  // particles solved
+  CNT_PAIRS = 3,     // neighbour list entries
+  CNT_GRAV_CELL = 4, // gravity: target x accepted-node interactions evaluated
+  CNT_GRAV_PP = 5,   // gravity: target x leaf-particle interactions evaluated
+  CNT_NBR_CAND = 6,  // neighbour search: target x candidate tests
+  CNT_WALK_NODES = 7, // nodes popped by all walks (gravity + neighbour search)
+  CNT_COUNT = 8
+};
+
+// Buffers of one Morton/sort/Karras/upsweep pipeline (sized for a capacity).
+struct TreeBufs {
+  int64_t cap = 0;   // particle capacity
+  int64_t lcap = 0;  // leaf capacity
+  int nb_cap = 0;    // radix tile capacity
+  uint32_t* bbox = nullptr;  // 6 order-preserving keys
+  uint32_t* keys[2] = {nullptr, nullptr};
+  uint32_t* vals[2] = {nullptr, nullptr};
+  uint32_t* hist = nullptr;  // 256 * nb
+  uint32_t* dtot = nullptr;  // 256
+  uint32_t* ccode = nullptr; // L
+  int2* child = nullptr;     // L-1 (left, right)
+  int2* range = nullptr;     // L-1 (first, last leaf)
+  uint32_t* parent = nullptr;  // 2L-1
+  uint32_t* ready = nullptr;   // L-1
+  float4* ncom = nullptr;   // 2L-1: com xyz, mass
+  float4* nbmin = nullptr;  // 2L-1: bmin xyz, max search radius
+  float4* nbmax = nullptr;  // 2L-1: bmax xyz, critical radius |bmax-bmin|/theta + |centre-com|
+  // walk groups: contiguous Morton ranges of at most GROUP_MAX particles with compact boxes, in leaf order
+  uint32_t* gstart = nullptr;   // L: group root node if leaf c is the first leaf of a group; GROUP_SPLIT if
+                                // leaf c is split into single-particle groups; none otherwise
+  uint32_t* gblk = nullptr;     // per-1024-leaf block: group count, then exclusive offset
+  int4* groups = nullptr;       // (first particle, count, node, 0), up to n entries
+  uint32_t* ngroups = nullptr;  // 1
+  uint32_t* wq = nullptr;       // 16 work-queue counters (8 per walk kernel, one per XCD)
+  // results of the last sort: sorted keys and the permutation (pointers into keys/vals)
+  uint32_t* skeys = nullptr;
+  uint32_t* perm = nullptr;
+};
+
+constexpr int GROUP_MAX = 64;            // one wavefront
+constexpr float GROUP_DIAG_HS = 6.0f;     // a group's box diagonal is at most this many search radii
+constexpr uint32_t GROUP_SPLIT = 0xfffffffeu;
+
+constexpr int RADIX_THREADS = 256;
+constexpr int RADIX_ITEMS = 8;
+constexpr int RADIX_TILE = RADIX_THREADS * RADIX_ITEMS;
+inline int radix_tiles(int64_t n) { return (int)((n + RADIX_TILE - 1) / RADIX_TILE); }
+
+// --- k_lbvh.hip ---
+// bbox + Morton codes from positions (PT = float or double, SoA), vals = identity.
+void launch_bbox_morton(hipStream_t s, int split, const void* px, const void* py, const void* pz, int64_t n,
+                        TreeBufs& tb);
+// Stable LSD radix sort of (keys[0], vals[0]) over `bits` low bits; sets tb.skeys / tb.perm.
+void launch_radix_sort(hipStream_t s, int64_t n, int bits, TreeBufs& tb);
+// Reorders runs of equal codes in tb.perm by ids[perm] (so the order is (code, id)).
+void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb);
+// Leaf cluster codes + Karras topology + counters reset.
+void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb);
+// Walk groups in leaf order -> tb.groups / tb.ngroups, and resets the walk work queues. A group is the
+// topmost subtree on a leaf's root path holding <= GROUP_MAX particles whose box diagonal is at most
+// GROUP_DIAG_HS times its largest search radius (no radius: no box limit); leaves without one are split
+// into single-particle groups. Needs the upsweep's boxes.
+void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb);
+// Leaf moments and ordered upsweep; xm = sorted (x,y,z,m), hs = sorted search radius (or null).
+void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb);
+
+// --- k_gravity.hip ---
+// Wave-coherent Barnes-Hut group walk over sorted particles; accg = (a/G xyz... see kernel), sorted order.
+// Persistent: waves pull groups from per-XCD queues (tb.wq[0..8)).
+void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, float soft2, const float4* xm,
+                         const TreeBufs& tb, float4* accg, uint32_t* flags, unsigned long long* counters);
+// Tiled direct O(N^2) sum (FP32), sorted or unsorted, acc4 = (acc xyz, potential).
+void launch_gravity_direct(hipStream_t s, int64_t n, float soft2, const float4* xm, float4* acc4);
+
+// --- k_sph.hip ---
+void launch_nbr_build(hipStream_t s, int64_t n, int K, const float4* xm, const float* hs, const TreeBufs& tb,
+                      uint32_t* nbr, uint32_t* ncount, int cap, int64_t stride, uint32_t* flags,
+                      unsigned long long* counters);
+void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, float* h, uint8_t* iters,
+                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol, int max_iter,
+                   uint32_t* flags, unsigned long long* counters);
+void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const float* vx,
+                    const float* vy, const float* vz, const void* u, const uint8_t* mat, const void* eos,
+                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float4* rpc, float4* vh, float* rho,
+                    float4* featv, uint32_t* flags);
+void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const float4* vh, const float4* rpc,
+                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, const float* av, float4* asph);
+
+// --- k_step.hip ---
+// Master particle state (SoA, current storage order). Positions and u are double in split precision,
+// float otherwise (PT).
+struct MasterBufs {
+  void* px = nullptr;
+  void* py = nullptr;
+  void* pz = nullptr;
+  float* vx = nullptr;
+  float* vy = nullptr;
+  float* vz = nullptr;
+  void* u = nullptr;
+  float* h = nullptr;
+  float* m = nullptr;
+  uint8_t* mat = nullptr;
+  uint32_t* id = nullptr;
+  uint32_t* prov = nullptr;
+  float* rho = nullptr;
+  float4* feat = nullptr;   // MLP inputs of the last density pass: (n_ngb, div v, |grad rho|, -)
+  uint8_t* hiter = nullptr; // Newton updates in the last h-solve
+};
+
+// Device-resident step control (kernels read it, so a captured step needs no host values).
+struct StepCtl {
+  double dt_fixed;  // > 0: fixed dt; otherwise cfl * min signal
+  double cfl;
+  double dt;        // dt of the last step
+  double t;         // simulated time
+  float av[2];      // artificial viscosity alpha, beta
+  float damp;       // relaxation damping
+  float omega;      // relaxation frame rotation rate about +z
+  uint64_t steps;
+};
+
+// 48-byte interchange record (gpu_sph.rs:25-37).
+struct SphParticle48 {
+  float pos[3];
+  float h;
+  float vel[3];
+  float u;
+  float mass;
+  uint32_t mat;
+  float rho;
+  uint32_t prov;
+};
+
+void launch_fe_init(hipStream_t s, uint32_t* flags);
+void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,
+                   const MasterBufs& dst, float4* xm, float* hs, float h_margin, float h_cap);
+void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph,
+                       const float4* accg, StepCtl* ctl, const uint32_t* flags);
+void launch_kick(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
+                 const StepCtl* ctl);
+void launch_relax(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
+                  StepCtl* ctl);
+// FP64 KE, IE, PE, p, L (9 values) into out; part needs 9 * 1024 doubles.
+void launch_energy(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* accg, double* part,
+                   double* out);
+void launch_unpack(hipStream_t s, int split, int64_t n, const SphParticle48* in, const MasterBufs& st);
+void launch_pack(hipStream_t s, int split, int64_t n, const MasterBufs& st, SphParticle48* out);
+// Scatter a sorted-order array to id order (elem = 4 or 16 bytes).
+void launch_scatter_by_id(hipStream_t s, int64_t n, const uint32_t* id, const void* src, void* dst, int elem);
+// Double-precision positions (split) or float positions widened -> (n,3) double in id order.
+void launch_pos_f64(hipStream_t s, int split, int64_t n, const MasterBufs& st, double* out3);
+// Standalone gravity helpers (k_gravity.hip).
+void launch_bodies_in(hipStream_t s, const double* pos3, const double* mass, int64_t n, double* px, double* py,
+                      double* pz, float* m);
+void launch_gather_xm(hipStream_t s, const uint32_t* perm, int64_t n, const double* px, const double* py,
+                      const double* pz, const float* m, float4* xm);
+void launch_acc_out(hipStream_t s, const uint32_t* perm, int64_t n, const float4* acc4, double* acc3, double* pot);

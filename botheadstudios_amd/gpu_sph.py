@@ -1,0 +1,221 @@
+"""Host-side mirror of the reference's GPU SPH stepper over the C ABI.
+
+`GpuSph` follows `GpuSph` in ie/crates/engine/src/gpu_sph.rs:742-922 method for method
+(new / upload / set_dt / set_av / encode_relax / encode_kdk / count / begin_readback / take_readback),
+and `GravityField.accelerations` follows ie/crates/engine/src/gpu_gravity.rs:556-578. Differences
+from the reference, by design:
+  - errors raise `IhsError` (the reference clamps silently and reports wgpu errors asynchronously);
+  - `set_dt(dt <= 0)` selects the device-side adaptive Courant dt (impact-run:403-410 read it back
+    to the host every step);
+  - the smoothing length can be solved per step (Newton) or seeded by the learned model.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from .ics import SPH_PARTICLE_DTYPE
+
+PRECISION = {"fp32": 0, "split": 1}
+H_MODE = {"fixed": 0, "newton": 1, "learned": 2}
+FIELDS = {"h": (0, np.float32, 1), "rho": (1, np.float32, 1), "rpc": (2, np.float32, 4), "asph": (3, np.float32, 4),
+          "agrav": (4, np.float32, 4), "hiter": (5, np.uint32, 1), "ncount": (6, np.uint32, 1),
+          "feat": (7, np.float32, 4), "pos64": (8, np.float64, 3), "hs": (9, np.float32, 1)}
+IHS_EAGAIN = -11
+
+
+class IhsError(RuntimeError):
+    pass
+
+
+def eos_records(eos) -> np.ndarray:
+    """(n_mat, 10) Tillotson table -> the 48-byte SphEos records."""
+    e = np.asarray(eos, dtype=np.float64)
+    if e.ndim == 2 and e.shape[1] == 12 and e.dtype == np.float32:
+        return np.ascontiguousarray(e)
+    out = np.zeros((e.shape[0], 12), np.float32)
+    out[:, :10] = e[:, :10]
+    return out
+
+
+def default_config() -> _lib.IhsConfig:
+    cfg = _lib.IhsConfig()
+    _lib.load().ihs_default_config(C.byref(cfg))
+    return cfg
+
+
+class GpuSph:
+    def __init__(self, capacity: int, device: int = 0, leaf_size: int = 16, precision: str = "fp32",
+                 h_mode: str = "newton", theta: float = 0.5, cfl: float = 0.1, n_ngb: float = 48.0,
+                 h_tol: float = 1e-4, h_max_iter: int = 40, h_margin: float = 1.25, h_cap_factor: float = 4.0,
+                 nbr_capacity: int = 192, use_graph: bool = True):
+        self._L = _lib.load()
+        cfg = default_config()
+        cfg.capacity = int(capacity)
+        cfg.leaf_size = int(leaf_size)
+        cfg.precision = PRECISION[precision]
+        cfg.h_mode = H_MODE[h_mode]
+        cfg.theta, cfg.cfl, cfg.n_ngb, cfg.h_tol = float(theta), float(cfl), float(n_ngb), float(h_tol)
+        cfg.h_max_iter, cfg.h_margin, cfg.h_cap_factor = int(h_max_iter), float(h_margin), float(h_cap_factor)
+        cfg.nbr_capacity, cfg.use_graph = int(nbr_capacity), int(bool(use_graph))
+        self.config = cfg
+        h = C.c_void_p()
+        rc = self._L.ihs_create(int(device), C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise IhsError(f"ihs_create failed ({rc}); check the device index and configuration")
+        self._h = h
+        self._pending = None
+
+    # ---- lifetime ----
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ihs_destroy(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            msg = self._L.ihs_last_error(self._h)
+            raise IhsError(f"ihs error {rc}: {msg.decode() if msg else ''}")
+        return rc
+
+    # ---- GpuSph surface ----
+    def upload(self, particles: np.ndarray, eos, softening: float):
+        p = np.ascontiguousarray(particles, dtype=SPH_PARTICLE_DTYPE)
+        e = eos_records(eos)
+        self._check(self._L.ihs_upload(self._h, p.ctypes.data, p.shape[0], e.ctypes.data, e.shape[0],
+                                       float(softening)))
+
+    def set_dt(self, dt: float, damp: float = 1.0):
+        self._check(self._L.ihs_set_dt(self._h, float(dt), float(damp)))
+
+    def set_av(self, alpha: float, beta: float):
+        self._check(self._L.ihs_set_av(self._h, float(alpha), float(beta)))
+
+    def set_omega(self, omega: float):
+        self._check(self._L.ihs_set_omega(self._h, float(omega)))
+
+    def encode_kdk(self, substeps: int):
+        self._check(self._L.ihs_step_kdk(self._h, int(substeps)))
+
+    def encode_relax(self, steps: int):
+        self._check(self._L.ihs_relax(self._h, int(steps)))
+
+    def force_eval(self):
+        self._check(self._L.ihs_force_eval(self._h))
+
+    def count(self) -> int:
+        return int(self._L.ihs_count(self._h))
+
+    def sync(self):
+        self._check(self._L.ihs_sync(self._h))
+
+    def begin_readback(self):
+        self._pending = np.zeros(self.count(), SPH_PARTICLE_DTYPE)
+
+    def take_readback(self):
+        """The particles if the stream is idle, else None (gpu_store.rs:147-152 semantics)."""
+        if self._pending is None:
+            return None
+        rc = self._L.ihs_readback(self._h, self._pending.ctypes.data, self._pending.shape[0], 0)
+        if rc == IHS_EAGAIN:
+            return None
+        self._check(rc)
+        out, self._pending = self._pending, None
+        return out
+
+    def readback(self) -> np.ndarray:
+        out = np.zeros(self.count(), SPH_PARTICLE_DTYPE)
+        self._check(self._L.ihs_readback(self._h, out.ctypes.data, out.shape[0], 1))
+        return out
+
+    # ---- diagnostics ----
+    def stats(self, reset: bool = False) -> dict:
+        s = _lib.IhsStats()
+        self._check(self._L.ihs_stats_get(self._h, C.byref(s), int(bool(reset))))
+        return {
+            "ke": s.ke, "ie": s.ie, "pe": s.pe, "etot": s.ke + s.ie + s.pe, "p": list(s.p), "L": list(s.L), "t": s.t,
+            "dt": s.dt, "steps": s.steps, "force_evals": s.force_evals, "mean_h_iters": s.mean_h_iters,
+            "mean_h_evals": s.mean_h_evals, "mean_neighbours": s.mean_neighbours, "nbr_overflow": s.nbr_overflow,
+            "walk_overflow": s.walk_overflow, "h_capped": s.h_capped, "h_unconverged": s.h_unconverged,
+            "nonfinite": s.nonfinite, "grav_cell_interactions": s.grav_cell_interactions,
+            "grav_pp_interactions": s.grav_pp_interactions, "nbr_candidates": s.nbr_candidates,
+            "walk_nodes": s.walk_nodes,
+        }
+
+    def profile(self, enable: bool = True):
+        """Per-stage HIP-event timing (steps launch without the graph while enabled)."""
+        self._check(self._L.ihs_profile(self._h, int(bool(enable))))
+
+    def profile_read(self, reset: bool = False) -> dict:
+        ms = np.zeros(len(_lib.STAGES), np.float64)
+        calls = np.zeros(len(_lib.STAGES), np.int64)
+        self._check(self._L.ihs_profile_read(self._h, ms.ctypes.data, calls.ctypes.data, int(bool(reset))))
+        return {name: (float(ms[i]), int(calls[i])) for i, name in enumerate(_lib.STAGES)}
+
+    def field(self, name: str) -> np.ndarray:
+        fid, dt, width = FIELDS[name]
+        n = self.count()
+        out = np.zeros((n, width) if width > 1 else n, dt)
+        self._check(self._L.ihs_debug_field(self._h, fid, out.ctypes.data))
+        return out
+
+    def debug_tree(self) -> dict:
+        n = self.count()
+        k = self.config.leaf_size
+        L = (n + k - 1) // k
+        sc = np.zeros(n, np.uint32)
+        order = np.zeros(n, np.uint32)
+        cc = np.zeros(L, np.uint32)
+        left = np.zeros(2 * L - 1, np.uint32)
+        right = np.zeros(2 * L - 1, np.uint32)
+        parent = np.zeros(2 * L - 1, np.uint32)
+        got = self._check(self._L.ihs_debug_tree(self._h, sc.ctypes.data, order.ctypes.data, cc.ctypes.data,
+                                                 left.ctypes.data, right.ctypes.data, parent.ctypes.data))
+        assert got == L
+        return dict(sorted_codes=sc, order=order, ccodes=cc, left=left, right=right, parent=parent)
+
+    # ---- GravityField ----
+    def gravity(self, pos, mass, softening: float, theta: float = 0.5, mode: str = "tree", with_pot: bool = False):
+        pos = np.ascontiguousarray(pos, np.float64)
+        mass = np.ascontiguousarray(mass, np.float64)
+        n = pos.shape[0]
+        acc = np.zeros((n, 3), np.float64)
+        pot = np.zeros(n, np.float64) if with_pot else None
+        self._check(self._L.ihs_gravity(self._h, pos.ctypes.data, mass.ctypes.data, n, float(softening), float(theta),
+                                        1 if mode == "tree" else 0, acc.ctypes.data,
+                                        pot.ctypes.data if with_pot else None))
+        return (acc, pot) if with_pot else acc
+
+
+class GravityField:
+    """gpu_gravity.rs:532-626: direct sum below the knee, LBVH tree at or above it (native: synchronous)."""
+
+    TREE_KNEE = 24_000  # gpu_gravity.rs:50
+    THETA = 0.5         # gpu_gravity.rs:55
+
+    def __init__(self, capacity: int, device: int = 0, leaf_size: int = 16):
+        self._sph = GpuSph(capacity, device=device, leaf_size=leaf_size, h_mode="fixed", nbr_capacity=1,
+                           use_graph=False)
+
+    def accelerations(self, pos, mass, softening: float):
+        if len(pos) == 0:
+            return np.zeros((0, 3))
+        mode = "tree" if len(pos) >= self.TREE_KNEE else "direct"
+        return self._sph.gravity(pos, mass, softening, self.THETA, mode)
+
+    def close(self):
+        self._sph.close()

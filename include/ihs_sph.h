@@ -1,0 +1,167 @@
+/*
+ * libihs_sph — MI355X (gfx950) SPH + self-gravity giant-impact step behind a C ABI.
+ *
+ * The entry points mirror the reference's step interface, `GpuSph`
+ * (integrity-engine/crates/engine/src/gpu_sph.rs:742-922), and its gravity backend, `GravityField` /
+ * `GpuGravity` (integrity-engine/crates/engine/src/gpu_gravity.rs:181-230,532-626). Conventions:
+ *   - every function returns an int status (IHS_OK = 0, < 0 = error) unless documented otherwise,
+ *     never aborts, and leaves a message for ihs_last_error();
+ *   - one handle = one HIP device + one stream; thread-compatible, not thread-safe (like GpuSph);
+ *   - the caller owns host memory, the library owns device memory;
+ *   - particles cross the boundary as the reference's 48-byte AoS record; the device keeps SoA.
+ */
+#ifndef IHS_SPH_H
+#define IHS_SPH_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IHS_ABI_VERSION 1
+
+#define IHS_OK 0
+#define IHS_ERR_ARG (-1)
+#define IHS_ERR_HIP (-2)
+#define IHS_ERR_CAPACITY (-3)
+#define IHS_ERR_STATE (-4)
+#define IHS_EAGAIN (-11)
+
+/* SphParticle, gpu_sph.rs:25-37 (std430, 48 bytes). */
+typedef struct {
+  float pos[3];
+  float h;
+  float vel[3];
+  float u;
+  float mass;
+  uint32_t mat;
+  float rho;
+  uint32_t prov;
+} ihs_particle;
+
+/* SphEos, gpu_sph.rs:39-55 (48 bytes): Tillotson rho0, a, b, A, B, E0, E_iv, E_cv, alpha, beta. */
+typedef struct {
+  float rho0, a, b, cap_a, cap_b, e0, e_iv, e_cv, alpha, beta, _p0, _p1;
+} ihs_eos;
+
+enum { IHS_PREC_FP32 = 0, IHS_PREC_SPLIT = 1 };
+/* Smoothing length: FIXED keeps the uploaded h (the reference's behaviour, hydrostatic.rs:58-61);
+ * NEWTON solves rho~(h) = m (eta/h)^3 for n_ngb neighbours each force evaluation; LEARNED seeds
+ * that Newton solve with the built-in network (Newton stays the judge). */
+enum { IHS_H_FIXED = 0, IHS_H_NEWTON = 1, IHS_H_LEARNED = 2 };
+enum { IHS_GRAVITY_DIRECT = 0, IHS_GRAVITY_TREE = 1 };
+
+typedef struct {
+  int32_t capacity;      /* max particles (GpuSph::new capacity) */
+  int32_t leaf_size;     /* LBVH leaf bucket K: 1,2,4,...,64 (default 16) */
+  int32_t precision;     /* IHS_PREC_* (default FP32) */
+  int32_t h_mode;        /* IHS_H_* (default NEWTON) */
+  float theta;           /* Barnes-Hut opening angle (0.5) */
+  float cfl;             /* Courant factor for adaptive dt (0.1, impact-run:583) */
+  float n_ngb;           /* target neighbours within h (48) */
+  float h_tol;           /* density-consistency tolerance |f|/rho (1e-4) */
+  int32_t h_max_iter;    /* Newton iteration cap (40) */
+  float h_margin;        /* neighbour search radius = h_margin * previous h (1.25) */
+  float h_cap_factor;    /* search radius never exceeds h_cap_factor * max uploaded h (4) */
+  int32_t nbr_capacity;  /* neighbour-list entries per particle (192) */
+  int32_t use_graph;     /* capture the KDK step in a hipGraph (1) */
+  int32_t reserved[7];
+} ihs_config;
+
+typedef struct ihs_handle ihs_handle;
+
+typedef struct {
+  double ke, ie, pe;      /* FP64: kinetic, internal, gravitational (1/2 sum m phi, tree potential) */
+  double p[3];            /* FP64 momentum */
+  double L[3];            /* FP64 angular momentum about the origin */
+  double t;               /* simulated time */
+  double dt;              /* last step's dt */
+  int64_t steps;          /* integration steps taken */
+  int64_t force_evals;    /* force evaluations since the last counter reset */
+  double mean_h_iters;    /* Newton updates per particle per force evaluation */
+  double mean_h_evals;    /* density evaluations per particle per force evaluation */
+  double mean_neighbours; /* neighbour-list entries per particle */
+  uint32_t nbr_overflow, walk_overflow, h_capped, h_unconverged, nonfinite, _pad;
+  int64_t grav_cell_interactions; /* target x accepted-node interactions evaluated (since reset) */
+  int64_t grav_pp_interactions;   /* target x leaf-particle interactions evaluated (since reset) */
+  int64_t nbr_candidates;         /* target x candidate tests of the neighbour search (since reset) */
+  int64_t walk_nodes;             /* tree nodes popped by all group walks (since reset) */
+} ihs_stats;
+
+/* Pipeline stages timed by ihs_profile (HIP events on the handle's stream). */
+enum {
+  IHS_STAGE_SORT = 0,    /* bbox, Morton keys, radix sort, tie order */
+  IHS_STAGE_GATHER = 1,  /* state permutation into Morton order */
+  IHS_STAGE_TREE = 2,    /* Karras build + upsweep */
+  IHS_STAGE_NBR = 3,     /* neighbour lists */
+  IHS_STAGE_HSOLVE = 4,  /* learned seed + Newton h-solve */
+  IHS_STAGE_DENSITY = 5, /* density + EOS + Courant signal */
+  IHS_STAGE_FORCES = 6,  /* SPH forces + du/dt */
+  IHS_STAGE_GRAVITY = 7, /* Barnes-Hut walk */
+  IHS_STAGE_KICK = 8,    /* kick / drift */
+  IHS_STAGE_COUNT = 9
+};
+
+/* Debug/inspection fields for ihs_debug_field (all returned in particle-id order). */
+enum {
+  IHS_FIELD_H = 0,       /* float */
+  IHS_FIELD_RHO = 1,     /* float */
+  IHS_FIELD_RPC = 2,     /* float4: rho, P/rho^2, c, P */
+  IHS_FIELD_ASPH = 3,    /* float4: SPH acceleration xyz, du/dt */
+  IHS_FIELD_AGRAV = 4,   /* float4: gravitational acceleration xyz, potential */
+  IHS_FIELD_HITER = 5,   /* uint32: Newton updates in the last h-solve */
+  IHS_FIELD_NCOUNT = 6,  /* uint32: neighbour-list length */
+  IHS_FIELD_FEAT = 7,    /* float4: n_ngb within h, div v, |grad rho|, 0 */
+  IHS_FIELD_POS64 = 8,   /* double[3] */
+  IHS_FIELD_HS = 9       /* float: searched radius */
+};
+
+void ihs_default_config(ihs_config* cfg);
+int ihs_abi_version(void);
+/* GpuSph::new (gpu_sph.rs:767). */
+int ihs_create(int device, const ihs_config* cfg, ihs_handle** out);
+int ihs_destroy(ihs_handle* h);
+const char* ihs_last_error(const ihs_handle* h);
+/* GpuSph::upload (gpu_sph.rs:839-850): n beyond capacity is an error (the reference clamps silently). */
+int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* eos, int32_t n_mat, float softening);
+/* GpuSph::set_dt (gpu_sph.rs:853-857). dt <= 0 selects the adaptive Courant dt (impact-run:403-410). */
+int ihs_set_dt(ihs_handle* h, double dt, float damp);
+/* GpuSph::set_av (gpu_sph.rs:864-868). */
+int ihs_set_av(ihs_handle* h, float alpha, float beta);
+/* SphParams::omega for relaxation in a rotating frame (gpu_sph.rs:68-76, sph_step.wgsl:252-256). */
+int ihs_set_omega(ihs_handle* h, float omega);
+/* GpuSph::encode_kdk (gpu_sph.rs:901-908): substeps KDK steps, two force evaluations each. Enqueued on
+ * the handle's stream; returns without waiting. */
+int ihs_step_kdk(ihs_handle* h, int32_t substeps);
+/* GpuSph::encode_relax (gpu_sph.rs:893-898): steps x (force evaluation + damped kick/drift). */
+int ihs_relax(ihs_handle* h, int32_t steps);
+/* One force evaluation at the current state, no integration (for inspection). */
+int ihs_force_eval(ihs_handle* h);
+/* begin_readback/take_readback (gpu_store.rs:126-152). blocking=0 returns IHS_EAGAIN while the
+ * stream is busy; otherwise the particle count written (<= cap), in upload order. */
+int64_t ihs_readback(ihs_handle* h, ihs_particle* out, int64_t cap, int32_t blocking);
+/* GpuSph::count (gpu_sph.rs:870). */
+int64_t ihs_count(const ihs_handle* h);
+int ihs_sync(ihs_handle* h);
+/* FP64 energy / momentum reductions and solver counters (blocks). reset=1 clears the counters. */
+int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset);
+/* GravityField::accelerations / GpuGravity::accelerations[_tree] (gpu_gravity.rs:181-230,556-578):
+ * softened self-gravity on n bodies (pos3 and acc3 are n*3 doubles). mode = IHS_GRAVITY_*;
+ * pot (may be NULL) receives the potential. Synchronous. Uses the handle's device and leaf size. */
+int ihs_gravity(ihs_handle* h, const double* pos3, const double* mass, int64_t n, double softening, double theta,
+                int32_t mode, double* acc3, double* pot);
+/* Inspection for parity tests: the LBVH of the last force evaluation. sorted_codes/order have n
+ * entries (order = particle ids in Morton order), ccodes L, left/right/parent 2L-1 (children of
+ * leaves are 0, the root's parent is 0xffffffff). Returns L. */
+int64_t ihs_debug_tree(ihs_handle* h, uint32_t* sorted_codes, uint32_t* order, uint32_t* ccodes, uint32_t* left,
+                       uint32_t* right, uint32_t* parent);
+int ihs_debug_field(ihs_handle* h, int32_t field, void* out);
+/* Per-stage timing: enable=1 makes ihs_step_kdk launch directly (no graph) with HIP events around every
+ * stage; ihs_profile_read returns accumulated milliseconds and launch counts per IHS_STAGE_*. */
+int ihs_profile(ihs_handle* h, int32_t enable);
+int ihs_profile_read(ihs_handle* h, double* ms, int64_t* calls, int32_t reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

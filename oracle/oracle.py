@@ -86,6 +86,7 @@ def _declare(L):
     L.or_sort_pairs.argtypes = [_up, i64, _up, _up]
     L.or_cluster_codes.argtypes = [_up, i64, i64, _up]
     L.or_karras.argtypes = [_up, i64, _up, _up, _up]
+    L.or_karras_ex.argtypes = [_up, i64, i32, _up, _up, _up]
     L.or_upsweep_f32.argtypes = [_fp, i64, i64, _up, _up, i64, _fp, _fp, _fp]
     L.or_bh_walk_f32.argtypes = [_fp, _up, i64, i64, _up, _up, _fp, _fp, _fp, i64, f32, f32, _fp]
     L.or_direct_f64.argtypes = [_dp, _dp, i64, d, C.c_void_p, i64, _dp, C.c_void_p, C.c_int]
@@ -175,21 +176,23 @@ def cluster_codes(sorted_codes, k):
     return cc
 
 
-def karras(ccodes):
+def karras(ccodes, fix_dups=False):
+    """fix_dups=False: the reference's split (bh_gravity.wgsl:159-160) verbatim; True: the
+    duplicate-run split consistent with the index-augmented ranges (what the GPU builds)."""
     ccodes = np.ascontiguousarray(ccodes, np.uint32)
     L = ccodes.shape[0]
     nn = 2 * L - 1
     left = np.zeros(nn, np.uint32)
     right = np.zeros(nn, np.uint32)
     parent = np.zeros(nn, np.uint32)
-    lib().or_karras(ccodes, L, left, right, parent)
+    lib().or_karras_ex(ccodes, L, int(bool(fix_dups)), left, right, parent)
     return left, right, parent
 
 
 class Lbvh:
     """The reference's CPU Morton leg + Karras tree + moments (gpu_gravity.rs:318-478)."""
 
-    def __init__(self, xyz32, mass32, k):
+    def __init__(self, xyz32, mass32, k, fix_dups=True):
         xyz32 = _f32(xyz32)
         n = xyz32.shape[0]
         self.n, self.k = n, k
@@ -198,7 +201,7 @@ class Lbvh:
         self.sorted_codes, self.order = sort_pairs(self.codes)
         self.ccodes = cluster_codes(self.sorted_codes, k)
         self.L = self.ccodes.shape[0]
-        self.left, self.right, self.parent = karras(self.ccodes)
+        self.left, self.right, self.parent = karras(self.ccodes, fix_dups)
         body = np.zeros((n, 4), np.float32)
         body[:, :3] = xyz32
         body[:, 3] = mass32

@@ -395,6 +395,14 @@ void or_cluster_codes(const uint32_t* sorted_codes, int64_t n, int64_t k, uint32
 // ie/shaders/bh_gravity.wgsl:114-184 — Karras 2012 radix tree over L cluster codes. Internal nodes
 // [0, L-1), leaves [L-1, 2L-1), root 0; parent of the root = 0xffffffff.
 void or_karras(const uint32_t* cc, int64_t L, uint32_t* left, uint32_t* right, uint32_t* parent) {
+  or_karras_ex(cc, L, 0, left, right, parent);
+}
+
+// fix_dups = 1: for a range of equal cluster codes split at the highest differing bit of the
+// index-augmented key (the split consistent with delta()'s ranges) instead of the reference's
+// (first + last) >> 1, which yields an invalid tree for runs of three or more equal codes.
+void or_karras_ex(const uint32_t* cc, int64_t L, int32_t fix_dups, uint32_t* left, uint32_t* right,
+                  uint32_t* parent) {
   const int64_t nn = 2 * L - 1;
   for (int64_t i = 0; i < nn; ++i) {
     left[i] = 0;
@@ -419,7 +427,12 @@ void or_karras(const uint32_t* cc, int64_t L, uint32_t* left, uint32_t* right, u
     int64_t split = first;
     const uint32_t fc = cc[first], lc = cc[last];
     if (fc == lc) {
-      split = (first + last) >> 1;
+      if (fix_dups) {
+        const int p = 31 - clz32((uint32_t)first ^ (uint32_t)last);
+        split = (int64_t)((((uint32_t)first >> p) + 1u) << p) - 1;
+      } else {
+        split = (first + last) >> 1;
+      }
     } else {
       const int cpfx = clz32(fc ^ lc);
       int64_t step = last - first;

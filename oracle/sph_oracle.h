@@ -56,6 +56,9 @@ void or_morton_f32(const float* xyz, int64_t n, const float* lo3, const float* h
 void or_sort_pairs(const uint32_t* codes, int64_t n, uint32_t* sorted_codes, uint32_t* order);
 void or_cluster_codes(const uint32_t* sorted_codes, int64_t n, int64_t k, uint32_t* ccodes);
 void or_karras(const uint32_t* ccodes, int64_t L, uint32_t* left, uint32_t* right, uint32_t* parent);
+/* fix_dups = 1: the duplicate-run split consistent with the index-augmented ranges (see .cpp). */
+void or_karras_ex(const uint32_t* ccodes, int64_t L, int32_t fix_dups, uint32_t* left, uint32_t* right,
+                  uint32_t* parent);
 /* Moments over the Karras arena (2L-1 nodes): com4 = (com xyz, mass), bmin3, bmax3 (f32).
  * sbody = bodies permuted to sorted order, n*4 floats (x,y,z,m). */
 void or_upsweep_f32(const float* sbody, int64_t n, int64_t k, const uint32_t* left, const uint32_t* right,

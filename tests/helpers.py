@@ -1,0 +1,78 @@
+"""Shared test inputs: the reference verifiers' configurations, restated."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from botheadstudios_amd.ics import SPH_PARTICLE_DTYPE
+from botheadstudios_amd.materials import eos_table
+
+
+def splitmix_stream(seed: int, count: int) -> np.ndarray:
+    """bhtree.rs:197-203 / gpu-bh-verify:116-122 splitmix64 -> uniform [0,1), sequential state."""
+    out = np.empty(count, np.float64)
+    s = seed & 0xFFFFFFFFFFFFFFFF
+    M = 0xFFFFFFFFFFFFFFFF
+    for k in range(count):
+        s = (s + 0x9E3779B97F4A7C15) & M
+        z = s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+        out[k] = ((z ^ (z >> 31)) >> 11) / float(1 << 53)
+    return out
+
+
+def bh_cloud(n: int = 1500, seed: int = 0xABCD1234):
+    """gpu-bh-verify:124-137 build_cloud: positions in a 1e6 m box, masses 1e18*(0.5+u)."""
+    r = splitmix_stream(seed, 4 * n).reshape(n, 4)
+    pos = r[:, :3] * 1.0e6
+    mass = 1.0e18 * (0.5 + r[:, 3])
+    return pos, mass
+
+
+def rnd(i: int, salt: int) -> float:
+    """sph-verify:253-258."""
+    M = 0xFFFFFFFFFFFFFFFF
+    x = ((i + salt) * 0x9E3779B97F4A7C15) & M
+    x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & M
+    return ((x >> 40) & 0xFFFF) / 32768.0 - 1.0
+
+
+def sph_verify_config(n: int = 300):
+    """sph-verify:260-289 build_config: Fibonacci cluster r0 = 1e5 m, iron core (mat 1) / basalt mantle
+    (mat 0), h = 2.6 spacing, u = 1e6, velocities 200*rnd. Returns (particles48, eos[basalt, iron])."""
+    r0 = 1.0e5
+    vol_per = 4.0 / 3.0 * math.pi * r0 ** 3 / n
+    spacing = vol_per ** (1.0 / 3.0)
+    h = np.float32(2.6 * spacing)
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    p = np.zeros(n, SPH_PARTICLE_DTYPE)
+    for i in range(n):
+        rr = r0 * ((i + 0.5) / n) ** (1.0 / 3.0)
+        y = 1.0 - 2.0 * (i + 0.5) / n
+        rad = math.sqrt(max(1.0 - y * y, 0.0))
+        th = golden * i
+        p["pos"][i] = (math.cos(th) * rad * rr, y * rr, math.sin(th) * rad * rr)
+        mat = 1 if rr < 0.5 * r0 else 0
+        rho0 = 7850.0 if mat == 1 else 2700.0
+        p["h"][i] = h
+        p["vel"][i] = (200.0 * rnd(i, 1), 200.0 * rnd(i, 2), 200.0 * rnd(i, 3))
+        p["u"][i] = 1.0e6
+        p["mass"][i] = rho0 * vol_per
+        p["mat"][i] = mat
+        p["rho"][i] = rho0
+    return p, eos_table(["basalt", "iron"])
+
+
+def rms_rel(approx, exact):
+    """gpu-bh-verify:164-181: sqrt(sum |err|^2 / sum |ref|^2) and the worst per-particle error."""
+    approx = np.asarray(approx, np.float64)
+    exact = np.asarray(exact, np.float64)
+    e2 = ((approx - exact) ** 2).reshape(len(exact), -1).sum(1)
+    r2 = (exact ** 2).reshape(len(exact), -1).sum(1)
+    return math.sqrt(e2.sum() / max(r2.sum(), 1e-300)), float(np.max(np.sqrt(e2) / np.maximum(np.sqrt(r2), 1e-30)))
+
+
+def particles_from_ic(ic) -> np.ndarray:
+    return ic.particles48()
