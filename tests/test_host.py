@@ -1,0 +1,110 @@
+"""CPU: host logic (ICs, materials, layouts) and the C-ABI library surface (loads, exports every symbol the header
+declares; no device calls)."""
+import ctypes as C
+import math
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from botheadstudios_amd import _lib, ics
+from botheadstudios_amd.materials import sph_eos48
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_header_symbols_exported():
+    hdr = open(os.path.join(ROOT, "include", "ihs_sph.h")).read()
+    declared = set(re.findall(r"\b(ihs_[a-z_0-9]+)\s*\(", hdr))
+    assert declared == set(_lib.EXPORTS)
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (ihs_[a-z_0-9]+)", out))
+    assert declared <= exported, declared - exported
+    L = _lib.load()
+    for name in declared:
+        assert getattr(L, name) is not None
+
+
+def test_struct_layouts(tmp_path):
+    """The ctypes mirrors match the C header's layouts, compiled by the host C compiler."""
+    assert ics.SPH_PARTICLE_DTYPE.itemsize == 48  # gpu_sph.rs:25-37
+    assert sph_eos48(["granite"]).nbytes == 48   # gpu_sph.rs:39-55
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "ihs_sph.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(ihs_particle), sizeof(ihs_eos),'
+                   ' sizeof(ihs_config), sizeof(ihs_stats), offsetof(ihs_stats, grav_cell_interactions),'
+                   ' offsetof(ihs_config, use_graph)); return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
+    assert got == [48, 48, C.sizeof(_lib.IhsConfig), C.sizeof(_lib.IhsStats), _lib.IhsStats.grav_cell_interactions.offset,
+                   _lib.IhsConfig.use_graph.offset]
+    cfg = _lib.IhsConfig()
+    _lib.load().ihs_default_config(C.byref(cfg))
+    assert (cfg.leaf_size, cfg.theta, cfg.n_ngb) == (16, 0.5, 48.0)
+    assert cfg.cfl == pytest.approx(0.1) and cfg.h_tol == pytest.approx(1e-4)
+
+
+def test_hash_jitter_matches_the_reference_formula():
+    """impact-run:179-189 restated with Python integers."""
+    M = (1 << 64) - 1
+
+    def ref(run, i, axis):
+        x = ((run * 0x9E3779B97F4A7C15) & M) ^ ((i * 0xD1B54A32D192ED03) & M) ^ ((axis * 0xCA5A826395121157) & M) \
+            ^ 0x2545F4914F6CDD1D
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & M
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & M
+        x ^= x >> 31
+        return float(x) / float(M) * 2.0 - 1.0
+    idx = np.array([0, 1, 2, 12345, 999999], np.uint64)
+    for ax in range(3):
+        got = ics.hash_jitter(0x1A7E0001, idx, ax)
+        assert np.allclose(got, [ref(0x1A7E0001, int(i), ax) for i in idx], rtol=0, atol=1e-15)
+        assert np.all(np.abs(got) <= 1.0)
+
+
+def test_impact_ic_geometry():
+    ic = ics.config_ic(1)
+    meta = ic.meta
+    assert ic.n == 16384 and meta["n_target"] == 14746 and meta["n_impactor"] == 1638
+    assert meta["particle_mass"] == pytest.approx(3.645e20, rel=1e-3)
+    assert meta["r_target"] / 1e3 == pytest.approx(7270, rel=2e-3)
+    assert meta["r_impactor"] / 1e3 == pytest.approx(3495, rel=2e-3)
+    assert meta["v_esc"] == pytest.approx(8605, rel=2e-3)
+    # 30% iron by mass in each body.
+    t = ic.prov == 0
+    assert np.mean(ic.mat[t] == ics.MAT_IRON) == pytest.approx(0.3, abs=1e-3)
+    # Centre-of-mass frame.
+    p = (ic.m[:, None] * ic.vel).sum(0)
+    assert np.linalg.norm(p) / (ic.m.sum() * meta["v0"]) < 1e-6
+    # Energy: starting at 1.6 contacts with the infall-corrected speed reaches v_c at contact (two points).
+    G, M = ics.G, ic.m.sum()
+    mt, mi = ic.m[t].sum(), ic.m[~t].sum()
+    mu = mt * mi / M
+    v_rel = np.linalg.norm(ic.vel[~t].mean(0) - ic.vel[t].mean(0))
+    e0 = 0.5 * mu * v_rel ** 2 - G * mt * mi / meta["d0"]
+    ec = 0.5 * mu * meta["v_contact"] ** 2 - G * mt * mi / (meta["r_target"] + meta["r_impactor"])
+    assert e0 == pytest.approx(ec, rel=1e-6)
+    # Angular momentum fixes the impact angle at contact.
+    r_rel = ic.pos[~t].mean(0) - ic.pos[t].mean(0)
+    ell = np.linalg.norm(np.cross(r_rel, ic.vel[~t].mean(0) - ic.vel[t].mean(0)))
+    rc = meta["r_target"] + meta["r_impactor"]
+    assert math.degrees(math.asin(ell / (rc * meta["v_contact"]))) == pytest.approx(45.0, abs=0.5)
+
+
+def test_impact_ic_configs_sizes():
+    for k, (nt, ni) in {3: (898779, 149797), 4: (943718, 104858)}.items():
+        c = dict(ics.CONFIGS[k])
+        mt, mi = c["m_target"], c["m_impactor"]
+        assert int(round(c["n"] * mt / (mt + mi))) == nt and c["n"] - nt == ni
+    sw = ics.sweep_scenarios(64)
+    ns = np.array([s["n"] for s in sw])
+    assert len(sw) == 64 and ns.min() >= 4096 and ns.max() <= 131072
+    assert all(0.05 <= s["m_impactor"] <= 0.2 and 0 <= s["angle_deg"] <= 60 for s in sw)
+
+
+def test_product_fails_loudly_without_library(tmp_path):
+    with pytest.raises(RuntimeError, match="not built"):
+        _lib.load(str(tmp_path / "missing.so"))
