@@ -39,8 +39,13 @@ EXPORTS = [
     "ihs_default_config", "ihs_abi_version", "ihs_create", "ihs_destroy", "ihs_last_error", "ihs_upload",
     "ihs_set_dt", "ihs_set_av", "ihs_set_omega", "ihs_step_kdk", "ihs_relax", "ihs_force_eval", "ihs_readback",
     "ihs_count", "ihs_sync", "ihs_stats_get", "ihs_gravity", "ihs_debug_tree", "ihs_debug_field",
-    "ihs_profile", "ihs_profile_read",
+    "ihs_profile", "ihs_profile_read", "ihs_mlp_set_weights", "ihs_mlp_get_weights", "ihs_mlp_record",
+    "ihs_mlp_samples", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict",
 ]
+
+MLP_NPARAM = 1313
+MLP_IN = 6
+MLP_SAMPLE = 8
 
 
 def load(path: str | None = None):
@@ -80,9 +85,18 @@ def load(path: str | None = None):
     L.ihs_debug_field.argtypes = [vp, i32, vp]
     L.ihs_profile.argtypes = [vp, i32]
     L.ihs_profile_read.argtypes = [vp, vp, vp, i32]
+    L.ihs_mlp_set_weights.argtypes = [vp, vp, i64]
+    L.ihs_mlp_get_weights.argtypes = [vp, vp, i64]
+    L.ihs_mlp_record.argtypes = [vp, i64]
+    L.ihs_mlp_samples.argtypes = [vp, vp, i64]
+    L.ihs_mlp_samples.restype = i64
+    L.ihs_mlp_train.argtypes = [vp, vp, i64, i32, i32, f32, C.c_uint64, vp]
+    L.ihs_mlp_gradient.argtypes = [vp, vp, vp, i64, vp]
+    L.ihs_mlp_predict.argtypes = [vp, vp, i64, vp]
     for name in ("ihs_create", "ihs_destroy", "ihs_upload", "ihs_set_dt", "ihs_set_av", "ihs_set_omega",
                  "ihs_step_kdk", "ihs_relax", "ihs_force_eval", "ihs_sync", "ihs_stats_get", "ihs_gravity",
-                 "ihs_debug_field", "ihs_profile", "ihs_profile_read"):
+                 "ihs_debug_field", "ihs_profile", "ihs_profile_read", "ihs_mlp_set_weights", "ihs_mlp_get_weights",
+                 "ihs_mlp_record", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict"):
         getattr(L, name).restype = C.c_int
     if path is None:
         _lib = L

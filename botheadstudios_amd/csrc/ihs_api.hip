@@ -224,7 +224,9 @@ void prof_flush(ihs_handle* h) {
 bool valid_leaf(int k) { return k >= 1 && k <= 64 && (k & (k - 1)) == 0; }
 
 // One force evaluation: sort, permute, tree, neighbour lists, h-solve, density/EOS, forces, gravity.
-void force_eval(ihs_handle* h) {
+// after_drift: the positions moved since the previous evaluation (the second evaluation of a KDK step), so the
+// learned seed applies and a recording run stores (features, converged h) samples.
+void force_eval(ihs_handle* h, bool after_drift) {
   hipStream_t s = h->stream;
   const int64_t n = h->n;
   const int K = h->cfg.leaf_size;
@@ -249,10 +251,13 @@ void force_eval(ihs_handle* h) {
   launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags, sc.counters);
   prof_mark(h, IHS_STAGE_NBR);
   const float eta = std::cbrt(3.0f * h->cfg.n_ngb / (4.0f * kPi));
+  const bool record = h->mlp.record && after_drift && h->cfg.h_mode != IHS_H_FIXED;
   if (h->cfg.h_mode != IHS_H_FIXED) {
-    if (h->cfg.h_mode == IHS_H_LEARNED) mlp_seed_h(s, h->mlp, n, D, sc.hs);
+    if (record) mlp_record_features(s, h->mlp, n, D, sc.eos, sc.ctl, eta);
+    if (h->cfg.h_mode == IHS_H_LEARNED && after_drift) mlp_seed_h(s, h->mlp, n, D, sc.hs, sc.eos, sc.ctl, eta);
     launch_hsolve(s, n, sc.xm, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta * eta * eta, h->cfg.h_tol,
                   h->cfg.h_max_iter, sc.flags, sc.counters);
+    if (record) mlp_record_targets(s, h->mlp, n, D);
     prof_mark(h, IHS_STAGE_HSOLVE);
   }
   const int feat = h->cfg.h_mode == IHS_H_LEARNED || h->mlp.record;
@@ -264,19 +269,49 @@ void force_eval(ihs_handle* h) {
   prof_mark(h, IHS_STAGE_FORCES);
   launch_gravity_walk(s, n, K, kahan, h->softening * h->softening, sc.xm, h->tb, sc.accg, sc.flags, sc.counters);
   prof_mark(h, IHS_STAGE_GRAVITY);
-  if (h->mlp.record) mlp_record(s, h->mlp, n, D);
   h->force_evals++;
   h->have_fe = true;
 }
 
 void kdk_step(ihs_handle* h) {
-  force_eval(h);
+  force_eval(h, false);
   launch_kick_drift(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, h->sc.flags);
   prof_mark(h, IHS_STAGE_KICK);
-  force_eval(h);
+  force_eval(h, true);
   launch_kick(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl);
   prof_mark(h, IHS_STAGE_KICK);
   prof_flush(h);
+}
+
+uint64_t splitmix64(uint64_t& x) {
+  uint64_t z = (x += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+uint64_t gcd64(uint64_t a, uint64_t b) {
+  while (b) {
+    const uint64_t t = a % b;
+    a = b;
+    b = t;
+  }
+  return a;
+}
+
+// Copies host floats to a temporary device buffer; returns nullptr and sets the error on failure.
+float* upload_tmp(ihs_handle* h, const float* src, size_t floats) {
+  float* d = nullptr;
+  if (hipMalloc(&d, std::max<size_t>(floats, 1) * sizeof(float)) != hipSuccess) {
+    fail(h, IHS_ERR_HIP, "temporary allocation of %zu floats failed", floats);
+    return nullptr;
+  }
+  if (hipMemcpy(d, src, floats * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+    hipFree(d);
+    fail(h, IHS_ERR_HIP, "upload of %zu floats failed", floats);
+    return nullptr;
+  }
+  return d;
 }
 
 void drop_graph(ihs_handle* h) {
@@ -396,7 +431,7 @@ int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* e
     for (int pass = 0; pass < 16; ++pass) {
       uint32_t before = 0, after = 0;
       HIP_TRY(h, hipMemcpy(&before, h->sc.flags + FLAG_H_CAPPED, 4, hipMemcpyDeviceToHost));
-      force_eval(h);
+      force_eval(h, false);
       HIP_TRY(h, hipStreamSynchronize(h->stream));
       HIP_TRY(h, hipMemcpy(&after, h->sc.flags + FLAG_H_CAPPED, 4, hipMemcpyDeviceToHost));
       const uint32_t capped = after - before;
@@ -457,7 +492,7 @@ int ihs_force_eval(ihs_handle* h) {
   if (!h) return IHS_ERR_ARG;
   if (h->n < 1) return fail(h, IHS_ERR_STATE, "no particles uploaded");
   hipSetDevice(h->device);
-  force_eval(h);
+  force_eval(h, false);
   HIP_TRY(h, hipGetLastError());
   return IHS_OK;
 }
@@ -496,7 +531,7 @@ int ihs_relax(ihs_handle* h, int32_t steps) {
   if (h->n < 1) return fail(h, IHS_ERR_STATE, "no particles uploaded");
   hipSetDevice(h->device);
   for (int i = 0; i < steps; ++i) {
-    force_eval(h);
+    force_eval(h, false);
     launch_relax(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl);
   }
   HIP_TRY(h, hipGetLastError());
@@ -698,6 +733,154 @@ int ihs_debug_field(ihs_handle* h, int32_t field, void* out) {
   }
   HIP_TRY(h, hipStreamSynchronize(s));
   HIP_TRY(h, hipMemcpy(out, tmp, bytes, hipMemcpyDeviceToHost));
+  return IHS_OK;
+}
+
+// ---- learned smoothing-length seed ----
+
+int ihs_mlp_set_weights(ihs_handle* h, const float* w, int64_t nparam) {
+  if (!h || !w) return IHS_ERR_ARG;
+  if (nparam != MLP_NPARAM) return fail(h, IHS_ERR_ARG, "expected %d parameters, got %lld", MLP_NPARAM, (long long)nparam);
+  for (int64_t i = 0; i < nparam; ++i)
+    if (!std::isfinite(w[i])) return fail(h, IHS_ERR_ARG, "parameter %lld is not finite", (long long)i);
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  drop_graph(h);  // captured steps bake in whether the seed runs
+  MlpDev& m = h->mlp;
+  HIP_TRY(h, hipMemcpy(m.w, w, MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemset(m.adam_m, 0, MLP_NPARAM * sizeof(float)));
+  HIP_TRY(h, hipMemset(m.adam_v, 0, MLP_NPARAM * sizeof(float)));
+  m.adam_t = 0;
+  mlp_sync_bf16(h->stream, m);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipGetLastError());
+  m.trained = true;
+  return IHS_OK;
+}
+
+int ihs_mlp_get_weights(ihs_handle* h, float* w, int64_t nparam) {
+  if (!h || !w) return IHS_ERR_ARG;
+  if (nparam < MLP_NPARAM) return fail(h, IHS_ERR_ARG, "need room for %d parameters", MLP_NPARAM);
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipMemcpy(w, h->mlp.w, MLP_NPARAM * sizeof(float), hipMemcpyDeviceToHost));
+  return IHS_OK;
+}
+
+int ihs_mlp_record(ihs_handle* h, int64_t max_samples) {
+  if (!h || max_samples < 0) return IHS_ERR_ARG;
+  if (max_samples > 0 && h->cfg.h_mode == IHS_H_FIXED)
+    return fail(h, IHS_ERR_STATE, "recording needs a Newton h-solve (h_mode NEWTON or LEARNED)");
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  MlpDev& m = h->mlp;
+  if (max_samples == 0) {
+    m.record = false;
+    return IHS_OK;
+  }
+  if (max_samples > m.sample_cap) {
+    if (m.samples) HIP_TRY(h, hipFree(m.samples));
+    m.samples = nullptr;
+    m.sample_cap = 0;
+    HIP_TRY(h, hipMalloc(&m.samples, (size_t)max_samples * MLP_SAMPLE * sizeof(float)));
+    m.sample_cap = max_samples;
+  }
+  m.nsamples = 0;
+  m.record = true;
+  return IHS_OK;
+}
+
+int64_t ihs_mlp_samples(ihs_handle* h, float* out, int64_t cap) {
+  if (!h || cap < 0) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  const int64_t k = std::min(cap, h->mlp.nsamples);
+  if (out && k > 0)
+    HIP_TRY(h, hipMemcpy(out, h->mlp.samples, (size_t)k * MLP_SAMPLE * sizeof(float), hipMemcpyDeviceToHost));
+  return out ? k : h->mlp.nsamples;
+}
+
+int ihs_mlp_train(ihs_handle* h, const float* rows_host, int64_t n_rows, int32_t steps, int32_t batch, float lr,
+                  uint64_t seed, float* loss) {
+  if (!h || steps < 0 || batch < 1 || !(lr > 0.f)) return IHS_ERR_ARG;
+  MlpDev& m = h->mlp;
+  if (!m.trained) return fail(h, IHS_ERR_STATE, "set initial weights before training");
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  const int64_t rows = rows_host ? n_rows : m.nsamples;
+  if (rows < 1) return fail(h, IHS_ERR_STATE, "no training rows");
+  float* tmp = nullptr;
+  const float* table = m.samples;
+  if (rows_host) {
+    tmp = upload_tmp(h, rows_host, (size_t)rows * MLP_SAMPLE);
+    if (!tmp) return IHS_ERR_HIP;
+    table = tmp;
+  }
+  float* dloss = nullptr;
+  hipError_t e = hipMalloc(&dloss, std::max<int>(steps, 1) * sizeof(float));
+  if (e != hipSuccess) {
+    if (tmp) hipFree(tmp);
+    return fail(h, IHS_ERR_HIP, "loss buffer: %s", hipGetErrorString(e));
+  }
+  drop_graph(h);
+  uint64_t rng = seed;
+  const int64_t nb = std::min<int64_t>(batch, rows);
+  for (int32_t it = 0; it < steps; ++it) {
+    const uint64_t off = splitmix64(rng) % (uint64_t)rows;
+    uint64_t stride = rows > 1 ? 1 + splitmix64(rng) % (uint64_t)(rows - 1) : 1;
+    while (gcd64(stride, (uint64_t)rows) != 1) stride = stride + 1 < (uint64_t)rows ? stride + 1 : 1;
+    mlp_gradient(h->stream, m, table, MLP_SAMPLE, table + MLP_IN, MLP_SAMPLE, nb, rows, (int64_t)off,
+                 (int64_t)stride);
+    const float frac = steps > 1 ? (float)it / (float)(steps - 1) : 0.f;
+    const float lr_it = lr * (0.05f + 0.95f * 0.5f * (1.0f + std::cos(kPi * frac)));
+    mlp_adam(h->stream, m, lr_it);
+    hipMemcpyAsync(dloss + it, m.grad + MLP_NPARAM, sizeof(float), hipMemcpyDeviceToDevice, h->stream);
+  }
+  e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess && loss && steps > 0) e = hipMemcpy(loss, dloss, steps * sizeof(float), hipMemcpyDeviceToHost);
+  hipFree(dloss);
+  if (tmp) hipFree(tmp);
+  if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "training: %s", hipGetErrorString(e));
+  HIP_TRY(h, hipGetLastError());
+  return IHS_OK;
+}
+
+int ihs_mlp_gradient(ihs_handle* h, const float* x, const float* y, int64_t n, float* grad) {
+  if (!h || !x || !y || !grad || n < 1) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  float* dx = upload_tmp(h, x, (size_t)n * MLP_IN);
+  if (!dx) return IHS_ERR_HIP;
+  float* dy = upload_tmp(h, y, (size_t)n);
+  if (!dy) {
+    hipFree(dx);
+    return IHS_ERR_HIP;
+  }
+  mlp_gradient(h->stream, h->mlp, dx, MLP_IN, dy, 1, n, n, 0, 1);
+  hipError_t e = hipStreamSynchronize(h->stream);
+  if (e == hipSuccess) e = hipMemcpy(grad, h->mlp.grad, (MLP_NPARAM + 1) * sizeof(float), hipMemcpyDeviceToHost);
+  hipFree(dx);
+  hipFree(dy);
+  if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "gradient: %s", hipGetErrorString(e));
+  return IHS_OK;
+}
+
+int ihs_mlp_predict(ihs_handle* h, const float* x, int64_t n, float* y) {
+  if (!h || !x || !y || n < 1) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  float* dx = upload_tmp(h, x, (size_t)n * MLP_IN);
+  if (!dx) return IHS_ERR_HIP;
+  float* dy = nullptr;
+  hipError_t e = hipMalloc(&dy, (size_t)n * sizeof(float));
+  if (e == hipSuccess) {
+    mlp_forward(h->stream, h->mlp, dx, n, dy);
+    e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipMemcpy(y, dy, (size_t)n * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(dy);
+  }
+  hipFree(dx);
+  if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "predict: %s", hipGetErrorString(e));
   return IHS_OK;
 }
 

@@ -188,6 +188,61 @@ class GpuSph:
         assert got == L
         return dict(sorted_codes=sc, order=order, ccodes=cc, left=left, right=right, parent=parent)
 
+    # ---- learned smoothing-length seed ----
+    def set_mlp_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32).ravel()
+        self._check(self._L.ihs_mlp_set_weights(self._h, w.ctypes.data, w.shape[0]))
+
+    def load_mlp(self, path: str):
+        """Loads trained weights written by `mlp.save_weights` and enables the seed."""
+        from .mlp import load_weights
+        self.set_mlp_weights(load_weights(path))
+
+    def mlp_weights(self) -> np.ndarray:
+        w = np.zeros(_lib.MLP_NPARAM, np.float32)
+        self._check(self._L.ihs_mlp_get_weights(self._h, w.ctypes.data, w.shape[0]))
+        return w
+
+    def mlp_record(self, max_samples: int):
+        """Start (max_samples > 0) or stop (0) recording (features, target) rows at post-drift Newton solves."""
+        self._check(self._L.ihs_mlp_record(self._h, int(max_samples)))
+
+    def mlp_samples(self) -> np.ndarray:
+        k = self._check(self._L.ihs_mlp_samples(self._h, None, 0))
+        out = np.zeros((k, _lib.MLP_SAMPLE), np.float32)
+        if k:
+            got = self._check(self._L.ihs_mlp_samples(self._h, out.ctypes.data, k))
+            out = out[:got]
+        return out
+
+    def mlp_train(self, steps: int, batch: int, lr: float, seed: int = 0, rows: np.ndarray | None = None) -> np.ndarray:
+        """Adam on the device over the recorded rows (or `rows`, shape (n, 8)); returns the per-step loss."""
+        loss = np.zeros(max(int(steps), 1), np.float32)
+        if rows is not None:
+            rows = np.ascontiguousarray(rows, np.float32)
+            assert rows.ndim == 2 and rows.shape[1] == _lib.MLP_SAMPLE
+            ptr, n = rows.ctypes.data, rows.shape[0]
+        else:
+            ptr, n = None, 0
+        self._check(self._L.ihs_mlp_train(self._h, ptr, n, int(steps), int(batch), float(lr),
+                                          int(seed) & (2 ** 64 - 1), loss.ctypes.data))
+        return loss[:int(steps)]
+
+    def mlp_gradient(self, x, y) -> tuple[np.ndarray, float]:
+        x = np.ascontiguousarray(x, np.float32)
+        y = np.ascontiguousarray(y, np.float32)
+        assert x.ndim == 2 and x.shape[1] == _lib.MLP_IN and y.shape == (x.shape[0],)
+        g = np.zeros(_lib.MLP_NPARAM + 1, np.float32)
+        self._check(self._L.ihs_mlp_gradient(self._h, x.ctypes.data, y.ctypes.data, x.shape[0], g.ctypes.data))
+        return g[:-1], float(g[-1])
+
+    def mlp_predict(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        assert x.ndim == 2 and x.shape[1] == _lib.MLP_IN
+        y = np.zeros(x.shape[0], np.float32)
+        self._check(self._L.ihs_mlp_predict(self._h, x.ctypes.data, x.shape[0], y.ctypes.data))
+        return y
+
     # ---- GravityField ----
     def gravity(self, pos, mass, softening: float, theta: float = 0.5, mode: str = "tree", with_pot: bool = False):
         pos = np.ascontiguousarray(pos, np.float64)

@@ -161,6 +161,37 @@ int ihs_debug_field(ihs_handle* h, int32_t field, void* out);
 int ihs_profile(ihs_handle* h, int32_t enable);
 int ihs_profile_read(ihs_handle* h, double* ms, int64_t* calls, int32_t reset);
 
+/* Learned smoothing-length seed (no reference counterpart; the reference keeps h fixed). A 6->32->32->1 GELU
+ * network, IHS_MLP_NPARAM parameters laid out W1[32][6], b1[32], W2[32][32], b2[32], W3[32], b3, used as BF16
+ * with FP32 accumulation. Features per particle (previous evaluation's state): x0 = 100 div v dt, ln(rho/rho0),
+ * (n_ngb - 47)/47, |grad rho| h / rho, ln(h / (eta (m/rho0)^(1/3))), material id. Output y: the correction to
+ * the continuity estimate, so that the seed is h_prev exp((x0/3 + y)/100) (clamped to [h/1.2, 1.2 h] and to the
+ * searched radius); a recorded target is 100 ln(h_new/h_prev) - x0/3. In IHS_H_LEARNED mode the second force
+ * evaluation of each KDK step starts the Newton solve from that seed; Newton stays the judge. */
+#define IHS_MLP_NPARAM 1313
+#define IHS_MLP_IN 6
+#define IHS_MLP_SAMPLE 8 /* floats per recorded sample: 6 features, target, 0 */
+/* Sets the FP32 master weights (and their BF16 copy), clears the Adam state and enables the seed. */
+int ihs_mlp_set_weights(ihs_handle* h, const float* w, int64_t nparam);
+int ihs_mlp_get_weights(ihs_handle* h, float* w, int64_t nparam);
+/* max_samples > 0: (re)start recording (features, converged target) pairs at every post-drift Newton solve
+ * (KDK steps then launch directly, not from a graph) until max_samples rows are held; 0 stops recording and
+ * keeps the rows. Needs h_mode NEWTON or LEARNED. */
+int ihs_mlp_record(ihs_handle* h, int64_t max_samples);
+/* Copies up to cap recorded rows (IHS_MLP_SAMPLE floats each) to out (may be NULL); returns the row count. */
+int64_t ihs_mlp_samples(ihs_handle* h, float* out, int64_t cap);
+/* Adam on the mean squared error over the recorded rows (rows_host == NULL) or over n_rows host rows
+ * (IHS_MLP_SAMPLE floats each), on the device. Each step draws `batch` rows as (off + k * stride) mod rows with
+ * off and a stride coprime with rows drawn from seed; the step size decays from lr to lr/20 on a cosine.
+ * loss (may be NULL) receives the mean 0.5 * err^2 of each step's batch. Requires weights to start from. */
+int ihs_mlp_train(ihs_handle* h, const float* rows_host, int64_t n_rows, int32_t steps, int32_t batch, float lr,
+                  uint64_t seed, float* loss);
+/* Exact gradient of mean(0.5 (f(x) - y)^2) over n host samples (x: n*6, y: n) at the current BF16 weights:
+ * grad receives IHS_MLP_NPARAM values then the loss. */
+int ihs_mlp_gradient(ihs_handle* h, const float* x, const float* y, int64_t n, float* grad);
+/* Network output for n host feature rows (x: n*6) into y (n). */
+int ihs_mlp_predict(ihs_handle* h, const float* x, int64_t n, float* y);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,137 @@
+"""GPU: the learned smoothing-length seed. Device forward/backward against the FP64 oracle at the BF16 weights,
+Adam, training, and what the trained seed gains over the plain Newton solve at equal tolerance (new code with no
+reference counterpart: parity is against the oracle's definition, and the gain is measured, not assumed)."""
+import numpy as np
+import pytest
+
+from botheadstudios_amd import ics, mlp
+from botheadstudios_amd.gpu_sph import GpuSph
+from oracle import mlp_oracle as mo
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _weights(seed=0):
+    rng = np.random.default_rng(seed)
+    w = mlp.init_weights(seed)
+    w[1280:1312] = rng.normal(0, 0.3, 32)
+    w[1312] = 0.1
+    return w
+
+
+def _data(seed, n):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0, 1, (n, 6)).astype(np.float32)
+    x[:, 5] = rng.integers(0, 2, n)
+    y = (np.sin(x[:, 0]) + 0.3 * x[:, 1] * x[:, 2] - 0.2 * x[:, 5]).astype(np.float32)
+    return x, y
+
+
+def test_forward_matches_oracle():
+    w = _weights(1)
+    x, _ = _data(2, 10000)
+    with GpuSph(64) as sph:
+        sph.set_mlp_weights(w)
+        got = sph.mlp_predict(x)
+        assert np.array_equal(sph.mlp_weights(), w)
+    want = mo.forward(mo.bf16_round(w), x.astype(np.float64))
+    assert np.max(np.abs(got - want)) < 2e-5 * max(1.0, np.abs(want).max()), np.max(np.abs(got - want))
+
+
+@pytest.mark.parametrize("n", [1, 63, 5000, 100003])
+def test_gradient_matches_oracle(n):
+    """The device gradient (FP32 accumulation in a fixed order) equals the FP64 oracle's at the BF16 weights:
+    relative L2 error < 1e-3 over all 1,313 parameters, and the loss to 1e-5."""
+    w = _weights(3)
+    x, y = _data(4, n)
+    with GpuSph(64) as sph:
+        sph.set_mlp_weights(w)
+        g, loss = sph.mlp_gradient(x, y)
+        g2, _ = sph.mlp_gradient(x, y)
+    assert np.array_equal(g, g2), "gradient is not deterministic"
+    lref, gref = mo.loss_and_grad(mo.bf16_round(w), x.astype(np.float64), y.astype(np.float64))
+    assert np.linalg.norm(g - gref) / np.linalg.norm(gref) < 1e-3
+    assert loss == pytest.approx(lref, rel=1e-5)
+
+
+def test_adam_first_step_and_full_batch_permutation():
+    """One step over a batch the size of the table visits every row once (coprime stride), so the update is
+    w - lr g / (|g| + 1e-8) with g the full gradient (Adam's first bias-corrected step)."""
+    w = _weights(5)
+    x, y = _data(6, 4099)
+    rows = np.zeros((len(x), 8), np.float32)
+    rows[:, :6], rows[:, 6] = x, y
+    lr = 1e-3
+    with GpuSph(64) as sph:
+        sph.set_mlp_weights(w)
+        g, _ = sph.mlp_gradient(x, y)
+        loss = sph.mlp_train(1, len(x), lr, seed=11, rows=rows)
+        w1 = sph.mlp_weights()
+    want = w - lr * g / (np.abs(g) + 1e-8)
+    big = np.abs(g) > 1e-4 * np.abs(g).max()
+    assert np.allclose(w1[big], want[big], rtol=0, atol=1e-6)
+    lref, _ = mo.loss_and_grad(mo.bf16_round(w), x.astype(np.float64), y.astype(np.float64))
+    assert loss[0] == pytest.approx(lref, rel=1e-4)
+
+
+def test_training_fits_a_smooth_target():
+    x, y = _data(7, 200000)
+    rows = np.zeros((len(x), 8), np.float32)
+    rows[:, :6], rows[:, 6] = x, y
+    with GpuSph(64) as sph:
+        sph.set_mlp_weights(mlp.init_weights(0))
+        loss = sph.mlp_train(1500, 8192, 3e-3, seed=1, rows=rows)
+        pred = sph.mlp_predict(x[:20000])
+    assert np.all(np.isfinite(loss))
+    assert loss[-50:].mean() < 0.05 * loss[:10].mean(), (loss[:10].mean(), loss[-50:].mean())
+    assert np.sqrt(np.mean((pred - y[:20000]) ** 2)) < 0.1
+
+
+def _run(ic, mode, steps, dt, weights=None):
+    with GpuSph(ic.n, leaf_size=16, h_mode=mode) as sph:
+        sph.upload(ic.particles48(), ic.eos, ic.softening)
+        if weights is not None:
+            sph.set_mlp_weights(weights)
+        sph.set_dt(dt)
+        sph.stats(reset=True)
+        sph.encode_kdk(steps)
+        st = sph.stats()
+        return st, sph.field("h").astype(np.float64), sph.field("pos64"), sph.field("hiter")
+
+
+def test_recorded_rows_train_a_seed_that_saves_newton_work():
+    """Rows recorded by the plain solve train the network on the device; the learned seed then needs fewer
+    density evaluations per solve than the previous h, and the converged state is the same to the solve's
+    tolerance (Newton judges every seed; nothing is accepted above |f|/rho = 1e-4)."""
+    ic = ics.config_ic(1, n=16384)
+    rows = mlp.record_run(ic, 40, 40 * ic.n)
+    assert rows.shape == (40 * ic.n, 8) and np.isfinite(rows).all()
+    assert np.all(rows[:, 7] == 0) and set(np.unique(rows[:, 5])) <= {0.0, 1.0}
+    with GpuSph(64) as sph:
+        sph.set_mlp_weights(mlp.init_weights(0))
+        loss = sph.mlp_train(800, 32768, 3e-3, seed=2, rows=rows)
+        w = sph.mlp_weights()
+        pred = sph.mlp_predict(rows[:, :6])
+    assert loss[-20:].mean() < loss[:5].mean()
+    hit_prev = mlp.seed_hit_rate(rows, -rows[:, 0] / 3.0)
+    hit_learned = mlp.seed_hit_rate(rows, pred)
+    assert hit_learned > hit_prev, (hit_learned, hit_prev)
+
+    with GpuSph(ic.n, h_mode="newton") as sph:  # one adaptive dt, then both runs use it
+        sph.upload(ic.particles48(), ic.eos, ic.softening)
+        sph.set_dt(0.0)
+        sph.encode_kdk(1)
+        dt = sph.stats()["dt"]
+    sn, hn, pn, _ = _run(ic, "newton", 10, dt)
+    sl, hl, pl, _ = _run(ic, "learned", 10, dt, w)
+    for st in (sn, sl):
+        assert st["h_unconverged"] == 0 and st["h_capped"] == 0 and st["nonfinite"] == 0, st
+    print(f"\nmean density evaluations per solve: newton {sn['mean_h_evals']:.4f}  learned {sl['mean_h_evals']:.4f}"
+          f"  (seed hit rate on the recorded rows: previous h {hit_prev:.3f}, learned {hit_learned:.3f})")
+    assert sl["mean_h_evals"] < sn["mean_h_evals"]
+    # Same converged state to tolerance: each h meets |f|/rho <= 1e-4, i.e. |d ln h| <~ 3.3e-5 from the root.
+    rel = np.abs(hl - hn) / hn
+    assert np.median(rel) < 1e-4 and np.max(rel) < 2e-3, (np.median(rel), np.max(rel))
+    resid, _ = oracle.hresidual(pl, ic.m, hl, ics.ETA_48)
+    assert np.max(resid) <= 1.2e-4
