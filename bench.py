@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--variants", default="plain,learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=65536)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-steps", type=int, default=12)
     return ap.parse_args()
 
 

@@ -89,6 +89,8 @@ def collect(runs, max_rows: int, log=print) -> np.ndarray:
     """runs: iterable of (ic, steps). Concatenated finite rows, at most max_rows."""
     out, total = [], 0
     for ic, steps in runs:
+        if max_rows - total < ic.n:
+            break
         t0 = time.time()
         r = record_run(ic, steps, max_rows - total)
         r = r[np.isfinite(r).all(1)]
