@@ -135,7 +135,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
-                                   "design-only, see DESIGN.md)"}))
+                                   "design-only)"}))
         sys.exit(2)
     from botheadstudios_amd import ics
     from botheadstudios_amd import mlp as mlp_mod

@@ -6,7 +6,7 @@
 // Output y: the correction to the continuity-equation estimate, i.e. the seed is
 //   h_prev * exp((MLP_PHYS * x0 + y) / MLP_YSCALE)
 // (h ~ rho^(-1/3) and d ln rho = -div v dt give ln(h_new / h_prev) ~ div v dt / 3 to first order), clamped to
-// [h_prev / MLP_CLAMP, h_prev * MLP_CLAMP] and to the searched radius. The Newton solve judges the seed.
+// [h_prev / MLP_CLAMP, h_prev * MLP_CLAMP] and to the searched radius. The Newton solve accepts or corrects the seed.
 // Zero weights give the continuity estimate alone.
 #pragma once
 #include <hip/hip_runtime.h>

@@ -47,7 +47,7 @@ typedef struct {
 enum { IHS_PREC_FP32 = 0, IHS_PREC_SPLIT = 1 };
 /* Smoothing length: FIXED keeps the uploaded h (the reference's behaviour, hydrostatic.rs:58-61);
  * NEWTON solves rho~(h) = m (eta/h)^3 for n_ngb neighbours each force evaluation; LEARNED seeds
- * that Newton solve with the built-in network (Newton stays the judge). */
+ * that Newton solve with the built-in network (Newton checks every seed). */
 enum { IHS_H_FIXED = 0, IHS_H_NEWTON = 1, IHS_H_LEARNED = 2 };
 enum { IHS_GRAVITY_DIRECT = 0, IHS_GRAVITY_TREE = 1 };
 
@@ -167,7 +167,7 @@ int ihs_profile_read(ihs_handle* h, double* ms, int64_t* calls, int32_t reset);
  * (n_ngb - 47)/47, |grad rho| h / rho, ln(h / (eta (m/rho0)^(1/3))), material id. Output y: the correction to
  * the continuity estimate, so that the seed is h_prev exp((x0/3 + y)/100) (clamped to [h/1.2, 1.2 h] and to the
  * searched radius); a recorded target is 100 ln(h_new/h_prev) - x0/3. In IHS_H_LEARNED mode the second force
- * evaluation of each KDK step starts the Newton solve from that seed; Newton stays the judge. */
+ * evaluation of each KDK step starts the Newton solve from that seed; Newton checks every seed. */
 #define IHS_MLP_NPARAM 1313
 #define IHS_MLP_IN 6
 #define IHS_MLP_SAMPLE 8 /* floats per recorded sample: 6 features, target, 0 */

@@ -103,7 +103,7 @@ def _run(ic, mode, steps, dt, weights=None):
 def test_recorded_rows_train_a_seed_that_saves_newton_work():
     """Rows recorded by the plain solve train the network on the device; the learned seed then needs fewer
     density evaluations per solve than the previous h, and the converged state is the same to the solve's
-    tolerance (Newton judges every seed; nothing is accepted above |f|/rho = 1e-4)."""
+    tolerance (Newton checks every seed; nothing is accepted above |f|/rho = 1e-4)."""
     ic = ics.config_ic(1, n=16384)
     rows = mlp.record_run(ic, 40, 40 * ic.n)
     assert rows.shape == (40 * ic.n, 8) and np.isfinite(rows).all()
