@@ -15,7 +15,8 @@ class IhsConfig(C.Structure):
         ("capacity", C.c_int32), ("leaf_size", C.c_int32), ("precision", C.c_int32), ("h_mode", C.c_int32),
         ("theta", C.c_float), ("cfl", C.c_float), ("n_ngb", C.c_float), ("h_tol", C.c_float),
         ("h_max_iter", C.c_int32), ("h_margin", C.c_float), ("h_cap_factor", C.c_float),
-        ("nbr_capacity", C.c_int32), ("use_graph", C.c_int32), ("reserved", C.c_int32 * 7),
+        ("nbr_capacity", C.c_int32), ("use_graph", C.c_int32), ("h_margin_seeded", C.c_float),
+        ("reserved", C.c_int32 * 6),
     ]
 
 

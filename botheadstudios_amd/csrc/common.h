@@ -136,9 +136,9 @@ struct Kahan3 {
 // Blocks for a persistent launch: every CU filled to the kernel's occupancy (a multiple of the 8 XCDs).
 inline unsigned persistent_blocks(const void* kernel, int block) {
   int dev = 0, cus = 0, occ = 0;
-  hipGetDevice(&dev);
-  hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, 0);
+  if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, 0) != hipSuccess) occ = 0;
   const unsigned nb = (unsigned)((cus > 0 ? cus : 256) * (occ > 0 ? occ : 1));
   return (nb + 7u) & ~7u;
 }

@@ -64,6 +64,10 @@ struct ihs_handle {
   float h_cap = 0.f;
   int64_t force_evals = 0;
   bool have_fe = false;
+  // The last force evaluation solved h at the current positions (nothing has drifted since), so the next
+  // evaluation's search radius only needs room for round-off.
+  bool h_settled = false;
+  HsolveScratch hw;
   // One captured KDK step per starting parity of the ping-pong state buffers.
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
@@ -153,6 +157,8 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &t.range, Lc));
   HIP_TRY(h, dalloc(h, &t.parent, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.ready, Lc));
+  HIP_TRY(h, dalloc(h, &t.pend, 2 * Lc));
+  HIP_TRY(h, dalloc(h, &t.npend, 1));
   HIP_TRY(h, dalloc(h, &t.ncom, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.nbmin, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.nbmax, 2 * Lc));
@@ -171,14 +177,17 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &s.nbr, (size_t)C * h->cfg.nbr_capacity));
   HIP_TRY(h, dalloc(h, &s.ncount, C));
   HIP_TRY(h, dalloc(h, &s.flags, FLAG_COUNT));
-  HIP_TRY(h, dalloc(h, &s.counters, CNT_COUNT));
+  HIP_TRY(h, dalloc(h, &s.counters, CNT_SLOTS * CNT_STRIDE));
   HIP_TRY(h, dalloc(h, &s.epart, 9 * 1024));
   HIP_TRY(h, dalloc(h, &s.eout, 16));
   HIP_TRY(h, dalloc(h, &s.ctl, 1));
   HIP_TRY(h, alloc_raw(h, &s.eos, 16 * sizeof(ihs_eos)));
   HIP_TRY(h, dalloc(h, &s.staging, C));
+  HIP_TRY(h, dalloc(h, &h->hw.pend, C));
+  HIP_TRY(h, dalloc(h, &h->hw.bracket, C));
+  HIP_TRY(h, dalloc(h, &h->hw.npend, 1));
   HIP_TRY(h, hipMemset(s.flags, 0, FLAG_COUNT * sizeof(uint32_t)));
-  HIP_TRY(h, hipMemset(s.counters, 0, CNT_COUNT * sizeof(unsigned long long)));
+  HIP_TRY(h, hipMemset(s.counters, 0, CNT_SLOTS * CNT_STRIDE * sizeof(unsigned long long)));
   HIP_TRY(h, hipMemset(s.nbr, 0, (size_t)C * h->cfg.nbr_capacity * sizeof(uint32_t)));
   HIP_TRY(h, hipMemset(s.ncount, 0, C * sizeof(uint32_t)));
   StepCtl ctl{};
@@ -223,9 +232,13 @@ void prof_flush(ihs_handle* h) {
 
 bool valid_leaf(int k) { return k >= 1 && k <= 64 && (k & (k - 1)) == 0; }
 
+// Search-radius margin of an evaluation whose h was solved at unchanged positions: room for round-off only.
+constexpr float kMarginSettled = 1.02f;
+
 // One force evaluation: sort, permute, tree, neighbour lists, h-solve, density/EOS, forces, gravity.
-// after_drift: the positions moved since the previous evaluation (the second evaluation of a KDK step), so the
-// learned seed applies and a recording run stores (features, converged h) samples.
+// after_drift: the positions moved since the previous evaluation (the second evaluation of a KDK step). Then a
+// trained network seeds the h-solve (LEARNED) and sets a tight search radius around its seed, with a wider-radius
+// pass that runs only if some solve reached that radius; a recording run stores (features, converged h) rows.
 void force_eval(ihs_handle* h, bool after_drift) {
   hipStream_t s = h->stream;
   const int64_t n = h->n;
@@ -234,33 +247,57 @@ void force_eval(ihs_handle* h, bool after_drift) {
   MasterBufs& D = h->st[h->cur ^ 1];
   Scratch& sc = h->sc;
   const bool kahan = h->split != 0;
+  const bool newton = h->cfg.h_mode != IHS_H_FIXED;
+  const bool seeded = h->cfg.h_mode == IHS_H_LEARNED && after_drift && h->mlp.trained;
+  const bool record = h->mlp.record && after_drift && h->cfg.h_mode == IHS_H_NEWTON;
+  const float eta = std::cbrt(3.0f * h->cfg.n_ngb / (4.0f * kPi));
   prof_mark(h, -1);
   launch_fe_init(s, sc.flags);
   launch_bbox_morton(s, h->split, S.px, S.py, S.pz, n, h->tb);
   launch_radix_sort(s, n, 30, h->tb);
   launch_fix_ties(s, n, S.id, h->tb);
   prof_mark(h, IHS_STAGE_SORT);
-  const float margin = h->cfg.h_mode == IHS_H_FIXED ? 1.0f : h->cfg.h_margin;
+  float margin = 1.0f;
+  if (newton) margin = (!after_drift && h->h_settled) ? kMarginSettled : h->cfg.h_margin;
   launch_gather(s, h->split, n, h->tb.perm, S, D, sc.xm, sc.hs, margin, h->h_cap);
   h->cur ^= 1;
   prof_mark(h, IHS_STAGE_GATHER);
+  if (seeded) {
+    mlp_seed_h(s, h->mlp, n, D, sc.hs, sc.eos, sc.ctl, eta, h->cfg.h_margin_seeded, h->h_cap);
+    prof_mark(h, IHS_STAGE_HSOLVE);
+  }
+  // Walk groups are bounded in units of h, whatever margin set this evaluation's search radius.
+  const float eff_margin = seeded ? h->cfg.h_margin_seeded : margin;
+  const float diag_hs = newton ? GROUP_DIAG_HS * h->cfg.h_margin / eff_margin : GROUP_DIAG_HS;
   launch_karras(s, n, K, h->tb);
   launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb);
-  launch_groups(s, n, K, h->tb);
+  launch_groups(s, n, K, h->tb, nullptr, diag_hs);
   prof_mark(h, IHS_STAGE_TREE);
   launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags, sc.counters);
   prof_mark(h, IHS_STAGE_NBR);
-  const float eta = std::cbrt(3.0f * h->cfg.n_ngb / (4.0f * kPi));
-  const bool record = h->mlp.record && after_drift && h->cfg.h_mode != IHS_H_FIXED;
-  if (h->cfg.h_mode != IHS_H_FIXED) {
+  if (newton) {
+    const float eta3 = eta * eta * eta;
     if (record) mlp_record_features(s, h->mlp, n, D, sc.eos, sc.ctl, eta);
-    if (h->cfg.h_mode == IHS_H_LEARNED && after_drift) mlp_seed_h(s, h->mlp, n, D, sc.hs, sc.eos, sc.ctl, eta);
-    launch_hsolve(s, n, sc.xm, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta * eta * eta, h->cfg.h_tol,
-                  h->cfg.h_max_iter, sc.flags, sc.counters);
+    launch_hsolve(s, n, sc.xm, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol, h->cfg.h_max_iter,
+                  h->hw, sc.flags, sc.counters, nullptr, seeded ? sc.flags + FLAG_H_RETRY : nullptr, h->h_cap);
     if (record) mlp_record_targets(s, h->mlp, n, D);
     prof_mark(h, IHS_STAGE_HSOLVE);
+    if (seeded) {
+      // Wider-radius pass, gated on the device: every kernel returns at once unless a solve hit the tight radius.
+      const uint32_t* gate = sc.flags + FLAG_H_RETRY;
+      launch_retry_hs(s, n, D.h, sc.hs, h->cfg.h_margin, h->h_cap, sc.flags);
+      launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, gate);
+      launch_groups(s, n, K, h->tb, gate);
+      launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags,
+                       sc.counters, gate);
+      launch_hsolve(s, n, sc.xm, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol, h->cfg.h_max_iter,
+                    h->hw, sc.flags, sc.counters, gate, nullptr, h->h_cap);
+      prof_mark(h, IHS_STAGE_HSOLVE);
+    }
+    h->h_settled = true;
   }
-  const int feat = h->cfg.h_mode == IHS_H_LEARNED || h->mlp.record;
+  // The network's inputs come from the evaluation before a drift.
+  const int feat = (h->cfg.h_mode == IHS_H_LEARNED || h->mlp.record) && !after_drift;
   launch_density(s, h->split, feat, n, sc.xm, D.h, D.vx, D.vy, D.vz, D.u, D.mat, sc.eos, sc.nbr, sc.ncount, n, sc.rpc,
                  sc.vh, D.rho, D.feat, sc.flags);
   prof_mark(h, IHS_STAGE_DENSITY);
@@ -276,6 +313,7 @@ void force_eval(ihs_handle* h, bool after_drift) {
 void kdk_step(ihs_handle* h) {
   force_eval(h, false);
   launch_kick_drift(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, h->sc.flags);
+  h->h_settled = false;
   prof_mark(h, IHS_STAGE_KICK);
   force_eval(h, true);
   launch_kick(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl);
@@ -345,6 +383,7 @@ void ihs_default_config(ihs_config* c) {
   c->h_cap_factor = 4.0f;
   c->nbr_capacity = 192;
   c->use_graph = 1;
+  c->h_margin_seeded = 1.05f;
 }
 
 int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
@@ -354,7 +393,8 @@ int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
   if (cfg) c = *cfg;
   else ihs_default_config(&c);
   if (c.capacity < 1 || !valid_leaf(c.leaf_size) || c.nbr_capacity < 1 || c.h_mode < 0 || c.h_mode > 2 ||
-      c.precision < 0 || c.precision > 1 || !(c.theta >= 0.f) || !(c.h_margin >= 1.0f))
+      c.precision < 0 || c.precision > 1 || !(c.theta >= 0.f) || !(c.h_margin >= 1.0f) ||
+      !(c.h_margin_seeded >= 1.0f))
     return IHS_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IHS_ERR_HIP;
@@ -413,6 +453,7 @@ int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* e
   h->softening = softening;
   h->h_cap = h->cfg.h_cap_factor * hmax;
   h->have_fe = false;
+  h->h_settled = false;
   HIP_TRY(h, hipMemcpy(h->sc.eos, eos, n_mat * sizeof(ihs_eos), hipMemcpyHostToDevice));
   HIP_TRY(h, hipMemcpy(h->sc.staging, p, n * sizeof(ihs_particle), hipMemcpyHostToDevice));
   launch_unpack(h->stream, h->split, n, (const SphParticle48*)h->sc.staging, h->st[0]);
@@ -431,6 +472,7 @@ int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* e
     for (int pass = 0; pass < 16; ++pass) {
       uint32_t before = 0, after = 0;
       HIP_TRY(h, hipMemcpy(&before, h->sc.flags + FLAG_H_CAPPED, 4, hipMemcpyDeviceToHost));
+      h->h_settled = false;  // every pass may grow h by the full margin
       force_eval(h, false);
       HIP_TRY(h, hipStreamSynchronize(h->stream));
       HIP_TRY(h, hipMemcpy(&after, h->sc.flags + FLAG_H_CAPPED, 4, hipMemcpyDeviceToHost));
@@ -441,7 +483,7 @@ int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* e
     HIP_TRY(h, hipGetLastError());
   }
   // Counters describe stepping only.
-  HIP_TRY(h, hipMemset(h->sc.counters, 0, CNT_COUNT * sizeof(unsigned long long)));
+  HIP_TRY(h, hipMemset(h->sc.counters, 0, CNT_SLOTS * CNT_STRIDE * sizeof(unsigned long long)));
   {
     uint32_t fl[FLAG_COUNT] = {0};
     fl[FLAG_SIGNAL_MIN] = 0xffffffffu;
@@ -503,6 +545,12 @@ int ihs_step_kdk(ihs_handle* h, int32_t substeps) {
   if (substeps < 0) return fail(h, IHS_ERR_ARG, "negative substeps");
   hipSetDevice(h->device);
   const bool graph = h->cfg.use_graph && !h->mlp.record && !h->prof_on;
+  if (graph && substeps > 0 && h->cfg.h_mode != IHS_H_FIXED && !h->h_settled) {
+    // Captured steps assume h was solved at the current positions (the state after any step); a step after
+    // an upload without a solve or after relaxation runs directly once.
+    kdk_step(h);
+    --substeps;
+  }
   if (!graph) {
     for (int i = 0; i < substeps; ++i) kdk_step(h);
     HIP_TRY(h, hipGetLastError());
@@ -533,6 +581,7 @@ int ihs_relax(ihs_handle* h, int32_t steps) {
   for (int i = 0; i < steps; ++i) {
     force_eval(h, false);
     launch_relax(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl);
+    h->h_settled = false;
   }
   HIP_TRY(h, hipGetLastError());
   return IHS_OK;
@@ -572,10 +621,13 @@ int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset) {
   if (h->have_fe) HIP_TRY(h, hipMemcpy(e, h->sc.eout, sizeof e, hipMemcpyDeviceToHost));
   StepCtl ctl;
   uint32_t fl[FLAG_COUNT];
-  unsigned long long cn[CNT_COUNT];
+  unsigned long long cn[CNT_COUNT] = {0};
+  std::vector<unsigned long long> slots(CNT_SLOTS * CNT_STRIDE);
   HIP_TRY(h, hipMemcpy(&ctl, h->sc.ctl, sizeof ctl, hipMemcpyDeviceToHost));
   HIP_TRY(h, hipMemcpy(fl, h->sc.flags, sizeof fl, hipMemcpyDeviceToHost));
-  HIP_TRY(h, hipMemcpy(cn, h->sc.counters, sizeof cn, hipMemcpyDeviceToHost));
+  HIP_TRY(h, hipMemcpy(slots.data(), h->sc.counters, slots.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  for (int sl = 0; sl < CNT_SLOTS; ++sl)
+    for (int c = 0; c < CNT_COUNT; ++c) cn[c] += slots[sl * CNT_STRIDE + c];
   out->ke = e[0];
   out->ie = e[1];
   out->pe = e[2];
@@ -602,7 +654,7 @@ int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset) {
   out->nbr_candidates = (int64_t)cn[CNT_NBR_CAND];
   out->walk_nodes = (int64_t)cn[CNT_WALK_NODES];
   if (reset) {
-    HIP_TRY(h, hipMemset(h->sc.counters, 0, sizeof cn));
+    HIP_TRY(h, hipMemset(h->sc.counters, 0, slots.size() * sizeof(unsigned long long)));
     fl[FLAG_NBR_OVERFLOW] = fl[FLAG_WALK_OVERFLOW] = fl[FLAG_H_CAPPED] = fl[FLAG_H_UNCONVERGED] = fl[FLAG_NONFINITE] = 0;
     HIP_TRY(h, hipMemcpy(h->sc.flags, fl, sizeof fl, hipMemcpyHostToDevice));
     h->force_evals = 0;
@@ -769,8 +821,8 @@ int ihs_mlp_get_weights(ihs_handle* h, float* w, int64_t nparam) {
 
 int ihs_mlp_record(ihs_handle* h, int64_t max_samples) {
   if (!h || max_samples < 0) return IHS_ERR_ARG;
-  if (max_samples > 0 && h->cfg.h_mode == IHS_H_FIXED)
-    return fail(h, IHS_ERR_STATE, "recording needs a Newton h-solve (h_mode NEWTON or LEARNED)");
+  if (max_samples > 0 && h->cfg.h_mode != IHS_H_NEWTON)
+    return fail(h, IHS_ERR_STATE, "recording needs the plain Newton h-solve (h_mode NEWTON)");
   hipSetDevice(h->device);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   MlpDev& m = h->mlp;

@@ -230,9 +230,9 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
     }
   }
   if (lane == 0) {
-    atomicAdd(&counters[CNT_GRAV_CELL], n_cell);
-    atomicAdd(&counters[CNT_GRAV_PP], n_pp);
-    atomicAdd(&counters[CNT_WALK_NODES], n_nodes);
+    cnt_add(counters, CNT_GRAV_CELL, n_cell);
+    cnt_add(counters, CNT_GRAV_PP, n_pp);
+    cnt_add(counters, CNT_WALK_NODES, n_nodes);
   }
 }
 

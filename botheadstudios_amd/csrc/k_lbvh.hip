@@ -240,8 +240,9 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ cc, int64_t L
 
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cc, int64_t L, int2* __restrict__ child,
                                                 int2* __restrict__ range, uint32_t* __restrict__ parent,
-                                                uint32_t* __restrict__ ready) {
+                                                uint32_t* __restrict__ ready, uint32_t* __restrict__ npend) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *npend = 0u;
   if (i >= L - 1) return;
   const int64_t d = kdelta(cc, L, i, i + 1) > kdelta(cc, L, i, i - 1) ? 1 : -1;
   const int dmin = kdelta(cc, L, i, i - d);
@@ -282,7 +283,8 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cc,
 // ---------------------------------------------------------------------------------------------
 // Upsweep. Phase 1: one thread per leaf sums its bucket, then climbs; a node is merged by the second
 // child to arrive (arrival counter with workgroup-scope acquire/release) as long as the node's leaf
-// range lies inside this block's leaves. Phase 2 (one workgroup): the remaining top nodes.
+// range lies inside this block's leaves. A climb that reaches a node spanning several blocks records that
+// arrival instead. Phase 2 (one workgroup) replays the recorded arrivals up the remaining top nodes.
 constexpr int UPS_LEAVES = 1024;
 
 __device__ __forceinline__ void node_merge(uint32_t p, const int2* __restrict__ child, float4* ncom, float4* nbmin,
@@ -313,7 +315,10 @@ __global__ __launch_bounds__(UPS_LEAVES) void k_upsweep_local(const float4* __re
                                                               int64_t n, int K, int64_t L, const int2* __restrict__ child,
                                                               const int2* __restrict__ range,
                                                               const uint32_t* __restrict__ parent, uint32_t* ready,
-                                                              float4* ncom, float4* nbmin, float4* nbmax, float inv_theta) {
+                                                              float4* ncom, float4* nbmin, float4* nbmax, float inv_theta,
+                                                              uint32_t* __restrict__ pend, uint32_t* __restrict__ npend,
+                                                              const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
   const int64_t c = (int64_t)blockIdx.x * UPS_LEAVES + threadIdx.x;
   if (c >= L) return;
   const int64_t blo = (int64_t)blockIdx.x * UPS_LEAVES, bhi = blo + UPS_LEAVES;
@@ -348,7 +353,10 @@ __global__ __launch_bounds__(UPS_LEAVES) void k_upsweep_local(const float4* __re
   uint32_t p = parent[li];
   while (p != IHS_NONE) {
     const int2 rg = range[p];
-    if (rg.x < blo || rg.y >= bhi) break;  // top node: phase 2
+    if (rg.x < blo || rg.y >= bhi) {  // top node: phase 2 replays this arrival
+      pend[atomicAdd(npend, 1u)] = p;
+      break;
+    }
     const uint32_t old = __hip_atomic_fetch_add(&ready[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old == 0u) break;
     node_merge(p, child, ncom, nbmin, nbmax, inv_theta);
@@ -356,29 +364,30 @@ __global__ __launch_bounds__(UPS_LEAVES) void k_upsweep_local(const float4* __re
   }
 }
 
-__device__ __forceinline__ bool is_top(const int2* __restrict__ range, int64_t L, uint32_t node) {
-  if (node >= (uint32_t)(L - 1)) return false;  // leaves are complete after phase 1
-  const int2 rg = range[node];
-  return (rg.x / UPS_LEAVES) != (rg.y / UPS_LEAVES);
+// Re-arms the arrival counters for a second upsweep over the same topology (gated).
+__global__ __launch_bounds__(256) void k_tree_rearm(int64_t L, uint32_t* __restrict__ ready, uint32_t* __restrict__ npend,
+                                                    const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *npend = 0u;
+  if (i < L - 1) ready[i] = 0u;
 }
 
-__global__ __launch_bounds__(1024) void k_upsweep_top(int64_t L, const int2* __restrict__ child,
-                                                      const int2* __restrict__ range, const uint32_t* __restrict__ parent,
-                                                      uint32_t* ready, float4* ncom, float4* nbmin, float4* nbmax,
-                                                      float inv_theta) {
-  for (int64_t p = threadIdx.x; p < L - 1; p += blockDim.x) {
-    if (!is_top(range, L, (uint32_t)p)) continue;
-    const int2 ch = child[p];
-    const uint32_t kids[2] = {(uint32_t)ch.x, (uint32_t)ch.y};
-    for (int k = 0; k < 2; ++k) {
-      if (is_top(range, L, kids[k])) continue;  // arrives through its own climb
-      uint32_t q = (uint32_t)p;
-      while (q != IHS_NONE) {
-        const uint32_t old = __hip_atomic_fetch_add(&ready[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-        if (old == 0u) break;
-        node_merge(q, child, ncom, nbmin, nbmax, inv_theta);
-        q = parent[q];
-      }
+// Each thread replays recorded arrivals; every node above the blocks is merged once, by its second arrival.
+__global__ __launch_bounds__(1024) void k_upsweep_top(const uint32_t* __restrict__ pend,
+                                                      const uint32_t* __restrict__ npend, const int2* __restrict__ child,
+                                                      const uint32_t* __restrict__ parent, uint32_t* ready,
+                                                      float4* ncom, float4* nbmin, float4* nbmax, float inv_theta,
+                                                      const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
+  const uint32_t np = *npend;
+  for (uint32_t e = threadIdx.x; e < np; e += blockDim.x) {
+    uint32_t q = pend[e];
+    while (q != IHS_NONE) {
+      const uint32_t old = __hip_atomic_fetch_add(&ready[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0u) break;
+      node_merge(q, child, ncom, nbmin, nbmax, inv_theta);
+      q = parent[q];
     }
   }
 }
@@ -400,13 +409,14 @@ __device__ __forceinline__ int64_t node_count(const int2* __restrict__ range, in
 }
 
 __device__ __forceinline__ bool group_ok(const int2* __restrict__ range, int64_t L, int K, int64_t n, uint32_t x,
-                                         const float4* __restrict__ nbmin, const float4* __restrict__ nbmax) {
+                                         const float4* __restrict__ nbmin, const float4* __restrict__ nbmax,
+                                         float diag_hs) {
   int64_t first;
   if (node_count(range, L, K, n, x, first) > GROUP_MAX) return false;
   const float4 b0 = nbmin[x], b1 = nbmax[x];
   if (!(b0.w > 0.f)) return true;
   const float dx = b1.x - b0.x, dy = b1.y - b0.y, dz = b1.z - b0.z;
-  return dx * dx + dy * dy + dz * dz <= (GROUP_DIAG_HS * b0.w) * (GROUP_DIAG_HS * b0.w);
+  return dx * dx + dy * dy + dz * dz <= (diag_hs * b0.w) * (diag_hs * b0.w);
 }
 
 // One thread per leaf: the topmost acceptable node on the leaf's root path is its group; the group's first
@@ -415,14 +425,16 @@ __global__ __launch_bounds__(256) void k_mark_groups(int64_t n, int K, int64_t L
                                                      const uint32_t* __restrict__ parent,
                                                      const float4* __restrict__ nbmin,
                                                      const float4* __restrict__ nbmax,
-                                                     uint32_t* __restrict__ gstart) {
+                                                     uint32_t* __restrict__ gstart, const uint32_t* __restrict__ gate,
+                                                     float diag_hs) {
+  if (gate && *gate == 0u) return;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= L) return;
   uint32_t best = IHS_NONE;
   for (uint32_t x = (uint32_t)(L - 1 + c); x != IHS_NONE; x = parent[x]) {
     int64_t first;
     if (node_count(range, L, K, n, x, first) > GROUP_MAX) break;  // counts only grow towards the root
-    if (group_ok(range, L, K, n, x, nbmin, nbmax)) best = x;
+    if (group_ok(range, L, K, n, x, nbmin, nbmax, diag_hs)) best = x;
   }
   if (best == IHS_NONE) {
     gstart[c] = GROUP_SPLIT;
@@ -466,7 +478,9 @@ __device__ __forceinline__ uint32_t block_excl_scan_1024(uint32_t v, uint32_t* s
 }
 
 __global__ __launch_bounds__(GRP_BLK) void k_group_count(const uint32_t* __restrict__ gstart, int64_t L, int K,
-                                                         int64_t n, uint32_t* __restrict__ gblk) {
+                                                         int64_t n, uint32_t* __restrict__ gblk,
+                                                         const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
   __shared__ uint32_t s_w[GRP_BLK / 64];
   const int64_t c = (int64_t)blockIdx.x * GRP_BLK + threadIdx.x;
   uint32_t tot;
@@ -475,7 +489,8 @@ __global__ __launch_bounds__(GRP_BLK) void k_group_count(const uint32_t* __restr
 }
 
 __global__ void k_group_offsets(uint32_t* __restrict__ gblk, int nblk, uint32_t* __restrict__ ngroups,
-                                uint32_t* __restrict__ wq) {
+                                uint32_t* __restrict__ wq, const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
   if (threadIdx.x < 16) wq[threadIdx.x] = 0u;
   if (threadIdx.x != 0) return;
   uint32_t run = 0;
@@ -489,7 +504,9 @@ __global__ void k_group_offsets(uint32_t* __restrict__ gblk, int nblk, uint32_t*
 
 __global__ __launch_bounds__(GRP_BLK) void k_group_emit(const uint32_t* __restrict__ gstart, int64_t L, int K, int64_t n,
                                                         const int2* __restrict__ range,
-                                                        const uint32_t* __restrict__ gblk, int4* __restrict__ groups) {
+                                                        const uint32_t* __restrict__ gblk, int4* __restrict__ groups,
+                                                        const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
   __shared__ uint32_t s_w[GRP_BLK / 64];
   const int64_t c = (int64_t)blockIdx.x * GRP_BLK + threadIdx.x;
   const uint32_t wgt = group_weight(gstart, L, K, n, c);
@@ -545,24 +562,29 @@ void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb
 void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb) {
   const int64_t L = (n + K - 1) / K;
   k_leaf_codes<<<grid_for(L, 256), 256, 0, s>>>(tb.skeys, L, K, tb.ccode, tb.parent, tb.gstart);
-  if (L > 1) k_karras<<<grid_for(L - 1, 256), 256, 0, s>>>(tb.ccode, L, tb.child, tb.range, tb.parent, tb.ready);
+  if (L > 1)
+    k_karras<<<grid_for(L - 1, 256), 256, 0, s>>>(tb.ccode, L, tb.child, tb.range, tb.parent, tb.ready, tb.npend);
 }
 
-void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb) {
+void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const uint32_t* gate, float diag_hs) {
   const int64_t L = (n + K - 1) / K;
   const int nblk = (int)grid_for(L, GRP_BLK);
-  k_mark_groups<<<grid_for(L, 256), 256, 0, s>>>(n, K, L, tb.range, tb.parent, tb.nbmin, tb.nbmax, tb.gstart);
-  k_group_count<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.gblk);
-  k_group_offsets<<<1, 64, 0, s>>>(tb.gblk, nblk, tb.ngroups, tb.wq);
-  k_group_emit<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.range, tb.gblk, tb.groups);
+  k_mark_groups<<<grid_for(L, 256), 256, 0, s>>>(n, K, L, tb.range, tb.parent, tb.nbmin, tb.nbmax, tb.gstart, gate,
+                                                 diag_hs);
+  k_group_count<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.gblk, gate);
+  k_group_offsets<<<1, 64, 0, s>>>(tb.gblk, nblk, tb.ngroups, tb.wq, gate);
+  k_group_emit<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.range, tb.gblk, tb.groups, gate);
 }
 
-void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb) {
+void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb,
+                    const uint32_t* gate) {
   const int64_t L = (n + K - 1) / K;
   const float inv_theta = theta > 0.f ? 1.0f / theta : 3.0e38f;
+  if (gate) k_tree_rearm<<<grid_for(L, 256), 256, 0, s>>>(L, tb.ready, tb.npend, gate);
   k_upsweep_local<<<grid_for(L, UPS_LEAVES), UPS_LEAVES, 0, s>>>(xm, hs, n, K, L, tb.child, tb.range, tb.parent,
-                                                                tb.ready, tb.ncom, tb.nbmin, tb.nbmax, inv_theta);
+                                                                tb.ready, tb.ncom, tb.nbmin, tb.nbmax, inv_theta,
+                                                                tb.pend, tb.npend, gate);
   if (L > UPS_LEAVES)
-    k_upsweep_top<<<1, 1024, 0, s>>>(L, tb.child, tb.range, tb.parent, tb.ready, tb.ncom, tb.nbmin, tb.nbmax,
-                                     inv_theta);
+    k_upsweep_top<<<1, 1024, 0, s>>>(tb.pend, tb.npend, tb.child, tb.parent, tb.ready, tb.ncom, tb.nbmin, tb.nbmax,
+                                     inv_theta, gate);
 }

@@ -69,9 +69,10 @@ __device__ __forceinline__ void features(int64_t t, const MasterBufs& D, const E
   x[5] = (float)mat;
 }
 
-__global__ __launch_bounds__(256) void k_mlp_seed(int64_t n, MasterBufs D, const float* __restrict__ hs,
+__global__ __launch_bounds__(256) void k_mlp_seed(int64_t n, MasterBufs D, float* __restrict__ hs,
                                                   const EosDev* __restrict__ eos, const StepCtl* __restrict__ ctl,
-                                                  float eta, const uint16_t* __restrict__ wb) {
+                                                  float eta, const uint16_t* __restrict__ wb, float margin,
+                                                  float hcap) {
   __shared__ float sw[MLP_NPARAM];
   load_weights(sw, wb);
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -81,7 +82,9 @@ __global__ __launch_bounds__(256) void k_mlp_seed(int64_t n, MasterBufs D, const
   const float y = forward(sw, x) + MLP_PHYS * x[0];
   float seed = h * __expf(fminf(fmaxf(y, -50.f), 50.f) * (1.0f / MLP_YSCALE));
   seed = fminf(fmaxf(seed, h * (1.0f / MLP_CLAMP)), h * MLP_CLAMP);
-  D.h[t] = fminf(seed, hs[t]);
+  seed = fminf(seed, fmaxf(hcap, h));
+  D.h[t] = seed;
+  hs[t] = fminf(seed * margin, fmaxf(hcap, seed));
 }
 
 __global__ __launch_bounds__(256) void k_mlp_features(int64_t n, MasterBufs D, const EosDev* __restrict__ eos,
@@ -248,10 +251,10 @@ void mlp_free(MlpDev& m) {
   m = MlpDev{};
 }
 
-void mlp_seed_h(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, const float* hs, const void* eos,
-                const StepCtl* ctl, float eta) {
+void mlp_seed_h(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, float* hs, const void* eos,
+                const StepCtl* ctl, float eta, float margin, float hcap) {
   if (!m.trained) return;
-  k_mlp_seed<<<g256(n), 256, 0, s>>>(n, D, hs, (const EosDev*)eos, ctl, eta, m.wb);
+  k_mlp_seed<<<g256(n), 256, 0, s>>>(n, D, hs, (const EosDev*)eos, ctl, eta, m.wb, margin, hcap);
 }
 
 void mlp_record_features(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, const void* eos, const StepCtl* ctl,

@@ -1,8 +1,9 @@
 // SPH passes over Morton-sorted particles (gfx950, wave64):
 //   k_nbr_build  — per-group LBVH walk (box vs box + search radius), LDS-staged leaf tiles, per-particle
 //                  neighbour lists in column-major HBM layout (lane-coalesced reads later);
-//   k_hsolve     — Newton smoothing-length solve to a density-consistency tolerance (new: the reference
-//                  keeps h fixed, ie/crates/engine/src/hydrostatic.rs:58-61);
+//   k_hsolve1/2  — Newton smoothing-length solve to a density-consistency tolerance (new: the reference
+//                  keeps h fixed, ie/crates/engine/src/hydrostatic.rs:58-61), the unconverged particles of
+//                  the first pass compacted into full waves for the second;
 //   k_density    — rho_i with the pairwise h_ij = (h_i+h_j)/2 (ie/shaders/sph_step.wgsl:118-149), then
 //                  the Tillotson P and c per particle (sph_step.wgsl:78-100) and the Courant signal
 //                  (sph_step.wgsl:262-269), fused;
@@ -157,7 +158,9 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nbr_build(const float4* __res
                                                              uint32_t* __restrict__ wq, uint4* __restrict__ nbr4,
                                                              uint32_t* __restrict__ ncount, int cap, int64_t stride,
                                                              uint32_t* __restrict__ flags,
-                                                             unsigned long long* __restrict__ counters) {
+                                                             unsigned long long* __restrict__ counters,
+                                                             const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
   __shared__ NbrLds lds[NB_WAVES];
   const int w = threadIdx.x >> 6;
   const uint32_t lane = threadIdx.x & 63;
@@ -178,28 +181,52 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nbr_build(const float4* __res
     }
   }
   if (lane == 0) {
-    atomicAdd(&counters[CNT_PAIRS], npairs);
-    atomicAdd(&counters[CNT_NBR_CAND], ncand);
-    atomicAdd(&counters[CNT_WALK_NODES], nnodes);
+    cnt_add(counters, CNT_PAIRS, npairs);
+    cnt_add(counters, CNT_NBR_CAND, ncand);
+    cnt_add(counters, CNT_WALK_NODES, nnodes);
   }
 }
 
 // Visits the neighbour list of particle t: 16-byte chunks of four indices, chunk c of particle t at
-// c * stride + t (lanes of a wave read consecutive chunks).
-template <class F>
+// c * stride + t (lanes of a wave read consecutive chunks). Every slot of a written chunk holds a valid
+// particle index (slots past the count repeat earlier entries, or 0), so the four sources of a chunk are
+// fetched together, before any of them is used, and the next chunk's indices are prefetched: the loop keeps
+// several loads in flight instead of waiting on one neighbour at a time. Slots past the count are masked.
+template <class Fetch, class Use>
 __device__ __forceinline__ void for_neighbours(const uint4* __restrict__ nbr4, int64_t stride, int64_t t, uint32_t cnt,
-                                               F&& f) {
+                                               Fetch&& fetch, Use&& use) {
+  if (cnt == 0) return;
+  uint4 q = nbr4[t];
   for (uint32_t k = 0; k < cnt; k += 4) {
-    const uint4 q = nbr4[(int64_t)(k >> 2) * stride + t];
-    f(q.x);
-    if (k + 1 < cnt) f(q.y);
-    if (k + 2 < cnt) f(q.z);
-    if (k + 3 < cnt) f(q.w);
+    const uint4 qn = k + 4 < cnt ? nbr4[(int64_t)((k >> 2) + 1) * stride + t] : q;
+    const auto a = fetch(q.x);
+    const auto b = fetch(q.y);
+    const auto c = fetch(q.z);
+    const auto d = fetch(q.w);
+    use(q.x, a, true);
+    use(q.y, b, k + 1 < cnt);
+    use(q.z, c, k + 2 < cnt);
+    use(q.w, d, k + 3 < cnt);
+    q = qn;
   }
 }
 
-// Own-h gather density rho~(h) = sum_j m_j W(r_ij, h) (self included) and h*d(rho~)/dh, using
-// dW/dh = -(3W + r dW/dr)/h. Returns rho~, writes h*drho/dh.
+// Cubic spline with support r < h (sph_step.wgsl:64-77) from q = r/h and 1/h: W and dW/dr.
+__device__ __forceinline__ void spline_wdw(float q, float hinv, float& w, float& dw) {
+  const float sig = (8.0f / kPI32) * hinv * hinv * hinv;
+  if (q < 0.5f) {
+    w = sig * (1.0f - 6.0f * q * q + 6.0f * q * q * q);
+    dw = sig * (-12.0f * q + 18.0f * q * q) * hinv;
+  } else if (q < 1.0f) {
+    const float tq = 1.0f - q;
+    w = sig * 2.0f * tq * tq * tq;
+    dw = sig * (-6.0f * tq * tq) * hinv;
+  } else {
+    w = 0.f;
+    dw = 0.f;
+  }
+}
+
 __device__ __forceinline__ float own_density(const float4 me, float hh, const uint4* __restrict__ nbr4, int64_t stride,
                                              int64_t t, uint32_t cnt, const float4* __restrict__ xm, float& hdrho,
                                              uint32_t& nngb) {
@@ -209,12 +236,11 @@ __device__ __forceinline__ float own_density(const float4 me, float hh, const ui
   float hd = -3.0f * rho;
   const float h2 = hh * hh;
   uint32_t c = 0;
-  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) {
-    const float4 q = xm[j];
+  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return xm[j]; }, [&](uint32_t, const float4 q, bool ok) {
     const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < h2) {
-      const float qq = sqrtf(r2) * hinv;
+    if (ok && r2 < h2) {
+      const float qq = __builtin_amdgcn_sqrtf(r2) * hinv;
       float wq, dq;
       if (qq < 0.5f) {
         wq = 1.0f - 6.0f * qq * qq + 6.0f * qq * qq * qq;
@@ -235,58 +261,154 @@ __device__ __forceinline__ float own_density(const float4 me, float hh, const ui
   return rho;
 }
 
-__global__ __launch_bounds__(256) void k_hsolve(const float4* __restrict__ xm, const float* __restrict__ hs,
-                                                float* __restrict__ h, uint8_t* __restrict__ iters,
-                                                const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
-                                                int64_t stride, int64_t n, float eta3, float tol, int max_iter,
-                                                uint32_t* __restrict__ flags, unsigned long long* __restrict__ counters) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const bool valid = t < n;
-  uint32_t it = 0, evals = 0;
-  bool capped = false, unconv = false;
-  if (valid) {
-    const float4 me = xm[t];
-    const float hsi = hs[t];
-    const uint32_t cnt = ncount[t];
-    float hh = fminf(h[t], hsi);
-    float lo = 0.f, hi = INFINITY;
-    for (;;) {
-      float hd;
-      uint32_t nn;
-      const float rho = own_density(me, hh, nbr4, stride, t, cnt, xm, hd, nn);
-      ++evals;
-      const float hinv = 1.0f / hh;
-      const float target = me.w * eta3 * hinv * hinv * hinv;
-      const float f = rho - target;
-      if (fabsf(f) <= tol * rho) break;
-      if ((int)it >= max_iter) {
-        unconv = true;
-        break;
-      }
-      if (f < 0.f) lo = hh;
-      else hi = hh;
-      if (f < 0.f && hh >= hsi) {
-        capped = true;
-        break;
-      }
-      const float hdf = hd + 3.0f * target;  // h * df/dh
-      float hn = hdf > 0.f ? hh - f * hh / hdf : -1.0f;
-      if (!(hn > lo && hn < hi)) hn = isinf(hi) ? 2.0f * hh : 0.5f * (lo + hi);
-      hh = fminf(hn, hsi);
-      ++it;
+// Newton iteration on f(h) = rho~(h) - m (eta/h)^3 from hh with the bracket (lo, hi): stops when
+// |f| <= tol rho (DONE), when the searched radius stops a growing h (CAPPED), after max_iter updates
+// (UNCONVERGED), or after max_evals density sums with an update pending (PENDING: resume from (h, lo, hi)).
+enum : uint32_t { HS_PENDING = 0, HS_DONE = 1, HS_CAPPED = 2, HS_UNCONVERGED = 3 };
+
+struct HsolveState {
+  float h, lo, hi;
+  uint32_t it, evals, state;
+};
+
+__device__ __forceinline__ void hsolve_loop(HsolveState& st, const float4 me, float hsi, uint32_t max_evals,
+                                            const uint4* __restrict__ nbr4, int64_t stride, int64_t t, uint32_t cnt,
+                                            const float4* __restrict__ xm, float eta3, float tol, int max_iter) {
+  float hh = st.h;
+  for (;;) {
+    float hd;
+    uint32_t nn;
+    const float rho = own_density(me, hh, nbr4, stride, t, cnt, xm, hd, nn);
+    ++st.evals;
+    const float hinv = 1.0f / hh;
+    const float target = me.w * eta3 * hinv * hinv * hinv;
+    const float f = rho - target;
+    if (fabsf(f) <= tol * rho) {
+      st.state = HS_DONE;
+      break;
     }
-    h[t] = hh;
-    iters[t] = (uint8_t)min(it, 255u);
+    if ((int)st.it >= max_iter) {
+      st.state = HS_UNCONVERGED;
+      break;
+    }
+    if (f < 0.f) st.lo = hh;
+    else st.hi = hh;
+    if (f < 0.f && hh >= hsi) {
+      st.state = HS_CAPPED;
+      break;
+    }
+    const float hdf = hd + 3.0f * target;  // h * df/dh
+    float hn = hdf > 0.f ? hh - f * hh / hdf : -1.0f;
+    if (!(hn > st.lo && hn < st.hi)) hn = isinf(st.hi) ? 2.0f * hh : 0.5f * (st.lo + st.hi);
+    hh = fminf(hn, hsi);
+    ++st.it;
+    if (st.evals >= max_evals) {
+      st.state = HS_PENDING;
+      break;
+    }
   }
-  const uint64_t mc = __ballot(capped), mu = __ballot(unconv);
-  const unsigned long long si = wave_sum((unsigned long long)it), se = wave_sum((unsigned long long)evals);
+  st.h = hh;
+}
+
+// Wave-aggregated flags and counters. A solve stopped by a search radius below the global cap asks for the
+// wider-radius pass instead of being reported capped when `retry` is given.
+__device__ __forceinline__ void hsolve_account(bool valid, const HsolveState& st, float hsi, uint32_t* __restrict__ retry,
+                                               float hcap, uint32_t* __restrict__ flags,
+                                               unsigned long long* __restrict__ counters, bool first_pass) {
+  const bool capped = valid && st.state == HS_CAPPED;
+  const bool again = capped && retry && hsi < hcap;
+  const uint64_t mc = __ballot(capped && !again), mr = __ballot(again);
+  const uint64_t mu = __ballot(valid && st.state == HS_UNCONVERGED);
+  const unsigned long long si = wave_sum((unsigned long long)(valid ? st.it : 0u));
+  const unsigned long long se = wave_sum((unsigned long long)(valid ? st.evals : 0u));
   const unsigned long long sn = (unsigned long long)__popcll(__ballot(valid));
   if ((threadIdx.x & 63) == 0 && sn) {
     if (mc) atomicAdd(&flags[FLAG_H_CAPPED], (uint32_t)__popcll(mc));
     if (mu) atomicAdd(&flags[FLAG_H_UNCONVERGED], (uint32_t)__popcll(mu));
-    atomicAdd(&counters[CNT_H_ITERS], si);
-    atomicAdd(&counters[CNT_H_EVALS], se);
-    atomicAdd(&counters[CNT_H_SOLVED], sn);
+    if (mr) atomicAdd(retry, (uint32_t)__popcll(mr));
+    cnt_add(counters, CNT_H_ITERS, si);
+    cnt_add(counters, CNT_H_EVALS, se);
+    if (first_pass) cnt_add(counters, CNT_H_SOLVED, sn);
+  }
+}
+
+__global__ void k_hsolve_begin(const uint32_t* __restrict__ gate, uint32_t* __restrict__ npend) {
+  if (gate && *gate == 0u) return;
+  if (threadIdx.x == 0) *npend = 0u;
+}
+
+// First pass: one density sum per particle at its seed h. Converged particles are done; the others take one
+// Newton update and join the pending list (wave-aggregated append, in particle order within a wave) with their
+// bracket, so that the second pass runs full waves of unconverged particles instead of every wave waiting for
+// its slowest lane.
+__global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, const float* __restrict__ hs,
+                                                 float* __restrict__ h, uint8_t* __restrict__ iters,
+                                                 const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
+                                                 int64_t stride, int64_t n, float eta3, float tol, int max_iter,
+                                                 const uint32_t* __restrict__ gate, uint32_t* __restrict__ retry,
+                                                 float hcap, uint32_t* __restrict__ pend, float2* __restrict__ bracket,
+                                                 uint32_t* __restrict__ npend, uint32_t* __restrict__ flags,
+                                                 unsigned long long* __restrict__ counters) {
+  if (gate && *gate == 0u) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < n;
+  HsolveState st{0.f, 0.f, INFINITY, 0u, 0u, HS_DONE};
+  float hsi = 0.f;
+  if (valid) {
+    hsi = hs[t];
+    st.h = fminf(h[t], hsi);
+    hsolve_loop(st, xm[t], hsi, 1u, nbr4, stride, t, ncount[t], xm, eta3, tol, max_iter);
+    h[t] = st.h;
+  }
+  const bool more = valid && st.state == HS_PENDING;
+  const uint64_t mm = __ballot(more);
+  if (mm) {
+    uint32_t base = 0;
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(npend, (uint32_t)__popcll(mm));
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    if (more) {
+      pend[base + lanes_below(mm)] = (uint32_t)t;
+      bracket[t] = make_float2(st.lo, st.hi);
+    }
+  }
+  if (valid && !more) iters[t] = (uint8_t)min(st.it, 255u);
+  hsolve_account(valid, st, hsi, retry, hcap, flags, counters, true);
+}
+
+// Second pass over the pending list (persistent grid-stride loop): resumes each solve from its first update.
+__global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, const float* __restrict__ hs,
+                                                 float* __restrict__ h, uint8_t* __restrict__ iters,
+                                                 const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
+                                                 int64_t stride, float eta3, float tol, int max_iter,
+                                                 const uint32_t* __restrict__ gate, uint32_t* __restrict__ retry,
+                                                 float hcap, const uint32_t* __restrict__ pend,
+                                                 const float2* __restrict__ bracket, const uint32_t* __restrict__ npend,
+                                                 uint32_t* __restrict__ flags,
+                                                 unsigned long long* __restrict__ counters) {
+  if (gate && *gate == 0u) return;
+  const uint32_t np = *npend;
+  const uint32_t stride_g = gridDim.x * blockDim.x;
+  // Every lane of a wave runs the same number of rounds so the wave-wide accounting stays converged.
+  const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
+  for (uint32_t b = base0; b < np; b += stride_g) {
+    const uint32_t k = b + (threadIdx.x & 63u);
+    const bool valid = k < np;
+    HsolveState st{0.f, 0.f, INFINITY, 1u, 0u, HS_DONE};
+    float hsi = 0.f;
+    int64_t t = 0;
+    if (valid) {
+      t = pend[k];
+      hsi = hs[t];
+      const float2 br = bracket[t];
+      st.h = h[t];
+      st.lo = br.x;
+      st.hi = br.y;
+      hsolve_loop(st, xm[t], hsi, 0xffffffffu, nbr4, stride, t, ncount[t], xm, eta3, tol, max_iter);
+      h[t] = st.h;
+      iters[t] = (uint8_t)min(st.it, 255u);
+    }
+    st.it = valid ? st.it - 1u : 0u;  // the first update was counted by the first pass
+    hsolve_account(valid, st, hsi, retry, hcap, flags, counters, false);
   }
 }
 
@@ -313,18 +435,25 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
     float rho = me.w * (8.0f / kPI32) * hinv_i * hinv_i * hinv_i;
     float dv = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
     uint32_t nn = 0;
-    for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) {
-      const float4 q = xm[j];
-      const float hj = h[j];
+    struct Src {
+      float4 q;
+      float hj;
+    };
+    for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return Src{xm[j], h[j]}; },
+                   [&](uint32_t j, const Src src, bool ok) {
+      const float4 q = src.q;
       const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
-      const float hij = 0.5f * (hi + hj);
-      if (FEAT && r2 < hi * hi) ++nn;
-      if (r2 < hij * hij) {
-        const float r = sqrtf(r2);
-        rho += q.w * sph_w32(r, hij);
+      const float hij = 0.5f * (hi + src.hj);
+      if (FEAT && ok && r2 < hi * hi) ++nn;
+      if (ok && r2 < hij * hij) {
+        const float r = __builtin_amdgcn_sqrtf(r2);
+        const float hinv = __builtin_amdgcn_rcpf(hij);
+        float w, dw;
+        spline_wdw(r * hinv, hinv, w, dw);
+        rho += q.w * w;
         if (FEAT && r > 1.0e-9f) {
-          const float dw = sph_dw32(r, hij) / r;
+          dw *= __builtin_amdgcn_rcpf(r);
           const float ddx = dx * dw, ddy = dy * dw, ddz = dz * dw;
           gx += q.w * ddx;
           gy += q.w * ddy;
@@ -345,8 +474,15 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
     key = f2key(sig);
     if (!(sig == sig) || !(rho == rho)) atomicAdd(&flags[FLAG_NONFINITE], 1u);
   }
+  // One atomic per block: the Courant signal is a single word every block would otherwise contend on.
+  __shared__ uint32_t skey[4];
   key = wave_min(key);
-  if ((threadIdx.x & 63) == 0 && key != 0xffffffffu) atomicMin(&flags[FLAG_SIGNAL_MIN], key);
+  if ((threadIdx.x & 63) == 0) skey[threadIdx.x >> 6] = key;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    key = min(min(skey[0], skey[1]), min(skey[2], skey[3]));
+    if (key != 0xffffffffu) atomicMin(&flags[FLAG_SIGNAL_MIN], key);
+  }
 }
 
 template <bool KAHAN>
@@ -364,15 +500,17 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
   float ax = 0.f, ay = 0.f, az = 0.f, de = 0.f;
   Kahan3 ka;
   ka.zero();
-  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) {
-    const float4 q = xm[j];
-    const float4 vj = vh[j];
+  struct Src {
+    float4 q, vj, rj;
+  };
+  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return Src{xm[j], vh[j], rpc[j]}; },
+                 [&](uint32_t, const Src src, bool ok) {
+    const float4 q = src.q, vj = src.vj, rj = src.rj;
     const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float hij = 0.5f * (vi.w + vj.w);
-    if (!(r2 < hij * hij) || !(r2 > 1.0e-18f)) return;
-    const float4 rj = rpc[j];
-    const float r = sqrtf(r2);
+    if (!ok || !(r2 < hij * hij) || !(r2 > 1.0e-18f)) return;
+    const float r = __builtin_amdgcn_sqrtf(r2);
     const float dvx = vi.x - vj.x, dvy = vi.y - vj.y, dvz = vi.z - vj.z;
     const float vr = dvx * dx + dvy * dy + dvz * dz;
     float pij = 0.f;
@@ -383,7 +521,10 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
       pij = (-alpha * cbar * mu + beta * mu * mu) / rbar;
     }
     const float coeff = ri.y + rj.y + pij;
-    const float gw = sph_dw32(r, hij) / r;
+    const float hinv = __builtin_amdgcn_rcpf(hij);
+    float w, dw;
+    spline_wdw(r * hinv, hinv, w, dw);
+    const float gw = dw * __builtin_amdgcn_rcpf(r);
     const float fx = dx * gw, fy = dy * gw, fz = dz * gw;
     const float s = -coeff * q.w;
     if (KAHAN) {
@@ -411,20 +552,26 @@ inline unsigned g256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 void launch_nbr_build(hipStream_t s, int64_t n, int K, const float4* xm, const float* hs, const TreeBufs& tb,
                       uint32_t* nbr, uint32_t* ncount, int cap, int64_t stride, uint32_t* flags,
-                      unsigned long long* counters) {
+                      unsigned long long* counters, const uint32_t* gate) {
   uint4* nbr4 = (uint4*)nbr;
   const int64_t L = (n + K - 1) / K;
   static unsigned nblk = 0;
   if (!nblk) nblk = persistent_blocks((const void*)k_nbr_build, NB_WAVES * 64);
   k_nbr_build<<<nblk, NB_WAVES * 64, 0, s>>>(xm, hs, n, K, L, tb.child, tb.nbmin, tb.nbmax, tb.groups, tb.ngroups,
-                                             tb.wq, nbr4, ncount, cap, stride, flags, counters);
+                                             tb.wq, nbr4, ncount, cap, stride, flags, counters, gate);
 }
 
 void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, float* h, uint8_t* iters,
                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol, int max_iter,
-                   uint32_t* flags, unsigned long long* counters) {
-  k_hsolve<<<g256(n), 256, 0, s>>>(xm, hs, h, iters, (const uint4*)nbr, ncount, stride, n, eta3, tol, max_iter, flags,
-                                   counters);
+                   const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters, const uint32_t* gate,
+                   uint32_t* retry, float hcap) {
+  static unsigned nblk2 = 0;
+  if (!nblk2) nblk2 = persistent_blocks((const void*)k_hsolve2, 256);
+  k_hsolve_begin<<<1, 64, 0, s>>>(gate, hw.npend);
+  k_hsolve1<<<g256(n), 256, 0, s>>>(xm, hs, h, iters, (const uint4*)nbr, ncount, stride, n, eta3, tol, max_iter, gate,
+                                    retry, hcap, hw.pend, hw.bracket, hw.npend, flags, counters);
+  k_hsolve2<<<nblk2, 256, 0, s>>>(xm, hs, h, iters, (const uint4*)nbr, ncount, stride, eta3, tol, max_iter, gate, retry,
+                                  hcap, hw.pend, hw.bracket, hw.npend, flags, counters);
 }
 
 void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const float* vx,

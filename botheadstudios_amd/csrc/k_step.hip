@@ -14,7 +14,21 @@ namespace {
 inline unsigned g256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 __global__ void k_fe_init(uint32_t* flags) {
-  if (threadIdx.x == 0) flags[FLAG_SIGNAL_MIN] = 0xffffffffu;
+  if (threadIdx.x == 0) {
+    flags[FLAG_SIGNAL_MIN] = 0xffffffffu;
+    flags[FLAG_H_RETRY] = 0u;
+  }
+}
+
+// Wider-radius pass of a seeded evaluation (runs only when the tight pass stopped some solve at its radius).
+__global__ __launch_bounds__(256) void k_retry_hs(int64_t n, const float* __restrict__ h, float* __restrict__ hs,
+                                                  float margin, float hcap, uint32_t* __restrict__ flags) {
+  if (flags[FLAG_H_RETRY] == 0u) return;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) atomicAdd(&flags[FLAG_RETRIES], 1u);
+  if (t >= n) return;
+  const float hh = h[t];
+  hs[t] = fminf(hh * margin, fmaxf(hcap, hh));
 }
 
 template <class PT>
@@ -240,6 +254,10 @@ __global__ __launch_bounds__(256) void k_pos_f64(int64_t n, MasterBufs S, double
 }  // namespace
 
 void launch_fe_init(hipStream_t s, uint32_t* flags) { k_fe_init<<<1, 64, 0, s>>>(flags); }
+
+void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float margin, float hcap, uint32_t* flags) {
+  k_retry_hs<<<std::max(1u, g256(n)), 256, 0, s>>>(n, h, hs, margin, hcap, flags);
+}
 
 void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,
                    const MasterBufs& dst, float4* xm, float* hs, float h_margin, float h_cap) {

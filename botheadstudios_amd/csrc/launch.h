@@ -12,6 +12,9 @@ enum FlagIndex : int {
   FLAG_H_UNCONVERGED = 3,  // h-solves that hit the iteration limit
   FLAG_SIGNAL_MIN = 4,     // order-preserving key of min_i h_i/max(c_i+|v_i|,1)
   FLAG_NONFINITE = 5,      // particles with non-finite results
+  FLAG_H_RETRY = 6,        // this evaluation: seeded solves stopped by the tight search radius (gate of the
+                           // wider-radius pass)
+  FLAG_RETRIES = 7,        // evaluations that took the wider-radius pass
   FLAG_COUNT = 8
 };
 // 64-bit counters (Workspace::counters).
@@ -26,6 +29,14 @@ enum CounterIndex : int {
   CNT_WALK_NODES = 7, // nodes popped by all walks (gravity + neighbour search)
   CNT_COUNT = 8
 };
+// Counters are striped over CNT_SLOTS cache lines (slot = block index mod CNT_SLOTS) so that per-wave
+// atomics do not serialise on one address; readers sum the slots.
+constexpr int CNT_SLOTS = 64;
+constexpr int CNT_STRIDE = 16;  // 128 bytes per slot
+static_assert(CNT_COUNT <= CNT_STRIDE, "counter slot too small");
+__device__ __forceinline__ void cnt_add(unsigned long long* counters, int id, unsigned long long v) {
+  atomicAdd(&counters[(blockIdx.x & (CNT_SLOTS - 1)) * CNT_STRIDE + id], v);
+}
 
 // Buffers of one Morton/sort/Karras/upsweep pipeline (sized for a capacity).
 struct TreeBufs {
@@ -42,6 +53,8 @@ struct TreeBufs {
   int2* range = nullptr;     // L-1 (first, last leaf)
   uint32_t* parent = nullptr;  // 2L-1
   uint32_t* ready = nullptr;   // L-1
+  uint32_t* pend = nullptr;    // 2L: upsweep arrivals at nodes above the 1024-leaf blocks
+  uint32_t* npend = nullptr;   // 1
   float4* ncom = nullptr;   // 2L-1: com xyz, mass
   float4* nbmin = nullptr;  // 2L-1: bmin xyz, max search radius
   float4* nbmax = nullptr;  // 2L-1: bmax xyz, critical radius |bmax-bmin|/theta + |centre-com|
@@ -58,7 +71,8 @@ struct TreeBufs {
 };
 
 constexpr int GROUP_MAX = 64;            // one wavefront
-constexpr float GROUP_DIAG_HS = 6.0f;     // a group's box diagonal is at most this many search radii
+constexpr float GROUP_DIAG_HS = 6.0f;     // a group's box diagonal is at most this many search radii (at the
+                                          // full search margin; scaled so the bound stays fixed in units of h)
 constexpr uint32_t GROUP_SPLIT = 0xfffffffeu;
 
 constexpr int RADIX_THREADS = 256;
@@ -78,11 +92,14 @@ void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb
 void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb);
 // Walk groups in leaf order -> tb.groups / tb.ngroups, and resets the walk work queues. A group is the
 // topmost subtree on a leaf's root path holding <= GROUP_MAX particles whose box diagonal is at most
-// GROUP_DIAG_HS times its largest search radius (no radius: no box limit); leaves without one are split
-// into single-particle groups. Needs the upsweep's boxes.
-void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb);
-// Leaf moments and ordered upsweep; xm = sorted (x,y,z,m), hs = sorted search radius (or null).
-void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb);
+// diag_hs times its largest search radius (no radius: no box limit); leaves without one are split into
+// single-particle groups. Needs the upsweep's boxes.
+void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const uint32_t* gate = nullptr,
+                   float diag_hs = GROUP_DIAG_HS);
+// Leaf moments and ordered upsweep; xm = sorted (x,y,z,m), hs = sorted search radius (or null). With a gate
+// the arrival counters are re-armed first (a second upsweep over the same topology).
+void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb,
+                    const uint32_t* gate = nullptr);
 
 // --- k_gravity.hip ---
 // Wave-coherent Barnes-Hut group walk over sorted particles; accg = (a/G xyz... see kernel), sorted order.
@@ -93,12 +110,23 @@ void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, float soft2
 void launch_gravity_direct(hipStream_t s, int64_t n, float soft2, const float4* xm, float4* acc4);
 
 // --- k_sph.hip ---
+// Kernels taking `gate` do nothing when *gate == 0 (gate == null: always run); the wider-radius pass of a
+// seeded evaluation is launched unconditionally and gated on FLAG_H_RETRY, so a captured step needs no host check.
 void launch_nbr_build(hipStream_t s, int64_t n, int K, const float4* xm, const float* hs, const TreeBufs& tb,
                       uint32_t* nbr, uint32_t* ncount, int cap, int64_t stride, uint32_t* flags,
-                      unsigned long long* counters);
+                      unsigned long long* counters, const uint32_t* gate = nullptr);
+struct HsolveScratch {
+  uint32_t* pend = nullptr;    // n: particles left unconverged by the first pass
+  float2* bracket = nullptr;   // n: their (lo, hi) after the first update
+  uint32_t* npend = nullptr;   // 1
+};
+// retry != null: solves stopped by a search radius below hcap increment *retry instead of FLAG_H_CAPPED.
 void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, float* h, uint8_t* iters,
                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol, int max_iter,
-                   uint32_t* flags, unsigned long long* counters);
+                   const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters,
+                   const uint32_t* gate = nullptr, uint32_t* retry = nullptr, float hcap = 0.f);
+// Wider-radius pass set-up (gated): hs = min(h * margin, max(hcap, h)); counts the pass in FLAG_RETRIES.
+void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float margin, float hcap, uint32_t* flags);
 void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const float* vx,
                     const float* vy, const float* vz, const void* u, const uint8_t* mat, const void* eos,
                     const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float4* rpc, float4* vh, float* rho,

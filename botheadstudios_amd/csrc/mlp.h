@@ -6,7 +6,7 @@
 // Output y: the correction to the continuity-equation estimate, i.e. the seed is
 //   h_prev * exp((MLP_PHYS * x0 + y) / MLP_YSCALE)
 // (h ~ rho^(-1/3) and d ln rho = -div v dt give ln(h_new / h_prev) ~ div v dt / 3 to first order), clamped to
-// [h_prev / MLP_CLAMP, h_prev * MLP_CLAMP] and to the searched radius. The Newton solve accepts or corrects the seed.
+// [h_prev / MLP_CLAMP, h_prev * MLP_CLAMP]. The Newton solve accepts or corrects the seed.
 // Zero weights give the continuity estimate alone.
 #pragma once
 #include <hip/hip_runtime.h>
@@ -49,9 +49,10 @@ struct MlpDev {
 
 int mlp_alloc(MlpDev& m, int64_t cap);
 void mlp_free(MlpDev& m);
-// Writes the Newton initial guess into D.h (clamped to the searched radius hs). ctl gives the drift dt.
-void mlp_seed_h(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, const float* hs, const void* eos,
-                const StepCtl* ctl, float eta);
+// Writes the Newton initial guess into D.h and the searched radius hs = min(seed * margin, max(hcap, seed)).
+// ctl gives the drift dt.
+void mlp_seed_h(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, float* hs, const void* eos,
+                const StepCtl* ctl, float eta, float margin, float hcap);
 // Recording: features before the solve, then the converged target after it.
 void mlp_record_features(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, const void* eos, const StepCtl* ctl,
                          float eta);

@@ -65,7 +65,9 @@ typedef struct {
   float h_cap_factor;    /* search radius never exceeds h_cap_factor * max uploaded h (4) */
   int32_t nbr_capacity;  /* neighbour-list entries per particle (192) */
   int32_t use_graph;     /* capture the KDK step in a hipGraph (1) */
-  int32_t reserved[7];
+  float h_margin_seeded; /* LEARNED: search radius = h_margin_seeded * seed after a drift (1.05); a wider-radius
+                            pass (h_margin) runs only when some solve reaches it */
+  int32_t reserved[6];
 } ihs_config;
 
 typedef struct ihs_handle ihs_handle;
@@ -176,7 +178,7 @@ int ihs_mlp_set_weights(ihs_handle* h, const float* w, int64_t nparam);
 int ihs_mlp_get_weights(ihs_handle* h, float* w, int64_t nparam);
 /* max_samples > 0: (re)start recording (features, converged target) pairs at every post-drift Newton solve
  * (KDK steps then launch directly, not from a graph) until max_samples rows are held; 0 stops recording and
- * keeps the rows. Needs h_mode NEWTON or LEARNED. */
+ * keeps the rows. Needs h_mode NEWTON. */
 int ihs_mlp_record(ihs_handle* h, int64_t max_samples);
 /* Copies up to cap recorded rows (IHS_MLP_SAMPLE floats each) to out (may be NULL); returns the row count. */
 int64_t ihs_mlp_samples(ihs_handle* h, float* out, int64_t cap);
