@@ -34,7 +34,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1048576)
-    ap.add_argument("--leaf", type=int, default=16)
+    ap.add_argument("--leaf", type=int, default=None, help="default: the autotuned value for this N")
+    ap.add_argument("--theta", type=float, default=None, help="default: the autotuned value for this N")
+    ap.add_argument("--group", type=int, default=None, help="default: the autotuned value for this N")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--variants", default="plain,learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -50,7 +52,8 @@ def log(msg):
 def run_variant(ic, precision, h_mode, args, weights=None):
     from botheadstudios_amd.gpu_sph import GpuSph
 
-    sph = GpuSph(ic.n, leaf_size=args.leaf, precision=precision, h_mode=h_mode)
+    sph = GpuSph(ic.n, leaf_size=args.leaf, precision=precision, h_mode=h_mode, theta=args.theta,
+                 group_max=args.group)
     try:
         if weights is not None:
             sph.load_mlp(weights)
@@ -72,12 +75,14 @@ def run_variant(ic, precision, h_mode, args, weights=None):
         prof = sph.profile_read(reset=True)
         pst = sph.stats(reset=True)
         sph.profile(False)
+        # Force error of the last evaluation's tree walk against the FP64 direct sum on 8192 sampled targets.
+        gerr = sph.gravity_error(8192)
         for k in ("nbr_overflow", "walk_overflow", "nonfinite"):
             if st[k] or pst[k]:
                 raise RuntimeError(f"{precision}/{h_mode}: {k} = {st[k]}/{pst[k]}")
         sec = t1 - t0
         return dict(sec=sec, ms_per_step=1e3 * sec / args.steps, value=ic.n * args.steps / sec, stats=st,
-                    profile=prof, profile_stats=pst)
+                    profile=prof, profile_stats=pst, gravity_error=gerr)
     finally:
         sph.close()
 
@@ -137,10 +142,15 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "design-only)"}))
         sys.exit(2)
-    from botheadstudios_amd import ics
+    from botheadstudios_amd import autotune, ics
     from botheadstudios_amd import mlp as mlp_mod
 
     ic = ics.config_ic(4, n=args.n)
+    tuned = autotune.tuned_params(ic.n)
+    args.leaf = tuned["leaf"] if args.leaf is None else args.leaf
+    args.theta = tuned["theta"] if args.theta is None else args.theta
+    args.group = tuned["group"] if args.group is None else args.group
+    log(f"walk parameters: leaf {args.leaf}, group {args.group}, theta {args.theta}")
     variants = {}
     wanted = args.variants.split(",")
     if "plain" in wanted:
@@ -161,11 +171,13 @@ def main():
         "data": "synthetic (seeded jittered-lattice two-body impact)",
         "config": {"workload": f"two-body giant impact, N={ic.n}, 0.9+0.1 Earth masses, 45 deg, 1.1 v_esc; "
                                f"KDK step = 2 force evaluations; variant={head}",
-                   "n": ic.n, "leaf": args.leaf, "theta": 0.5, "n_ngb": 48, "h_tol": 1e-4, "variant": head},
+                   "n": ic.n, "leaf": args.leaf, "group": args.group, "theta": args.theta, "n_ngb": 48, "h_tol": 1e-4, "variant": head},
         "roofline": roofline(res, ic.n, args),
         "variants": {k: {"value": v["value"], "ms_per_step": v["ms_per_step"],
                          "mean_h_evals": v["stats"]["mean_h_evals"], "mean_h_iters": v["stats"]["mean_h_iters"],
                          "mean_neighbours": v["stats"]["mean_neighbours"],
+                         "gravity_rms_rel_err": v["gravity_error"]["rms"], "gravity_max_rel_err": v["gravity_error"]["max"],
+                         "h_capped": v["stats"]["h_capped"], "h_unconverged": v["stats"]["h_unconverged"],
                          "stage_ms_per_step": {s: round(ms / max(args.profile_steps, 1), 4)
                                                for s, (ms, c) in v["profile"].items()}}
                      for k, v in variants.items()},

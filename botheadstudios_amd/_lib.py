@@ -16,7 +16,7 @@ class IhsConfig(C.Structure):
         ("theta", C.c_float), ("cfl", C.c_float), ("n_ngb", C.c_float), ("h_tol", C.c_float),
         ("h_max_iter", C.c_int32), ("h_margin", C.c_float), ("h_cap_factor", C.c_float),
         ("nbr_capacity", C.c_int32), ("use_graph", C.c_int32), ("h_margin_seeded", C.c_float),
-        ("reserved", C.c_int32 * 6),
+        ("group_max", C.c_int32), ("reserved", C.c_int32 * 5),
     ]
 
 
@@ -41,7 +41,7 @@ EXPORTS = [
     "ihs_set_dt", "ihs_set_av", "ihs_set_omega", "ihs_step_kdk", "ihs_relax", "ihs_force_eval", "ihs_readback",
     "ihs_count", "ihs_sync", "ihs_stats_get", "ihs_gravity", "ihs_debug_tree", "ihs_debug_field",
     "ihs_profile", "ihs_profile_read", "ihs_mlp_set_weights", "ihs_mlp_get_weights", "ihs_mlp_record",
-    "ihs_mlp_samples", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict",
+    "ihs_mlp_samples", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict", "ihs_direct_f64",
 ]
 
 MLP_NPARAM = 1313
@@ -94,10 +94,11 @@ def load(path: str | None = None):
     L.ihs_mlp_train.argtypes = [vp, vp, i64, i32, i32, f32, C.c_uint64, vp]
     L.ihs_mlp_gradient.argtypes = [vp, vp, vp, i64, vp]
     L.ihs_mlp_predict.argtypes = [vp, vp, i64, vp]
+    L.ihs_direct_f64.argtypes = [vp, vp, i64, vp]
     for name in ("ihs_create", "ihs_destroy", "ihs_upload", "ihs_set_dt", "ihs_set_av", "ihs_set_omega",
                  "ihs_step_kdk", "ihs_relax", "ihs_force_eval", "ihs_sync", "ihs_stats_get", "ihs_gravity",
                  "ihs_debug_field", "ihs_profile", "ihs_profile_read", "ihs_mlp_set_weights", "ihs_mlp_get_weights",
-                 "ihs_mlp_record", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict"):
+                 "ihs_mlp_record", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict", "ihs_direct_f64"):
         getattr(L, name).restype = C.c_int
     if path is None:
         _lib = L

@@ -1,0 +1,127 @@
+"""Autotuner for the tree walk: leaf size K, walk-group size and opening angle theta.
+
+The step's cost is dominated by the Barnes-Hut walk. Its cost and its force error both depend on:
+- the leaf bucket K (exact particle-particle sums inside opened leaves vs monopoles of small subtrees);
+- the group size (how many particles share one walk);
+- theta.
+
+The reference fixes theta = 0.5 with K = 1 (ie/crates/engine/src/gpu_gravity.rs:55,61) at an RMS force error of
+0.70% (gpu-bh-verify:552-569). Here the force error is a bound, RMS relative error <= 1e-3 against the FP64
+direct sum on 8,192 sampled targets, so the tuner picks the fastest (K, group, theta) whose measured error stays
+below the bound with a margin.
+
+Each candidate is measured on the device:
+- ms per KDK step of the plain Newton step, from direct launches;
+- the walk's RMS/max error from `GpuSph.gravity_error`.
+
+Results are stored per particle-count bucket in botheadstudios_amd/data/tuned.json, which bench.py and callers
+read through `tuned_params(n)`.
+
+    python -m botheadstudios_amd.autotune --cfg 4          # tune for the 1M-particle impact
+"""
+from __future__ import annotations
+
+import json
+import math
+import os
+import time
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TUNED = os.path.join(HERE, "data", "tuned.json")
+DEFAULT = {"leaf": 16, "group": 64, "theta": 0.5}
+
+
+def measure(ic, leaf: int, theta: float, group: int = 64, steps: int = 3, warmup: int = 1, n_targets: int = 8192,
+            device: int = 0) -> dict:
+    """ms/step (plain Newton, FP32) and the walk's force error for one setting on an impact IC."""
+    from .gpu_sph import GpuSph
+
+    with GpuSph(ic.n, device=device, leaf_size=leaf, theta=theta, group_max=group, h_mode="newton",
+                use_graph=True) as sph:
+        sph.upload(ic.particles48(), ic.eos, ic.softening)
+        sph.set_dt(0.0)
+        sph.encode_kdk(warmup)
+        sph.sync()
+        t0 = time.perf_counter()
+        sph.encode_kdk(steps)
+        sph.sync()
+        ms = 1e3 * (time.perf_counter() - t0) / steps
+        st = sph.stats()
+        err = sph.gravity_error(n_targets)
+    return {"leaf": leaf, "group": group, "theta": theta, "ms_per_step": ms, "err_rms": err["rms"],
+            "err_max": err["max"], "ok": st["nbr_overflow"] == 0 and st["walk_overflow"] == 0 and st["nonfinite"] == 0}
+
+
+def tune(ic, err_target: float = 1e-3, margin: float = 0.92, leaves=(1, 2, 4, 8, 16), groups=(64, 32),
+         thetas=(0.5, 0.45, 0.42, 0.4, 0.375, 0.35), steps: int = 3, log=print) -> dict:
+    """Fastest (leaf, group, theta) with err_rms <= margin * err_target.
+
+    For each (leaf, group), theta is lowered from the largest candidate until the bound holds. A smaller theta
+    is only slower, so the search stops there."""
+    table = []
+    for leaf in leaves:
+        for group in groups:
+            if group < leaf:
+                continue
+            for th in thetas:
+                m = measure(ic, leaf, th, group, steps=steps)
+                table.append(m)
+                log(f"  K={leaf:2d} group={group} theta={th:.3f}: {m['ms_per_step']:.2f} ms/step, "
+                    f"err rms {m['err_rms']:.2e} max {m['err_max']:.2e}")
+                if m["ok"] and m["err_rms"] <= margin * err_target:
+                    break
+    feasible = [m for m in table if m["ok"] and m["err_rms"] <= margin * err_target]
+    best = min(feasible, key=lambda m: m["ms_per_step"]) if feasible else None
+    return {"n": ic.n, "err_target": err_target, "margin": margin, "best": best, "table": table}
+
+
+def bucket(n: int) -> str:
+    """Particle-count bucket key (powers of two)."""
+    return str(1 << max(0, int(round(math.log2(max(n, 1))))))
+
+
+def save(result: dict, path: str = TUNED):
+    data = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            data = json.load(f)
+    data[bucket(result["n"])] = result
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1)
+
+
+def tuned_params(n: int, path: str = TUNED) -> dict:
+    """The tuned {leaf, group, theta} for n particles (nearest tuned bucket), or the defaults."""
+    if not os.path.exists(path):
+        return dict(DEFAULT)
+    with open(path) as f:
+        data = json.load(f)
+    cands = [(abs(math.log2(int(k)) - math.log2(max(n, 1))), v) for k, v in data.items() if v.get("best")]
+    if not cands:
+        return dict(DEFAULT)
+    best = min(cands, key=lambda c: c[0])[1]["best"]
+    return {"leaf": int(best["leaf"]), "group": int(best["group"]), "theta": float(best["theta"])}
+
+
+def main(argv=None):
+    import argparse
+
+    from . import ics
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cfg", type=int, default=4)
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--err", type=float, default=1e-3)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--out", default=TUNED)
+    a = ap.parse_args(argv)
+    ic = ics.config_ic(a.cfg, n=a.n)
+    t0 = time.time()
+    res = tune(ic, err_target=a.err, steps=a.steps)
+    save(res, a.out)
+    print(json.dumps({"best": res["best"], "n": ic.n, "seconds": round(time.time() - t0, 1)}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -7,6 +7,7 @@
 #define IHS_NONE 0xffffffffu
 
 constexpr float kG32 = 6.674e-11f;  // ie/shaders/bh_gravity.wgsl:11
+constexpr double kG64 = 6.674e-11;  // ie/crates/engine/src/orbit.rs:18
 constexpr float kPI32 = 3.14159265359f;
 
 // ---------------------------------------------------------------------------------------------

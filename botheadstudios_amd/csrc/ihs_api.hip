@@ -145,6 +145,7 @@ int alloc_all(ihs_handle* h) {
   t.cap = C;
   t.lcap = Lc;
   t.nb_cap = radix_tiles(C);
+  t.group_max = h->cfg.group_max;
   HIP_TRY(h, dalloc(h, &t.bbox, 8));
   for (int b = 0; b < 2; ++b) {
     HIP_TRY(h, dalloc(h, &t.keys[b], C));
@@ -384,6 +385,7 @@ void ihs_default_config(ihs_config* c) {
   c->nbr_capacity = 192;
   c->use_graph = 1;
   c->h_margin_seeded = 1.05f;
+  c->group_max = 64;
 }
 
 int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
@@ -394,7 +396,7 @@ int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
   else ihs_default_config(&c);
   if (c.capacity < 1 || !valid_leaf(c.leaf_size) || c.nbr_capacity < 1 || c.h_mode < 0 || c.h_mode > 2 ||
       c.precision < 0 || c.precision > 1 || !(c.theta >= 0.f) || !(c.h_margin >= 1.0f) ||
-      !(c.h_margin_seeded >= 1.0f))
+      !(c.h_margin_seeded >= 1.0f) || c.group_max < c.leaf_size || c.group_max > GROUP_MAX)
     return IHS_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IHS_ERR_HIP;
@@ -785,6 +787,35 @@ int ihs_debug_field(ihs_handle* h, int32_t field, void* out) {
   }
   HIP_TRY(h, hipStreamSynchronize(s));
   HIP_TRY(h, hipMemcpy(out, tmp, bytes, hipMemcpyDeviceToHost));
+  return IHS_OK;
+}
+
+int ihs_direct_f64(ihs_handle* h, const uint32_t* ids, int64_t nt, double* acc3) {
+  if (!h || !ids || !acc3 || nt < 1) return IHS_ERR_ARG;
+  if (h->n < 1) return fail(h, IHS_ERR_STATE, "no particles uploaded");
+  for (int64_t k = 0; k < nt; ++k)
+    if (ids[k] >= (uint64_t)h->n) return fail(h, IHS_ERR_ARG, "target id %u >= n", ids[k]);
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  uint32_t *d_ids = nullptr, *d_tpos = nullptr;
+  double* d_acc = nullptr;
+  hipError_t e = hipMalloc(&d_ids, nt * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&d_tpos, nt * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc(&d_acc, nt * 3 * sizeof(double));
+  if (e == hipSuccess) e = hipMemcpy(d_ids, ids, nt * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const MasterBufs& S = h->st[h->cur];
+    const double soft = h->softening;
+    launch_direct_f64_targets(h->stream, h->split, S.px, S.py, S.pz, S.m, S.id, h->n, d_ids, nt, soft * soft, h->hw.pend,
+                              d_tpos, d_acc);
+    e = hipStreamSynchronize(h->stream);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpy(acc3, d_acc, nt * 3 * sizeof(double), hipMemcpyDeviceToHost);
+  }
+  if (d_ids) hipFree(d_ids);
+  if (d_tpos) hipFree(d_tpos);
+  if (d_acc) hipFree(d_acc);
+  if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "direct sum: %s", hipGetErrorString(e));
   return IHS_OK;
 }
 

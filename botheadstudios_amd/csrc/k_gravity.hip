@@ -289,6 +289,69 @@ __global__ __launch_bounds__(256) void k_acc_out(const uint32_t* __restrict__ pe
   if (pot) pot[i] = a.w;
 }
 
+// FP64 direct sum on selected targets (the accuracy probe of the walk): DT_TILE targets per block, every
+// thread strides over the sources and the block reduces in LDS. Positions in storage order (PT = float or
+// double); tpos = storage index of each target.
+constexpr int DT_TILE = 8;
+
+template <class PT>
+__global__ __launch_bounds__(256) void k_direct_f64(const PT* __restrict__ px, const PT* __restrict__ py,
+                                                    const PT* __restrict__ pz, const float* __restrict__ m, int64_t n,
+                                                    const uint32_t* __restrict__ tpos, int64_t nt, double soft2,
+                                                    double* __restrict__ acc3) {
+  __shared__ double red[256];
+  const int64_t t0 = (int64_t)blockIdx.x * DT_TILE;
+  double tx[DT_TILE], ty[DT_TILE], tz[DT_TILE], ax[DT_TILE], ay[DT_TILE], az[DT_TILE];
+  int64_t ti[DT_TILE];
+#pragma unroll
+  for (int k = 0; k < DT_TILE; ++k) {
+    const int64_t q = t0 + k < nt ? (int64_t)tpos[t0 + k] : 0;
+    ti[k] = q;
+    tx[k] = (double)px[q];
+    ty[k] = (double)py[q];
+    tz[k] = (double)pz[q];
+    ax[k] = ay[k] = az[k] = 0.0;
+  }
+  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+    const double xj = (double)px[j], yj = (double)py[j], zj = (double)pz[j], mj = (double)m[j];
+#pragma unroll
+    for (int k = 0; k < DT_TILE; ++k) {
+      if (j == ti[k]) continue;
+      const double dx = xj - tx[k], dy = yj - ty[k], dz = zj - tz[k];
+      const double r2 = dx * dx + dy * dy + dz * dz + soft2;
+      const double rinv = 1.0 / sqrt(r2);
+      const double f = mj * rinv * rinv * rinv;
+      ax[k] += f * dx;
+      ay[k] += f * dy;
+      az[k] += f * dz;
+    }
+  }
+  for (int k = 0; k < DT_TILE; ++k) {
+    for (int c = 0; c < 3; ++c) {
+      red[threadIdx.x] = c == 0 ? ax[k] : (c == 1 ? ay[k] : az[k]);
+      __syncthreads();
+      for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+      }
+      if (threadIdx.x == 0 && t0 + k < nt) acc3[3 * (t0 + k) + c] = kG64 * red[0];
+      __syncthreads();
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_storage_index(const uint32_t* __restrict__ id, int64_t n,
+                                                       uint32_t* __restrict__ inv) {
+  const int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (t < n) inv[id[t]] = (uint32_t)t;
+}
+
+__global__ __launch_bounds__(256) void k_map_ids(const uint32_t* __restrict__ inv, const uint32_t* __restrict__ ids,
+                                                 int64_t nt, uint32_t* __restrict__ tpos) {
+  const int64_t k = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (k < nt) tpos[k] = inv[ids[k]];
+}
+
 }  // namespace
 
 void launch_bodies_in(hipStream_t s, const double* pos3, const double* mass, int64_t n, double* px, double* py,
@@ -328,4 +391,18 @@ void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, float soft2
 
 void launch_gravity_direct(hipStream_t s, int64_t n, float soft2, const float4* xm, float4* acc4) {
   k_gravity_direct<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(xm, n, soft2, acc4);
+}
+
+void launch_direct_f64_targets(hipStream_t s, int split, const void* px, const void* py, const void* pz, const float* m,
+                               const uint32_t* id, int64_t n, const uint32_t* ids, int64_t nt, double soft2,
+                               uint32_t* inv, uint32_t* tpos, double* acc3) {
+  k_storage_index<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(id, n, inv);
+  k_map_ids<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(inv, ids, nt, tpos);
+  const unsigned nb = (unsigned)((nt + DT_TILE - 1) / DT_TILE);
+  if (split)
+    k_direct_f64<double><<<nb, 256, 0, s>>>((const double*)px, (const double*)py, (const double*)pz, m, n, tpos, nt,
+                                            soft2, acc3);
+  else
+    k_direct_f64<float><<<nb, 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, m, n, tpos, nt, soft2,
+                                           acc3);
 }

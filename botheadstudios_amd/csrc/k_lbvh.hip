@@ -393,7 +393,7 @@ __global__ __launch_bounds__(1024) void k_upsweep_top(const uint32_t* __restrict
 }
 
 // ---------------------------------------------------------------------------------------------
-// Walk groups: node x is a group root when it holds <= GROUP_MAX particles and its parent holds more.
+// Walk groups: node x is a group root when it holds <= gmax particles and its parent holds more.
 // Along any leaf-to-root path the particle count grows, so the roots partition the leaves.
 __device__ __forceinline__ int64_t node_count(const int2* __restrict__ range, int64_t L, int K, int64_t n, uint32_t x,
                                               int64_t& first) {
@@ -410,9 +410,9 @@ __device__ __forceinline__ int64_t node_count(const int2* __restrict__ range, in
 
 __device__ __forceinline__ bool group_ok(const int2* __restrict__ range, int64_t L, int K, int64_t n, uint32_t x,
                                          const float4* __restrict__ nbmin, const float4* __restrict__ nbmax,
-                                         float diag_hs) {
+                                         float diag_hs, int gmax) {
   int64_t first;
-  if (node_count(range, L, K, n, x, first) > GROUP_MAX) return false;
+  if (node_count(range, L, K, n, x, first) > gmax) return false;
   const float4 b0 = nbmin[x], b1 = nbmax[x];
   if (!(b0.w > 0.f)) return true;
   const float dx = b1.x - b0.x, dy = b1.y - b0.y, dz = b1.z - b0.z;
@@ -426,15 +426,15 @@ __global__ __launch_bounds__(256) void k_mark_groups(int64_t n, int K, int64_t L
                                                      const float4* __restrict__ nbmin,
                                                      const float4* __restrict__ nbmax,
                                                      uint32_t* __restrict__ gstart, const uint32_t* __restrict__ gate,
-                                                     float diag_hs) {
+                                                     float diag_hs, int gmax) {
   if (gate && *gate == 0u) return;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= L) return;
   uint32_t best = IHS_NONE;
   for (uint32_t x = (uint32_t)(L - 1 + c); x != IHS_NONE; x = parent[x]) {
     int64_t first;
-    if (node_count(range, L, K, n, x, first) > GROUP_MAX) break;  // counts only grow towards the root
-    if (group_ok(range, L, K, n, x, nbmin, nbmax, diag_hs)) best = x;
+    if (node_count(range, L, K, n, x, first) > gmax) break;  // counts only grow towards the root
+    if (group_ok(range, L, K, n, x, nbmin, nbmax, diag_hs, gmax)) best = x;
   }
   if (best == IHS_NONE) {
     gstart[c] = GROUP_SPLIT;
@@ -488,18 +488,25 @@ __global__ __launch_bounds__(GRP_BLK) void k_group_count(const uint32_t* __restr
   if (threadIdx.x == 0) gblk[blockIdx.x] = tot;
 }
 
-__global__ void k_group_offsets(uint32_t* __restrict__ gblk, int nblk, uint32_t* __restrict__ ngroups,
-                                uint32_t* __restrict__ wq, const uint32_t* __restrict__ gate) {
+// Exclusive scan of the per-block group counts (one workgroup; each thread owns a contiguous run of blocks).
+__global__ __launch_bounds__(GRP_BLK) void k_group_offsets(uint32_t* __restrict__ gblk, int nblk,
+                                                           uint32_t* __restrict__ ngroups, uint32_t* __restrict__ wq,
+                                                           const uint32_t* __restrict__ gate) {
   if (gate && *gate == 0u) return;
+  __shared__ uint32_t s_w[GRP_BLK / 64];
   if (threadIdx.x < 16) wq[threadIdx.x] = 0u;
-  if (threadIdx.x != 0) return;
-  uint32_t run = 0;
-  for (int b = 0; b < nblk; ++b) {
+  const int per = (nblk + GRP_BLK - 1) / GRP_BLK;
+  const int b0 = (int)threadIdx.x * per, b1 = min(b0 + per, nblk);
+  uint32_t sum = 0;
+  for (int b = b0; b < b1; ++b) sum += gblk[b];
+  uint32_t total;
+  uint32_t run = block_excl_scan_1024(sum, s_w, &total);
+  for (int b = b0; b < b1; ++b) {
     const uint32_t c = gblk[b];
     gblk[b] = run;
     run += c;
   }
-  *ngroups = run;
+  if (threadIdx.x == 0) *ngroups = total;
 }
 
 __global__ __launch_bounds__(GRP_BLK) void k_group_emit(const uint32_t* __restrict__ gstart, int64_t L, int K, int64_t n,
@@ -570,9 +577,9 @@ void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const uint32_t
   const int64_t L = (n + K - 1) / K;
   const int nblk = (int)grid_for(L, GRP_BLK);
   k_mark_groups<<<grid_for(L, 256), 256, 0, s>>>(n, K, L, tb.range, tb.parent, tb.nbmin, tb.nbmax, tb.gstart, gate,
-                                                 diag_hs);
+                                                 diag_hs, tb.group_max);
   k_group_count<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.gblk, gate);
-  k_group_offsets<<<1, 64, 0, s>>>(tb.gblk, nblk, tb.ngroups, tb.wq, gate);
+  k_group_offsets<<<1, GRP_BLK, 0, s>>>(tb.gblk, nblk, tb.ngroups, tb.wq, gate);
   k_group_emit<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.range, tb.gblk, tb.groups, gate);
 }
 

@@ -43,6 +43,7 @@ struct TreeBufs {
   int64_t cap = 0;   // particle capacity
   int64_t lcap = 0;  // leaf capacity
   int nb_cap = 0;    // radix tile capacity
+  int group_max = 64;  // walk-group size limit (K <= group_max <= 64 lanes)
   uint32_t* bbox = nullptr;  // 6 order-preserving keys
   uint32_t* keys[2] = {nullptr, nullptr};
   uint32_t* vals[2] = {nullptr, nullptr};
@@ -58,7 +59,7 @@ struct TreeBufs {
   float4* ncom = nullptr;   // 2L-1: com xyz, mass
   float4* nbmin = nullptr;  // 2L-1: bmin xyz, max search radius
   float4* nbmax = nullptr;  // 2L-1: bmax xyz, critical radius |bmax-bmin|/theta + |centre-com|
-  // walk groups: contiguous Morton ranges of at most GROUP_MAX particles with compact boxes, in leaf order
+  // walk groups: contiguous Morton ranges of at most group_max particles with compact boxes, in leaf order
   uint32_t* gstart = nullptr;   // L: group root node if leaf c is the first leaf of a group; GROUP_SPLIT if
                                 // leaf c is split into single-particle groups; none otherwise
   uint32_t* gblk = nullptr;     // per-1024-leaf block: group count, then exclusive offset
@@ -70,7 +71,7 @@ struct TreeBufs {
   uint32_t* perm = nullptr;
 };
 
-constexpr int GROUP_MAX = 64;            // one wavefront
+constexpr int GROUP_MAX = 64;            // one wavefront: the largest walk group
 constexpr float GROUP_DIAG_HS = 6.0f;     // a group's box diagonal is at most this many search radii (at the
                                           // full search margin; scaled so the bound stays fixed in units of h)
 constexpr uint32_t GROUP_SPLIT = 0xfffffffeu;
@@ -91,7 +92,7 @@ void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb
 // Leaf cluster codes + Karras topology + counters reset.
 void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb);
 // Walk groups in leaf order -> tb.groups / tb.ngroups, and resets the walk work queues. A group is the
-// topmost subtree on a leaf's root path holding <= GROUP_MAX particles whose box diagonal is at most
+// topmost subtree on a leaf's root path holding <= tb.group_max particles whose box diagonal is at most
 // diag_hs times its largest search radius (no radius: no box limit); leaves without one are split into
 // single-particle groups. Needs the upsweep's boxes.
 void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const uint32_t* gate = nullptr,
@@ -106,6 +107,11 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
 // Persistent: waves pull groups from per-XCD queues (tb.wq[0..8)).
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, float soft2, const float4* xm,
                          const TreeBufs& tb, float4* accg, uint32_t* flags, unsigned long long* counters);
+// FP64 direct-sum accelerations (G included) at the particles with the given ids, from the state's positions
+// in storage order (accuracy probe of the walk). inv needs n entries, tpos nt.
+void launch_direct_f64_targets(hipStream_t s, int split, const void* px, const void* py, const void* pz, const float* m,
+                               const uint32_t* id, int64_t n, const uint32_t* ids, int64_t nt, double soft2,
+                               uint32_t* inv, uint32_t* tpos, double* acc3);
 // Tiled direct O(N^2) sum (FP32), sorted or unsorted, acc4 = (acc xyz, potential).
 void launch_gravity_direct(hipStream_t s, int64_t n, float soft2, const float4* xm, float4* acc4);
 

@@ -50,7 +50,8 @@ class GpuSph:
     def __init__(self, capacity: int, device: int = 0, leaf_size: int = 16, precision: str = "fp32",
                  h_mode: str = "newton", theta: float = 0.5, cfl: float = 0.1, n_ngb: float = 48.0,
                  h_tol: float = 1e-4, h_max_iter: int = 40, h_margin: float = 1.25, h_cap_factor: float = 4.0,
-                 nbr_capacity: int = 192, use_graph: bool = True):
+                 nbr_capacity: int = 192, use_graph: bool = True, h_margin_seeded: float = 1.05,
+                 group_max: int = 64):
         self._L = _lib.load()
         cfg = default_config()
         cfg.capacity = int(capacity)
@@ -60,6 +61,7 @@ class GpuSph:
         cfg.theta, cfg.cfl, cfg.n_ngb, cfg.h_tol = float(theta), float(cfl), float(n_ngb), float(h_tol)
         cfg.h_max_iter, cfg.h_margin, cfg.h_cap_factor = int(h_max_iter), float(h_margin), float(h_cap_factor)
         cfg.nbr_capacity, cfg.use_graph = int(nbr_capacity), int(bool(use_graph))
+        cfg.h_margin_seeded, cfg.group_max = float(h_margin_seeded), int(group_max)
         self.config = cfg
         h = C.c_void_p()
         rc = self._L.ihs_create(int(device), C.byref(cfg), C.byref(h))
@@ -187,6 +189,24 @@ class GpuSph:
                                                  left.ctypes.data, right.ctypes.data, parent.ctypes.data))
         assert got == L
         return dict(sorted_codes=sc, order=order, ccodes=cc, left=left, right=right, parent=parent)
+
+    def direct_f64(self, ids) -> np.ndarray:
+        """FP64 direct-sum gravitational accelerations (n_t, 3) at the given upload-order ids (force-error probe)."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        acc = np.zeros((ids.shape[0], 3), np.float64)
+        self._check(self._L.ihs_direct_f64(self._h, ids.ctypes.data, ids.shape[0], acc.ctypes.data))
+        return acc
+
+    def gravity_error(self, n_targets: int = 8192, seed: int = 7) -> dict:
+        """RMS and max relative error of the last evaluation's tree-walk gravity against the FP64 direct sum
+        on sampled targets (the same norm as gpu-bh-verify:552-569)."""
+        n = self.count()
+        rng = np.random.default_rng(seed)
+        ids = np.sort(rng.choice(n, size=min(n_targets, n), replace=False)).astype(np.uint32)
+        ref = self.direct_f64(ids)
+        got = self.field("agrav")[ids, :3].astype(np.float64)
+        err = np.linalg.norm(got - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-300)
+        return {"rms": float(np.sqrt(np.mean(err ** 2))), "max": float(err.max()), "targets": int(len(ids))}
 
     # ---- learned smoothing-length seed ----
     def set_mlp_weights(self, w):

@@ -67,7 +67,8 @@ typedef struct {
   int32_t use_graph;     /* capture the KDK step in a hipGraph (1) */
   float h_margin_seeded; /* LEARNED: search radius = h_margin_seeded * seed after a drift (1.05); a wider-radius
                             pass (h_margin) runs only when some solve reaches it */
-  int32_t reserved[6];
+  int32_t group_max;     /* particles per tree-walk group, leaf_size..64 (64): one wavefront walks one group */
+  int32_t reserved[5];
 } ihs_config;
 
 typedef struct ihs_handle ihs_handle;
@@ -158,6 +159,11 @@ int ihs_gravity(ihs_handle* h, const double* pos3, const double* mass, int64_t n
 int64_t ihs_debug_tree(ihs_handle* h, uint32_t* sorted_codes, uint32_t* order, uint32_t* ccodes, uint32_t* left,
                        uint32_t* right, uint32_t* parent);
 int ihs_debug_field(ihs_handle* h, int32_t field, void* out);
+/* Force-error probe: FP64 direct-sum gravitational accelerations (G, the upload's softening; the particle itself
+ * excluded) at the particles with the given upload-order ids, from the current positions (FP64 in split
+ * precision). Compare with IHS_FIELD_AGRAV of the last force evaluation (gpu-bh-verify's cpu_direct check,
+ * ie/tools/gpu-bh-verify/src/main.rs:140-158, on the device). acc3 receives nt*3 doubles. Synchronous. */
+int ihs_direct_f64(ihs_handle* h, const uint32_t* ids, int64_t nt, double* acc3);
 /* Per-stage timing: enable=1 makes ihs_step_kdk launch directly (no graph) with HIP events around every
  * stage; ihs_profile_read returns accumulated milliseconds and launch counts per IHS_STAGE_*. */
 int ihs_profile(ihs_handle* h, int32_t enable);

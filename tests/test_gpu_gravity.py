@@ -86,3 +86,21 @@ def test_sph_path_gravity_impact_ic():
     ref = oracle.direct_f64(pos, ic.m, ic.softening, targets=idx)
     rms, worst = rms_rel(ag[idx], ref)
     assert rms <= 1e-3, (rms, worst)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "split"])
+def test_direct_f64_probe_matches_oracle(precision):
+    """The device FP64 direct sum used to measure the walk's force error equals the oracle's FP64 direct sum
+    (gpu-bh-verify:140-158) on the same positions, to round-off."""
+    ic = ics.config_ic(1, n=16384)
+    with GpuSph(ic.n, leaf_size=16, precision=precision, h_mode="fixed") as sph:
+        sph.upload(ic.particles48(), ic.eos, ic.softening)
+        sph.force_eval()
+        pos = sph.field("pos64")
+        ids = np.arange(0, ic.n, 37, dtype=np.uint32)
+        got = sph.direct_f64(ids)
+        err = sph.gravity_error(512)
+    m32 = ic.m.astype(np.float32).astype(np.float64)  # masses cross the boundary as f32 (gpu_sph.rs:25-37)
+    ref = oracle.direct_f64(pos, m32, np.float32(ic.softening), targets=ids.astype(np.int64))
+    assert np.max(np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)) < 1e-12
+    assert err["targets"] == 512 and err["rms"] < 2e-3, err
