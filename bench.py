@@ -37,6 +37,7 @@ def parse():
     ap.add_argument("--leaf", type=int, default=None, help="default: the autotuned value for this N")
     ap.add_argument("--theta", type=float, default=None, help="default: the autotuned value for this N")
     ap.add_argument("--group", type=int, default=None, help="default: the autotuned value for this N")
+    ap.add_argument("--quadrupole", type=int, default=None, help="default: the autotuned value for this N")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--variants", default="plain,learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -53,7 +54,7 @@ def run_variant(ic, precision, h_mode, args, weights=None):
     from botheadstudios_amd.gpu_sph import GpuSph
 
     sph = GpuSph(ic.n, leaf_size=args.leaf, precision=precision, h_mode=h_mode, theta=args.theta,
-                 group_max=args.group)
+                 group_max=args.group, quadrupole=args.quadrupole)
     try:
         if weights is not None:
             sph.load_mlp(weights)
@@ -150,7 +151,8 @@ def main():
     args.leaf = tuned["leaf"] if args.leaf is None else args.leaf
     args.theta = tuned["theta"] if args.theta is None else args.theta
     args.group = tuned["group"] if args.group is None else args.group
-    log(f"walk parameters: leaf {args.leaf}, group {args.group}, theta {args.theta}")
+    args.quadrupole = tuned["quadrupole"] if args.quadrupole is None else bool(args.quadrupole)
+    log(f"walk parameters: leaf {args.leaf}, group {args.group}, theta {args.theta}, quadrupole {args.quadrupole}")
     variants = {}
     wanted = args.variants.split(",")
     if "plain" in wanted:
@@ -171,7 +173,7 @@ def main():
         "data": "synthetic (seeded jittered-lattice two-body impact)",
         "config": {"workload": f"two-body giant impact, N={ic.n}, 0.9+0.1 Earth masses, 45 deg, 1.1 v_esc; "
                                f"KDK step = 2 force evaluations; variant={head}",
-                   "n": ic.n, "leaf": args.leaf, "group": args.group, "theta": args.theta, "n_ngb": 48, "h_tol": 1e-4, "variant": head},
+                   "n": ic.n, "leaf": args.leaf, "group": args.group, "theta": args.theta, "quadrupole": args.quadrupole, "n_ngb": 48, "h_tol": 1e-4, "variant": head},
         "roofline": roofline(res, ic.n, args),
         "variants": {k: {"value": v["value"], "ms_per_step": v["ms_per_step"],
                          "mean_h_evals": v["stats"]["mean_h_evals"], "mean_h_iters": v["stats"]["mean_h_iters"],

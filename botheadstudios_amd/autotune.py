@@ -1,13 +1,14 @@
-"""Autotuner for the tree walk: leaf size K, walk-group size and opening angle theta.
+"""Autotuner for the tree walk: multipole order, leaf size K, walk-group size and opening angle theta.
 
 The step's cost is dominated by the Barnes-Hut walk. Its cost and its force error both depend on:
 - the leaf bucket K (exact particle-particle sums inside opened leaves vs monopoles of small subtrees);
 - the group size (how many particles share one walk);
-- theta.
+- theta;
+- the multipole order of accepted cells (monopole, as the reference, or monopole + quadrupole).
 
 The reference fixes theta = 0.5 with K = 1 (ie/crates/engine/src/gpu_gravity.rs:55,61) at an RMS force error of
 0.70% (gpu-bh-verify:552-569). Here the force error is a bound, RMS relative error <= 1e-3 against the FP64
-direct sum on 8,192 sampled targets, so the tuner picks the fastest (K, group, theta) whose measured error stays
+direct sum on 8,192 sampled targets, so the tuner picks the fastest setting whose measured error stays
 below the bound with a margin.
 
 Each candidate is measured on the device:
@@ -28,16 +29,18 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(HERE, "data", "tuned.json")
-DEFAULT = {"leaf": 16, "group": 64, "theta": 0.5}
+DEFAULT = {"leaf": 16, "group": 64, "theta": 0.5, "quadrupole": True}
+QUAD_THETAS = (1.2, 1.1, 1.0, 0.9, 0.85, 0.8, 0.75, 0.7, 0.6, 0.5, 0.45, 0.4)
+MONO_THETAS = (0.5, 0.45, 0.42, 0.4, 0.375, 0.35)
 
 
-def measure(ic, leaf: int, theta: float, group: int = 64, steps: int = 3, warmup: int = 1, n_targets: int = 8192,
-            device: int = 0) -> dict:
+def measure(ic, leaf: int, theta: float, group: int = 64, quadrupole: bool = True, steps: int = 3, warmup: int = 1,
+            n_targets: int = 8192, device: int = 0) -> dict:
     """ms/step (plain Newton, FP32) and the walk's force error for one setting on an impact IC."""
     from .gpu_sph import GpuSph
 
-    with GpuSph(ic.n, device=device, leaf_size=leaf, theta=theta, group_max=group, h_mode="newton",
-                use_graph=True) as sph:
+    with GpuSph(ic.n, device=device, leaf_size=leaf, theta=theta, group_max=group, quadrupole=quadrupole,
+                h_mode="newton", use_graph=True) as sph:
         sph.upload(ic.particles48(), ic.eos, ic.softening)
         sph.set_dt(0.0)
         sph.encode_kdk(warmup)
@@ -48,28 +51,30 @@ def measure(ic, leaf: int, theta: float, group: int = 64, steps: int = 3, warmup
         ms = 1e3 * (time.perf_counter() - t0) / steps
         st = sph.stats()
         err = sph.gravity_error(n_targets)
-    return {"leaf": leaf, "group": group, "theta": theta, "ms_per_step": ms, "err_rms": err["rms"],
+    return {"leaf": leaf, "group": group, "theta": theta, "quadrupole": bool(quadrupole), "ms_per_step": ms,
+            "err_rms": err["rms"],
             "err_max": err["max"], "ok": st["nbr_overflow"] == 0 and st["walk_overflow"] == 0 and st["nonfinite"] == 0}
 
 
-def tune(ic, err_target: float = 1e-3, margin: float = 0.92, leaves=(1, 2, 4, 8, 16), groups=(64, 32),
-         thetas=(0.5, 0.45, 0.42, 0.4, 0.375, 0.35), steps: int = 3, log=print) -> dict:
-    """Fastest (leaf, group, theta) with err_rms <= margin * err_target.
+def tune(ic, err_target: float = 1e-3, margin: float = 0.92, leaves=(1, 2, 4, 8, 16), groups=(128, 64),
+         multipoles=((True, QUAD_THETAS), (False, MONO_THETAS)), steps: int = 3, log=print) -> dict:
+    """Fastest (multipole order, leaf, group, theta) with err_rms <= margin * err_target.
 
-    For each (leaf, group), theta is lowered from the largest candidate until the bound holds. A smaller theta
-    is only slower, so the search stops there."""
+    For each (order, leaf, group), theta is lowered from the largest candidate until the bound holds. A smaller
+    theta is only slower, so the search stops there."""
     table = []
-    for leaf in leaves:
-        for group in groups:
-            if group < leaf:
-                continue
-            for th in thetas:
-                m = measure(ic, leaf, th, group, steps=steps)
-                table.append(m)
-                log(f"  K={leaf:2d} group={group} theta={th:.3f}: {m['ms_per_step']:.2f} ms/step, "
-                    f"err rms {m['err_rms']:.2e} max {m['err_max']:.2e}")
-                if m["ok"] and m["err_rms"] <= margin * err_target:
-                    break
+    for quad, thetas in multipoles:
+        for leaf in leaves:
+            for group in groups:
+                if group < leaf:
+                    continue
+                for th in thetas:
+                    m = measure(ic, leaf, th, group, quad, steps=steps)
+                    table.append(m)
+                    log(f"  {'quad' if quad else 'mono'} K={leaf:2d} group={group} theta={th:.3f}: "
+                        f"{m['ms_per_step']:.2f} ms/step, err rms {m['err_rms']:.2e} max {m['err_max']:.2e}")
+                    if m["ok"] and m["err_rms"] <= margin * err_target:
+                        break
     feasible = [m for m in table if m["ok"] and m["err_rms"] <= margin * err_target]
     best = min(feasible, key=lambda m: m["ms_per_step"]) if feasible else None
     return {"n": ic.n, "err_target": err_target, "margin": margin, "best": best, "table": table}
@@ -101,7 +106,8 @@ def tuned_params(n: int, path: str = TUNED) -> dict:
     if not cands:
         return dict(DEFAULT)
     best = min(cands, key=lambda c: c[0])[1]["best"]
-    return {"leaf": int(best["leaf"]), "group": int(best["group"]), "theta": float(best["theta"])}
+    return {"leaf": int(best["leaf"]), "group": int(best["group"]), "theta": float(best["theta"]),
+            "quadrupole": bool(best.get("quadrupole", False))}
 
 
 def main(argv=None):

@@ -163,10 +163,20 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &t.ncom, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.nbmin, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.nbmax, 2 * Lc));
+  HIP_TRY(h, dalloc(h, &t.nqa, 2 * Lc));
+  HIP_TRY(h, dalloc(h, &t.nqb, 2 * Lc));
+  t.quadrupole = h->cfg.quadrupole;
   HIP_TRY(h, dalloc(h, &t.gstart, Lc));
   HIP_TRY(h, dalloc(h, &t.gblk, Lc / 1024 + 2));
   HIP_TRY(h, dalloc(h, &t.groups, C + 1));
   HIP_TRY(h, dalloc(h, &t.ngroups, 1));
+  if (t.group_max != GROUP_MAX) {
+    HIP_TRY(h, dalloc(h, &t.ggroups, C + 1));
+    HIP_TRY(h, dalloc(h, &t.nggroups, 1));
+  } else {
+    t.ggroups = t.groups;
+    t.nggroups = t.ngroups;
+  }
   HIP_TRY(h, dalloc(h, &t.wq, 16));
   Scratch& s = h->sc;
   HIP_TRY(h, dalloc(h, &s.xm, C));
@@ -240,7 +250,7 @@ constexpr float kMarginSettled = 1.02f;
 // after_drift: the positions moved since the previous evaluation (the second evaluation of a KDK step). Then a
 // trained network seeds the h-solve (LEARNED) and sets a tight search radius around its seed, with a wider-radius
 // pass that runs only if some solve reached that radius; a recording run stores (features, converged h) rows.
-void force_eval(ihs_handle* h, bool after_drift) {
+void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   hipStream_t s = h->stream;
   const int64_t n = h->n;
   const int K = h->cfg.leaf_size;
@@ -273,6 +283,7 @@ void force_eval(ihs_handle* h, bool after_drift) {
   launch_karras(s, n, K, h->tb);
   launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb);
   launch_groups(s, n, K, h->tb, nullptr, diag_hs);
+  if (h->tb.group_max != GROUP_MAX) launch_groups(s, n, K, h->tb, nullptr, diag_hs, true);
   prof_mark(h, IHS_STAGE_TREE);
   launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags, sc.counters);
   prof_mark(h, IHS_STAGE_NBR);
@@ -305,14 +316,15 @@ void force_eval(ihs_handle* h, bool after_drift) {
   launch_forces(s, kahan, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, n,
                 (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph);
   prof_mark(h, IHS_STAGE_FORCES);
-  launch_gravity_walk(s, n, K, kahan, h->softening * h->softening, sc.xm, h->tb, sc.accg, sc.flags, sc.counters);
+  launch_gravity_walk(s, n, K, kahan, h->softening * h->softening, sc.xm, h->tb, sc.accg, sc.flags, sc.counters,
+                      need_pot);
   prof_mark(h, IHS_STAGE_GRAVITY);
   h->force_evals++;
   h->have_fe = true;
 }
 
 void kdk_step(ihs_handle* h) {
-  force_eval(h, false);
+  force_eval(h, false, false);  // the potential is read only after the step's second evaluation
   launch_kick_drift(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, h->sc.flags);
   h->h_settled = false;
   prof_mark(h, IHS_STAGE_KICK);
@@ -386,6 +398,7 @@ void ihs_default_config(ihs_config* c) {
   c->use_graph = 1;
   c->h_margin_seeded = 1.05f;
   c->group_max = 64;
+  c->quadrupole = 1;
 }
 
 int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
@@ -396,7 +409,8 @@ int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
   else ihs_default_config(&c);
   if (c.capacity < 1 || !valid_leaf(c.leaf_size) || c.nbr_capacity < 1 || c.h_mode < 0 || c.h_mode > 2 ||
       c.precision < 0 || c.precision > 1 || !(c.theta >= 0.f) || !(c.h_margin >= 1.0f) ||
-      !(c.h_margin_seeded >= 1.0f) || c.group_max < c.leaf_size || c.group_max > GROUP_MAX)
+      !(c.h_margin_seeded >= 1.0f) || c.group_max < c.leaf_size || c.group_max > 2 * GROUP_MAX ||
+      c.quadrupole < 0 || c.quadrupole > 1)
     return IHS_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IHS_ERR_HIP;
@@ -724,6 +738,7 @@ int ihs_gravity(ihs_handle* h, const double* pos3, const double* mass, int64_t n
     launch_karras(s, n, K, h->tb);
     launch_upsweep(s, n, K, (float)theta, sc.xm, nullptr, h->tb);
     launch_groups(s, n, K, h->tb);
+    if (h->tb.group_max != GROUP_MAX) launch_groups(s, n, K, h->tb, nullptr, GROUP_DIAG_HS, true);
     launch_gravity_walk(s, n, K, 0, soft2, sc.xm, h->tb, sc.accg, sc.flags, sc.counters);
     launch_acc_out(s, h->tb.perm, n, sc.accg, sc.gacc, sc.gpot);
     h->have_fe = false;  // the tree buffers now describe these bodies, not the SPH state

@@ -19,9 +19,12 @@ constexpr int GW_WAVES = 4;
 constexpr int GW_STACK = 1024;
 constexpr int GW_RING = 128;
 
+template <bool QUAD>
 struct WalkLds {
   uint32_t stack[GW_STACK];
   float4 cell[GW_RING];
+  float4 qa[QUAD ? GW_RING : 1];  // accepted cells' quadrupoles (xx, yy, zz, xy), (xz, yz)
+  float2 qb[QUAD ? GW_RING : 1];
   uint32_t leaf[GW_RING];
   float4 part[64];
   uint32_t pidx[64];
@@ -51,6 +54,7 @@ __device__ __forceinline__ void acc_add(Acc& A, float px, float py, float pz, fl
   }
 }
 
+template <bool POT = true>
 __device__ __forceinline__ void interact(const float4 s, const float4 me, float soft2, bool self, float& ax, float& ay,
                                          float& az, float& pot) {
   const float dx = s.x - me.x, dy = s.y - me.y, dz = s.z - me.z;
@@ -62,31 +66,72 @@ __device__ __forceinline__ void interact(const float4 s, const float4 me, float 
   ax = fmaf(dx, mr3, ax);
   ay = fmaf(dy, mr3, ay);
   az = fmaf(dz, mr3, az);
-  pot -= mr;
+  if (POT) pot -= mr;
 }
 
-// Walk of one group by one wave. SELF: compare source indices to skip the particle itself (needed only
-// without softening; with softening the self term contributes zero force and a known potential -m/eps that
-// is removed at the end).
-template <bool KAHAN, bool SELF>
-__device__ __forceinline__ void walk_group(WalkLds& S, const int4 grp, uint32_t lane, const float4* __restrict__ xm,
+// Monopole + traceless quadrupole of a cell (com, M, mass-normalised q = Q/M) at d = com - x,
+// r^2 = |d|^2 + eps^2: a = M [d / r^3 - q d / r^5 + 5/2 (d.q.d) d / r^7], phi = -M [1/r + (d.q.d) / (2 r^5)]
+// (G applied later). M r^-k is formed by repeated multiplication so that no intermediate leaves FP32 range.
+template <bool POT>
+__device__ __forceinline__ void interact_quad(const float4 s, const float4 qa, const float2 qb, const float4 me,
+                                              float soft2, float& ax, float& ay, float& az, float& pot) {
+  const float dx = s.x - me.x, dy = s.y - me.y, dz = s.z - me.z;
+  const float r2 = fmaf(dx, dx, fmaf(dy, dy, fmaf(dz, dz, soft2)));
+  const float rinv = __builtin_amdgcn_rsqf(r2);
+  const float rinv2 = rinv * rinv;
+  const float mr = s.w * rinv;
+  const float mr3 = mr * rinv2;
+  const float mr5 = mr3 * rinv2;
+  const float mr7 = mr5 * rinv2;
+  const float qdx = fmaf(qa.x, dx, fmaf(qa.w, dy, qb.x * dz));
+  const float qdy = fmaf(qa.w, dx, fmaf(qa.y, dy, qb.y * dz));
+  const float qdz = fmaf(qb.x, dx, fmaf(qb.y, dy, qa.z * dz));
+  const float dqd = fmaf(dx, qdx, fmaf(dy, qdy, dz * qdz));
+  const float f = fmaf(2.5f * dqd, mr7, mr3);
+  ax = fmaf(dx, f, fmaf(-qdx, mr5, ax));
+  ay = fmaf(dy, f, fmaf(-qdy, mr5, ay));
+  az = fmaf(dz, f, fmaf(-qdz, mr5, az));
+  if (POT) pot -= fmaf(0.5f * dqd, mr5, mr);
+}
+
+// Walk of one group by one wave; each lane owns T targets (particles t0 + lane + 64 q, q < T), so a group holds
+// up to 64 T particles, one walk serves all of them, and every source read from LDS feeds T interactions.
+// SELF: compare source indices to skip the particle itself. This is needed only without softening: with
+// softening the self term contributes zero force and a known potential -m/eps, which is removed at the end.
+// POT: accumulate the potential. QUAD: accepted cells act with their quadrupoles too.
+template <bool KAHAN, bool SELF, int T, bool POT, bool QUAD>
+__device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uint32_t lane, const float4* __restrict__ xm,
                                            int64_t n, int K, int64_t L, const int2* __restrict__ child,
                                            const float4* __restrict__ ncom, const float4* __restrict__ nbmax,
+                                           const float4* __restrict__ nqa, const float2* __restrict__ nqb,
                                            float soft2, float4* __restrict__ accg, uint32_t* __restrict__ flags,
                                            uint64_t& n_cell, uint64_t& n_pp, uint64_t& n_nodes) {
   const int64_t t0 = grp.x;
-  const int64_t t = t0 + lane;
-  const bool valid = lane < (uint32_t)grp.y;
-  const float4 me = xm[valid ? t : t0];
-  // The group's box (invalid lanes repeat the first particle).
-  const float bx0 = wave_min(me.x), by0 = wave_min(me.y), bz0 = wave_min(me.z);
-  const float bx1 = wave_max(me.x), by1 = wave_max(me.y), bz1 = wave_max(me.z);
-  const uint64_t nvalid = (uint64_t)__popcll(__ballot(valid));
+  int64_t t[T];
+  bool valid[T];
+  float4 me[T];
+  float bx0 = 3.0e38f, by0 = 3.0e38f, bz0 = 3.0e38f, bx1 = -3.0e38f, by1 = -3.0e38f, bz1 = -3.0e38f;
+  uint32_t nv = 0;
+#pragma unroll
+  for (int q = 0; q < T; ++q) {
+    t[q] = t0 + lane + 64 * q;
+    valid[q] = lane + 64u * q < (uint32_t)grp.y;
+    me[q] = xm[valid[q] ? t[q] : t0];  // invalid slots repeat the first particle: the group box is unchanged
+    bx0 = fminf(bx0, me[q].x); by0 = fminf(by0, me[q].y); bz0 = fminf(bz0, me[q].z);
+    bx1 = fmaxf(bx1, me[q].x); by1 = fmaxf(by1, me[q].y); bz1 = fmaxf(bz1, me[q].z);
+    nv += valid[q] ? 1u : 0u;
+  }
+  bx0 = wave_min(bx0); by0 = wave_min(by0); bz0 = wave_min(bz0);
+  bx1 = wave_max(bx1); by1 = wave_max(by1); bz1 = wave_max(bz1);
+  const uint64_t nvalid = (uint64_t)wave_sum((unsigned long long)nv);
 
-  Acc A;
-  A.a[0] = A.a[1] = A.a[2] = A.pot = 0.f;
-  A.k.zero();
-  A.kpot = A.cpot = 0.f;
+  Acc A[T];
+#pragma unroll
+  for (int q = 0; q < T; ++q) {
+    A[q].a[0] = A[q].a[1] = A[q].a[2] = A[q].pot = 0.f;
+    A[q].k.zero();
+    A[q].kpot = A[q].cpot = 0.f;
+  }
 
   const uint32_t leaf0 = (uint32_t)(L - 1);
   const uint32_t lpc = 64u / (uint32_t)K;
@@ -98,13 +143,30 @@ __device__ __forceinline__ void walk_group(WalkLds& S, const int4 grp, uint32_t 
   // Accepted nodes: chead stays a multiple of 64 until the final flush, so a chunk is contiguous in the
   // 128-entry ring; cnt is a multiple of 4 (the tail is padded with massless entries).
   auto eval_cells = [&](uint32_t cnt, uint32_t real) {
-    const float4* src = &S.cell[chead & (GW_RING - 1)];
-    float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
+    const uint32_t h0 = chead & (GW_RING - 1);
+    const float4* src = &S.cell[h0];
+    const float4* sqa = &S.qa[h0];
+    const float2* sqb = &S.qb[h0];
+    float ax[T], ay[T], az[T], pp[T];
+#pragma unroll
+    for (int q = 0; q < T; ++q) ax[q] = ay[q] = az[q] = pp[q] = 0.f;
     for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
 #pragma unroll
-      for (uint32_t e = 0; e < 4; ++e) interact(src[k0 + e], me, soft2, false, ax, ay, az, pp);
+      for (uint32_t e = 0; e < 4; ++e) {
+        const float4 c = src[k0 + e];
+        if (QUAD) {
+          const float4 qa = sqa[k0 + e];
+          const float2 qb = sqb[k0 + e];
+#pragma unroll
+          for (int q = 0; q < T; ++q) interact_quad<POT>(c, qa, qb, me[q], soft2, ax[q], ay[q], az[q], pp[q]);
+        } else {
+#pragma unroll
+          for (int q = 0; q < T; ++q) interact<POT>(c, me[q], soft2, false, ax[q], ay[q], az[q], pp[q]);
+        }
+      }
     }
-    acc_add<KAHAN>(A, ax, ay, az, pp);
+#pragma unroll
+    for (int q = 0; q < T; ++q) acc_add<KAHAN>(A[q], ax[q], ay[q], az[q], pp[q]);
     chead += cnt;
     n_cell += nvalid * real;
   };
@@ -118,14 +180,21 @@ __device__ __forceinline__ void walk_group(WalkLds& S, const int4 grp, uint32_t 
     S.part[lane] = ok ? xm[src] : make_float4(0.f, 0.f, 0.f, 0.f);
     if (SELF) S.pidx[lane] = ok ? (uint32_t)src : IHS_NONE;
     wave_sync();
-    float ax = 0.f, ay = 0.f, az = 0.f, pp = 0.f;
+    float ax[T], ay[T], az[T], pp[T];
+#pragma unroll
+    for (int q = 0; q < T; ++q) ax[q] = ay[q] = az[q] = pp[q] = 0.f;
     const uint32_t cnt = (min(64u, nl * (uint32_t)K) + 3u) & ~3u;  // padding slots are massless
     for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
 #pragma unroll
-      for (uint32_t e = 0; e < 4; ++e)
-        interact(S.part[k0 + e], me, soft2, SELF && S.pidx[k0 + e] == (uint32_t)t, ax, ay, az, pp);
+      for (uint32_t e = 0; e < 4; ++e) {
+        const float4 c = S.part[k0 + e];
+#pragma unroll
+        for (int q = 0; q < T; ++q)
+          interact<POT>(c, me[q], soft2, SELF && S.pidx[k0 + e] == (uint32_t)t[q], ax[q], ay[q], az[q], pp[q]);
+      }
     }
-    acc_add<KAHAN>(A, ax, ay, az, pp);
+#pragma unroll
+    for (int q = 0; q < T; ++q) acc_add<KAHAN>(A[q], ax[q], ay[q], az[q], pp[q]);
     lhead += nl;
     n_pp += nvalid * (uint64_t)__popcll(__ballot(ok));
     wave_sync();
@@ -150,12 +219,21 @@ __device__ __forceinline__ void walk_group(WalkLds& S, const int4 grp, uint32_t 
       const float dy = fmaxf(fmaxf(by0 - c.y, c.y - by1), 0.f);
       const float dz = fmaxf(fmaxf(bz0 - c.z, c.z - bz1), 0.f);
       const float d2 = dx * dx + dy * dy + dz * dz + soft2;
-      acc = d2 >= rc * rc;
+      // Without softening a leaf touching the group box may hold a target itself: it takes the
+      // particle-particle path, which skips the self pair.
+      acc = d2 >= rc * rc && (!SELF || node < leaf0 || d2 > 0.f);
     }
     const bool isleaf = live && !acc && node >= leaf0;
     const bool open = live && !acc && node < leaf0;
     const uint64_t ma = __ballot(acc), mo = __ballot(open), ml = __ballot(isleaf);
-    if (acc) S.cell[(ctail + lanes_below(ma)) & (GW_RING - 1)] = c;
+    if (acc) {
+      const uint32_t slot = (ctail + lanes_below(ma)) & (GW_RING - 1);
+      S.cell[slot] = c;
+      if (QUAD) {
+        S.qa[slot] = nqa[node];
+        S.qb[slot] = nqb[node];
+      }
+    }
     ctail += (uint32_t)__popcll(ma);
     if (isleaf) S.leaf[(ltail + lanes_below(ml)) & (GW_RING - 1)] = node - leaf0;
     ltail += (uint32_t)__popcll(ml);
@@ -178,42 +256,54 @@ __device__ __forceinline__ void walk_group(WalkLds& S, const int4 grp, uint32_t 
   if (ctail != chead) {
     const uint32_t real = ctail - chead;
     const uint32_t pad = (4u - (real & 3u)) & 3u;
-    if (lane < pad) S.cell[(ctail + lane) & (GW_RING - 1)] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (lane < pad) {
+      const uint32_t slot = (ctail + lane) & (GW_RING - 1);
+      S.cell[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+      if (QUAD) {
+        S.qa[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+        S.qb[slot] = make_float2(0.f, 0.f);
+      }
+    }
     wave_sync();
     ctail += pad;
     eval_cells(ctail - chead, real);
   }
   while (ltail != lhead) eval_leaves(min(lpc, ltail - lhead));
 
-  if (valid) {
+#pragma unroll
+  for (int q = 0; q < T; ++q) {
+    if (!valid[q]) continue;
     float a0, a1, a2, p;
     if (KAHAN) {
-      a0 = A.k.s[0]; a1 = A.k.s[1]; a2 = A.k.s[2]; p = A.kpot;
+      a0 = A[q].k.s[0]; a1 = A[q].k.s[1]; a2 = A[q].k.s[2]; p = A[q].kpot;
     } else {
-      a0 = A.a[0]; a1 = A.a[1]; a2 = A.a[2]; p = A.pot;
+      a0 = A[q].a[0]; a1 = A[q].a[1]; a2 = A[q].a[2]; p = A[q].pot;
     }
-    if (!SELF) p += me.w * __builtin_amdgcn_rsqf(soft2);  // remove the self term -m/eps
-    accg[t] = make_float4(kG32 * a0, kG32 * a1, kG32 * a2, kG32 * p);
+    if (!POT) p = 0.f;
+    else if (!SELF) p += me[q].w * __builtin_amdgcn_rsqf(soft2);  // remove the self term -m/eps
+    accg[t[q]] = make_float4(kG32 * a0, kG32 * a1, kG32 * a2, kG32 * p);
   }
 }
 
 // Persistent walk: every wave pulls groups from its XCD's queue (a contiguous eighth of the groups, so that
 // XCD's L2 serves neighbouring groups), then helps the other queues; irregular groups therefore overlap
 // instead of leaving the machine idle behind the slowest one.
-template <bool KAHAN, bool SELF>
+template <bool KAHAN, bool SELF, int T, bool POT, bool QUAD>
 __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __restrict__ xm, int64_t n, int K,
                                                                  int64_t L, const int2* __restrict__ child,
                                                                  const float4* __restrict__ ncom,
                                                                  const float4* __restrict__ nbmax,
+                                                                 const float4* __restrict__ nqa,
+                                                                 const float2* __restrict__ nqb,
                                                                  const int4* __restrict__ groups,
                                                                  const uint32_t* __restrict__ ngroups,
                                                                  uint32_t* __restrict__ wq, float soft2,
                                                                  float4* __restrict__ accg, uint32_t* __restrict__ flags,
                                                                  unsigned long long* __restrict__ counters) {
-  __shared__ WalkLds lds[GW_WAVES];
+  __shared__ WalkLds<QUAD> lds[GW_WAVES];
   const int w = threadIdx.x >> 6;
   const uint32_t lane = threadIdx.x & 63;
-  WalkLds& S = lds[w];
+  WalkLds<QUAD>& S = lds[w];
   const uint32_t ng = *ngroups;
   const uint32_t home = blockIdx.x & 7u;
   uint64_t n_cell = 0, n_pp = 0, n_nodes = 0;
@@ -225,8 +315,8 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
       if (lane == 0) g = atomicAdd(&wq[q], 1u);
       g = (uint32_t)__builtin_amdgcn_readfirstlane((int)g) + lo;
       if (g >= hi) break;
-      walk_group<KAHAN, SELF>(S, groups[g], lane, xm, n, K, L, child, ncom, nbmax, soft2, accg, flags, n_cell, n_pp,
-                              n_nodes);
+      walk_group<KAHAN, SELF, T, POT, QUAD>(S, groups[g], lane, xm, n, K, L, child, ncom, nbmax, nqa, nqb, soft2, accg,
+                                            flags, n_cell, n_pp, n_nodes);
     }
   }
   if (lane == 0) {
@@ -367,25 +457,46 @@ void launch_acc_out(hipStream_t s, const uint32_t* perm, int64_t n, const float4
 }
 
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, float soft2, const float4* xm, const TreeBufs& tb,
-                         float4* accg, uint32_t* flags, unsigned long long* counters) {
+                         float4* accg, uint32_t* flags, unsigned long long* counters, bool pot) {
   const int64_t L = (n + K - 1) / K;
-  static unsigned nblk[4] = {0, 0, 0, 0};
   const bool self = !(soft2 > 0.f);
-  const int variant = (kahan ? 2 : 0) + (self ? 1 : 0);
-#define IHS_WALK(KA, SE)                                                                                          \
-  do {                                                                                                            \
-    if (!nblk[variant]) nblk[variant] = persistent_blocks((const void*)k_gravity_walk<KA, SE>, GW_WAVES * 64);    \
-    k_gravity_walk<KA, SE><<<nblk[variant], GW_WAVES * 64, 0, s>>>(xm, n, K, L, tb.child, tb.ncom, tb.nbmax,     \
-                                                                  tb.groups, tb.ngroups, tb.wq, soft2, accg,   \
-                                                                  flags, counters);                            \
+  const bool two = tb.group_max > GROUP_MAX;
+  const bool quad = tb.quadrupole != 0;
+  static unsigned nblk[32] = {0};
+  const int variant = (quad ? 16 : 0) + (kahan ? 8 : 0) + (self ? 4 : 0) + (two ? 2 : 0) + (pot ? 1 : 0);
+#define IHS_WALK(KA, SE, T, PO, QU)                                                                           \
+  do {                                                                                                        \
+    if (!nblk[variant])                                                                                       \
+      nblk[variant] = persistent_blocks((const void*)k_gravity_walk<KA, SE, T, PO, QU>, GW_WAVES * 64);       \
+    k_gravity_walk<KA, SE, T, PO, QU><<<nblk[variant], GW_WAVES * 64, 0, s>>>(                                \
+        xm, n, K, L, tb.child, tb.ncom, tb.nbmax, tb.nqa, tb.nqb, tb.ggroups, tb.nggroups, tb.wq, soft2, accg,  \
+        flags, counters);                                                                                     \
+  } while (0)
+#define IHS_WALK_P(KA, SE, T, QU)           \
+  do {                                      \
+    if (pot) IHS_WALK(KA, SE, T, true, QU); \
+    else IHS_WALK(KA, SE, T, false, QU);    \
+  } while (0)
+#define IHS_WALK_T(KA, SE, QU)                   \
+  do {                                           \
+    if (two) IHS_WALK_P(KA, SE, 2, QU);          \
+    else IHS_WALK_P(KA, SE, 1, QU);              \
+  } while (0)
+#define IHS_WALK_Q(KA, SE)                       \
+  do {                                           \
+    if (quad) IHS_WALK_T(KA, SE, true);          \
+    else IHS_WALK_T(KA, SE, false);              \
   } while (0)
   if (kahan) {
-    if (self) IHS_WALK(true, true);
-    else IHS_WALK(true, false);
+    if (self) IHS_WALK_Q(true, true);
+    else IHS_WALK_Q(true, false);
   } else {
-    if (self) IHS_WALK(false, true);
-    else IHS_WALK(false, false);
+    if (self) IHS_WALK_Q(false, true);
+    else IHS_WALK_Q(false, false);
   }
+#undef IHS_WALK_Q
+#undef IHS_WALK_T
+#undef IHS_WALK_P
 #undef IHS_WALK
 }
 

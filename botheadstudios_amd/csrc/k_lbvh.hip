@@ -287,12 +287,29 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cc,
 // arrival instead. Phase 2 (one workgroup) replays the recorded arrivals up the remaining top nodes.
 constexpr int UPS_LEAVES = 1024;
 
+// Mass-normalised traceless quadrupole q = Q / M (units m^2: Q itself overflows FP32 in SI units) merged about
+// the parent's centre of mass (parallel-axis shift): q = sum_c (m_c / M) (q_c + 3 s s^T - |s|^2 I),
+// s = com_c - com. w = m_c / M.
+__device__ __forceinline__ void quad_shift(float4& qa, float2& qb, const float4 cq, const float4 qca, const float2 qcb,
+                                           const float4 c, float w) {
+  const float sx = cq.x - c.x, sy = cq.y - c.y, sz = cq.z - c.z;
+  const float s2 = sx * sx + sy * sy + sz * sz;
+  qa.x += w * (qca.x + 3.0f * sx * sx - s2);
+  qa.y += w * (qca.y + 3.0f * sy * sy - s2);
+  qa.z += w * (qca.z + 3.0f * sz * sz - s2);
+  qa.w += w * (qca.w + 3.0f * sx * sy);
+  qb.x += w * (qcb.x + 3.0f * sx * sz);
+  qb.y += w * (qcb.y + 3.0f * sy * sz);
+}
+
 __device__ __forceinline__ void node_merge(uint32_t p, const int2* __restrict__ child, float4* ncom, float4* nbmin,
-                                           float4* nbmax, float inv_theta) {
+                                           float4* nbmax, float4* nqa, float2* nqb, float inv_theta) {
   const int2 ch = child[p];
   const float4 cl = ncom[ch.x], cr = ncom[ch.y];
   const float4 bl0 = nbmin[ch.x], br0 = nbmin[ch.y];
   const float4 bl1 = nbmax[ch.x], br1 = nbmax[ch.y];
+  const float4 qla = nqa[ch.x], qra = nqa[ch.y];
+  const float2 qlb = nqb[ch.x], qrb = nqb[ch.y];
   const float m = cl.w + cr.w;
   float4 c = make_float4(0.f, 0.f, 0.f, m);
   if (m > 0.f) {
@@ -300,6 +317,13 @@ __device__ __forceinline__ void node_merge(uint32_t p, const int2* __restrict__ 
     c.x = (cl.x * cl.w + cr.x * cr.w) * im;
     c.y = (cl.y * cl.w + cr.y * cr.w) * im;
     c.z = (cl.z * cl.w + cr.z * cr.w) * im;
+  }
+  float4 qa = make_float4(0.f, 0.f, 0.f, 0.f);
+  float2 qb = make_float2(0.f, 0.f);
+  if (m > 0.f) {
+    const float im = 1.0f / m;
+    quad_shift(qa, qb, cl, qla, qlb, c, cl.w * im);
+    quad_shift(qa, qb, cr, qra, qrb, c, cr.w * im);
   }
   const float4 b0 = make_float4(fminf(bl0.x, br0.x), fminf(bl0.y, br0.y), fminf(bl0.z, br0.z), fmaxf(bl0.w, br0.w));
   float4 b1 = make_float4(fmaxf(bl1.x, br1.x), fmaxf(bl1.y, br1.y), fmaxf(bl1.z, br1.z), 0.f);
@@ -309,14 +333,17 @@ __device__ __forceinline__ void node_merge(uint32_t p, const int2* __restrict__ 
   ncom[p] = c;
   nbmin[p] = b0;
   nbmax[p] = b1;
+  nqa[p] = qa;
+  nqb[p] = qb;
 }
 
 __global__ __launch_bounds__(UPS_LEAVES) void k_upsweep_local(const float4* __restrict__ xm, const float* __restrict__ hs,
                                                               int64_t n, int K, int64_t L, const int2* __restrict__ child,
                                                               const int2* __restrict__ range,
                                                               const uint32_t* __restrict__ parent, uint32_t* ready,
-                                                              float4* ncom, float4* nbmin, float4* nbmax, float inv_theta,
-                                                              uint32_t* __restrict__ pend, uint32_t* __restrict__ npend,
+                                                              float4* ncom, float4* nbmin, float4* nbmax, float4* nqa,
+                                                              float2* nqb, float inv_theta, uint32_t* __restrict__ pend,
+                                                              uint32_t* __restrict__ npend,
                                                               const uint32_t* __restrict__ gate) {
   if (gate && *gate == 0u) return;
   const int64_t c = (int64_t)blockIdx.x * UPS_LEAVES + threadIdx.x;
@@ -341,8 +368,20 @@ __global__ __launch_bounds__(UPS_LEAVES) void k_upsweep_local(const float4* __re
     cm.y = my / m;
     cm.z = mz / m;
   }
+  // Second pass for the quadrupole about the leaf's centre of mass (no large-coordinate cancellation).
+  float4 qa = make_float4(0.f, 0.f, 0.f, 0.f);
+  float2 qb = make_float2(0.f, 0.f);
+  if (end - start > 1 && m > 0.f) {
+    const float im = 1.0f / m;
+    for (int64_t s = start; s < end; ++s) {
+      const float4 b = xm[s];
+      quad_shift(qa, qb, b, make_float4(0.f, 0.f, 0.f, 0.f), make_float2(0.f, 0.f), cm, b.w * im);
+    }
+  }
   const uint32_t li = (uint32_t)(L - 1 + c);
   ncom[li] = cm;
+  nqa[li] = qa;
+  nqb[li] = qb;
   nbmin[li] = make_float4(lo[0], lo[1], lo[2], hm);
   {
     const float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
@@ -359,7 +398,7 @@ __global__ __launch_bounds__(UPS_LEAVES) void k_upsweep_local(const float4* __re
     }
     const uint32_t old = __hip_atomic_fetch_add(&ready[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
     if (old == 0u) break;
-    node_merge(p, child, ncom, nbmin, nbmax, inv_theta);
+    node_merge(p, child, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
     p = parent[p];
   }
 }
@@ -377,8 +416,8 @@ __global__ __launch_bounds__(256) void k_tree_rearm(int64_t L, uint32_t* __restr
 __global__ __launch_bounds__(1024) void k_upsweep_top(const uint32_t* __restrict__ pend,
                                                       const uint32_t* __restrict__ npend, const int2* __restrict__ child,
                                                       const uint32_t* __restrict__ parent, uint32_t* ready,
-                                                      float4* ncom, float4* nbmin, float4* nbmax, float inv_theta,
-                                                      const uint32_t* __restrict__ gate) {
+                                                      float4* ncom, float4* nbmin, float4* nbmax, float4* nqa,
+                                                      float2* nqb, float inv_theta, const uint32_t* __restrict__ gate) {
   if (gate && *gate == 0u) return;
   const uint32_t np = *npend;
   for (uint32_t e = threadIdx.x; e < np; e += blockDim.x) {
@@ -386,7 +425,7 @@ __global__ __launch_bounds__(1024) void k_upsweep_top(const uint32_t* __restrict
     while (q != IHS_NONE) {
       const uint32_t old = __hip_atomic_fetch_add(&ready[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (old == 0u) break;
-      node_merge(q, child, ncom, nbmin, nbmax, inv_theta);
+      node_merge(q, child, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
       q = parent[q];
     }
   }
@@ -573,14 +612,17 @@ void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb) {
     k_karras<<<grid_for(L - 1, 256), 256, 0, s>>>(tb.ccode, L, tb.child, tb.range, tb.parent, tb.ready, tb.npend);
 }
 
-void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const uint32_t* gate, float diag_hs) {
+void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const uint32_t* gate, float diag_hs, bool gravity) {
   const int64_t L = (n + K - 1) / K;
   const int nblk = (int)grid_for(L, GRP_BLK);
+  const int gmax = gravity ? tb.group_max : GROUP_MAX;
+  int4* out = gravity ? tb.ggroups : tb.groups;
+  uint32_t* nout = gravity ? tb.nggroups : tb.ngroups;
   k_mark_groups<<<grid_for(L, 256), 256, 0, s>>>(n, K, L, tb.range, tb.parent, tb.nbmin, tb.nbmax, tb.gstart, gate,
-                                                 diag_hs, tb.group_max);
+                                                 diag_hs, gmax);
   k_group_count<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.gblk, gate);
-  k_group_offsets<<<1, GRP_BLK, 0, s>>>(tb.gblk, nblk, tb.ngroups, tb.wq, gate);
-  k_group_emit<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.range, tb.gblk, tb.groups, gate);
+  k_group_offsets<<<1, GRP_BLK, 0, s>>>(tb.gblk, nblk, nout, tb.wq, gate);
+  k_group_emit<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.range, tb.gblk, out, gate);
 }
 
 void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb,
@@ -589,9 +631,9 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
   const float inv_theta = theta > 0.f ? 1.0f / theta : 3.0e38f;
   if (gate) k_tree_rearm<<<grid_for(L, 256), 256, 0, s>>>(L, tb.ready, tb.npend, gate);
   k_upsweep_local<<<grid_for(L, UPS_LEAVES), UPS_LEAVES, 0, s>>>(xm, hs, n, K, L, tb.child, tb.range, tb.parent,
-                                                                tb.ready, tb.ncom, tb.nbmin, tb.nbmax, inv_theta,
-                                                                tb.pend, tb.npend, gate);
+                                                                tb.ready, tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb,
+                                                                inv_theta, tb.pend, tb.npend, gate);
   if (L > UPS_LEAVES)
     k_upsweep_top<<<1, 1024, 0, s>>>(tb.pend, tb.npend, tb.child, tb.parent, tb.ready, tb.ncom, tb.nbmin, tb.nbmax,
-                                     inv_theta, gate);
+                                     tb.nqa, tb.nqb, inv_theta, gate);
 }

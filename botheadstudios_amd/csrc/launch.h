@@ -43,7 +43,7 @@ struct TreeBufs {
   int64_t cap = 0;   // particle capacity
   int64_t lcap = 0;  // leaf capacity
   int nb_cap = 0;    // radix tile capacity
-  int group_max = 64;  // walk-group size limit (K <= group_max <= 64 lanes)
+  int group_max = 64;  // gravity walk-group size limit (K <= group_max <= 2 * GROUP_MAX: up to 2 targets per lane)
   uint32_t* bbox = nullptr;  // 6 order-preserving keys
   uint32_t* keys[2] = {nullptr, nullptr};
   uint32_t* vals[2] = {nullptr, nullptr};
@@ -59,12 +59,18 @@ struct TreeBufs {
   float4* ncom = nullptr;   // 2L-1: com xyz, mass
   float4* nbmin = nullptr;  // 2L-1: bmin xyz, max search radius
   float4* nbmax = nullptr;  // 2L-1: bmax xyz, critical radius |bmax-bmin|/theta + |centre-com|
+  float4* nqa = nullptr;    // 2L-1: mass-normalised traceless quadrupole about the com,
+                            // sum (m / M) (3 y y^T - |y|^2 I): xx, yy, zz, xy
+  float2* nqb = nullptr;    // 2L-1: xz, yz
+  int quadrupole = 1;       // the walk adds the cells' quadrupole terms
   // walk groups: contiguous Morton ranges of at most group_max particles with compact boxes, in leaf order
   uint32_t* gstart = nullptr;   // L: group root node if leaf c is the first leaf of a group; GROUP_SPLIT if
                                 // leaf c is split into single-particle groups; none otherwise
   uint32_t* gblk = nullptr;     // per-1024-leaf block: group count, then exclusive offset
-  int4* groups = nullptr;       // (first particle, count, node, 0), up to n entries
+  int4* groups = nullptr;       // (first particle, count, node, 0), up to n entries: neighbour-search groups
   uint32_t* ngroups = nullptr;  // 1
+  int4* ggroups = nullptr;      // gravity-walk groups (the same list when group_max == GROUP_MAX)
+  uint32_t* nggroups = nullptr; // 1
   uint32_t* wq = nullptr;       // 16 work-queue counters (8 per walk kernel, one per XCD)
   // results of the last sort: sorted keys and the permutation (pointers into keys/vals)
   uint32_t* skeys = nullptr;
@@ -92,11 +98,12 @@ void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb
 // Leaf cluster codes + Karras topology + counters reset.
 void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb);
 // Walk groups in leaf order -> tb.groups / tb.ngroups, and resets the walk work queues. A group is the
-// topmost subtree on a leaf's root path holding <= tb.group_max particles whose box diagonal is at most
+// topmost subtree on a leaf's root path holding <= GROUP_MAX (gravity: tb.group_max) particles whose box
+// diagonal is at most
 // diag_hs times its largest search radius (no radius: no box limit); leaves without one are split into
 // single-particle groups. Needs the upsweep's boxes.
 void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const uint32_t* gate = nullptr,
-                   float diag_hs = GROUP_DIAG_HS);
+                   float diag_hs = GROUP_DIAG_HS, bool gravity = false);
 // Leaf moments and ordered upsweep; xm = sorted (x,y,z,m), hs = sorted search radius (or null). With a gate
 // the arrival counters are re-armed first (a second upsweep over the same topology).
 void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb,
@@ -105,8 +112,10 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
 // --- k_gravity.hip ---
 // Wave-coherent Barnes-Hut group walk over sorted particles; accg = (a/G xyz... see kernel), sorted order.
 // Persistent: waves pull groups from per-XCD queues (tb.wq[0..8)).
+// Groups come from tb.ggroups; pot = false skips the potential (accg.w = 0).
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, float soft2, const float4* xm,
-                         const TreeBufs& tb, float4* accg, uint32_t* flags, unsigned long long* counters);
+                         const TreeBufs& tb, float4* accg, uint32_t* flags, unsigned long long* counters,
+                         bool pot = true);
 // FP64 direct-sum accelerations (G included) at the particles with the given ids, from the state's positions
 // in storage order (accuracy probe of the walk). inv needs n entries, tpos nt.
 void launch_direct_f64_targets(hipStream_t s, int split, const void* px, const void* py, const void* pz, const float* m,

@@ -67,8 +67,11 @@ typedef struct {
   int32_t use_graph;     /* capture the KDK step in a hipGraph (1) */
   float h_margin_seeded; /* LEARNED: search radius = h_margin_seeded * seed after a drift (1.05); a wider-radius
                             pass (h_margin) runs only when some solve reaches it */
-  int32_t group_max;     /* particles per tree-walk group, leaf_size..64 (64): one wavefront walks one group */
-  int32_t reserved[5];
+  int32_t group_max;     /* particles per gravity-walk group, leaf_size..128 (64): one wavefront walks one group,
+                            each lane owning one (<= 64) or two (<= 128) particles */
+  int32_t quadrupole;    /* 1: accepted cells act with monopole + traceless quadrupole (default); 0: monopole only,
+                            as the reference (bh_gravity.wgsl:375-386) */
+  int32_t reserved[4];
 } ihs_config;
 
 typedef struct ihs_handle ihs_handle;

@@ -104,3 +104,51 @@ def test_direct_f64_probe_matches_oracle(precision):
     ref = oracle.direct_f64(pos, m32, np.float32(ic.softening), targets=ids.astype(np.int64))
     assert np.max(np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)) < 1e-12
     assert err["targets"] == 512 and err["rms"] < 2e-3, err
+
+
+def _far_cluster(dist, n=2000, seed=3):
+    """An elongated ball of radius ~1e6 m at the origin plus a clump of 64 light probes at distance `dist` along
+    (1,1,1): the cluster sits in the low Morton octant and the probes in the high one, so the root's left child
+    holds exactly the cluster and the probes' walk accepts it as one cell."""
+    rng = np.random.default_rng(seed)
+    u = rng.normal(size=(n, 3))
+    ball = u / np.linalg.norm(u, axis=1, keepdims=True) * (1e6 * rng.uniform(0, 1, (n, 1)) ** (1 / 3))
+    ball[:, 0] *= 1.7  # elongated: a non-zero quadrupole
+    probes = dist / np.sqrt(3.0) + rng.uniform(-1e3, 1e3, (64, 3))
+    pos = np.concatenate([ball, probes])
+    mass = np.concatenate([rng.uniform(0.5, 1.5, n) * 1e20, np.full(64, 1.0)])
+    return pos, mass
+
+
+@pytest.mark.parametrize("leaf", [1, 8])
+def test_quadrupole_far_field_order(leaf):
+    """Far field of an elongated cluster, seen from probes at distance D where the walk accepts the cluster's top
+    cells: the monopole error falls as (a/D)^2 and the quadrupole error as (a/D)^3, and the quadrupole error is
+    far below the monopole's. This fixes the sign and the 5/2 factor of the quadrupole term, and the
+    parallel-axis shift of the upsweep (any of those wrong leaves an O((a/D)^2) error)."""
+    errs = {}
+    for quad in (False, True):
+        for dist in (2.0e7, 4.0e7):
+            pos, mass = _far_cluster(dist)
+            ref = oracle.direct_f64(pos, mass.astype(np.float32).astype(np.float64), 0.0,
+                                    targets=np.arange(2000, 2064))
+            with GpuSph(4096, leaf_size=leaf, quadrupole=quad) as sph:
+                acc = sph.gravity(pos, mass, 0.0, theta=0.9, mode="tree")[2000:]
+            errs[(quad, dist)] = np.sqrt(np.mean(np.sum((acc - ref) ** 2, 1) / np.sum(ref ** 2, 1)))
+    mono_ratio = errs[(False, 2.0e7)] / errs[(False, 4.0e7)]
+    quad_ratio = errs[(True, 2.0e7)] / errs[(True, 4.0e7)]
+    assert 3.0 < mono_ratio < 5.5, errs
+    assert quad_ratio > 6.0, errs
+    assert errs[(True, 2.0e7)] < 0.2 * errs[(False, 2.0e7)], errs
+
+
+def test_quadrupole_never_worse_on_the_cloud(cloud):
+    """On the 1500-body cloud the quadrupole walk's RMS error is below the monopole walk's at every theta."""
+    pos, mass, soft, ref = cloud
+    for theta in (0.3, 0.5, 0.8):
+        r = {}
+        for quad in (False, True):
+            with GpuSph(2048, leaf_size=1, quadrupole=quad) as sph:
+                acc = sph.gravity(pos, mass, soft, theta=theta, mode="tree")
+            r[quad], _ = rms_rel(acc, ref)
+        assert r[True] < r[False], (theta, r)
