@@ -180,6 +180,8 @@ def main():
                          "mean_neighbours": v["stats"]["mean_neighbours"],
                          "gravity_rms_rel_err": v["gravity_error"]["rms"], "gravity_max_rel_err": v["gravity_error"]["max"],
                          "h_capped": v["stats"]["h_capped"], "h_unconverged": v["stats"]["h_unconverged"],
+                         "nbr_candidates_per_particle": v["stats"]["nbr_candidates"] / max(1, v["stats"]["force_evals"]) / ic.n,
+                         "walk_nodes_per_group_eval": v["stats"]["walk_nodes"] / max(1, v["stats"]["force_evals"]),
                          "stage_ms_per_step": {s: round(ms / max(args.profile_steps, 1), 4)
                                                for s, (ms, c) in v["profile"].items()}}
                      for k, v in variants.items()},

@@ -285,7 +285,9 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cc,
 // child to arrive (arrival counter with workgroup-scope acquire/release) as long as the node's leaf
 // range lies inside this block's leaves. A climb that reaches a node spanning several blocks records that
 // arrival instead. Phase 2 (one workgroup) replays the recorded arrivals up the remaining top nodes.
-constexpr int UPS_LEAVES = 1024;
+constexpr int UPS_THREADS = 1024;
+constexpr int UPS_PER_THREAD = 4;
+constexpr int UPS_LEAVES = UPS_THREADS * UPS_PER_THREAD;  // leaves per block of the first phase
 
 // Mass-normalised traceless quadrupole q = Q / M (units m^2: Q itself overflows FP32 in SI units) merged about
 // the parent's centre of mass (parallel-axis shift): q = sum_c (m_c / M) (q_c + 3 s s^T - |s|^2 I),
@@ -302,9 +304,8 @@ __device__ __forceinline__ void quad_shift(float4& qa, float2& qb, const float4 
   qb.y += w * (qcb.y + 3.0f * sy * sz);
 }
 
-__device__ __forceinline__ void node_merge(uint32_t p, const int2* __restrict__ child, float4* ncom, float4* nbmin,
-                                           float4* nbmax, float4* nqa, float2* nqb, float inv_theta) {
-  const int2 ch = child[p];
+__device__ __forceinline__ void node_merge(uint32_t p, const int2 ch, float4* ncom, float4* nbmin, float4* nbmax,
+                                           float4* nqa, float2* nqb, float inv_theta) {
   const float4 cl = ncom[ch.x], cr = ncom[ch.y];
   const float4 bl0 = nbmin[ch.x], br0 = nbmin[ch.y];
   const float4 bl1 = nbmax[ch.x], br1 = nbmax[ch.y];
@@ -337,18 +338,10 @@ __device__ __forceinline__ void node_merge(uint32_t p, const int2* __restrict__ 
   nqb[p] = qb;
 }
 
-__global__ __launch_bounds__(UPS_LEAVES) void k_upsweep_local(const float4* __restrict__ xm, const float* __restrict__ hs,
-                                                              int64_t n, int K, int64_t L, const int2* __restrict__ child,
-                                                              const int2* __restrict__ range,
-                                                              const uint32_t* __restrict__ parent, uint32_t* ready,
-                                                              float4* ncom, float4* nbmin, float4* nbmax, float4* nqa,
-                                                              float2* nqb, float inv_theta, uint32_t* __restrict__ pend,
-                                                              uint32_t* __restrict__ npend,
-                                                              const uint32_t* __restrict__ gate) {
-  if (gate && *gate == 0u) return;
-  const int64_t c = (int64_t)blockIdx.x * UPS_LEAVES + threadIdx.x;
-  if (c >= L) return;
-  const int64_t blo = (int64_t)blockIdx.x * UPS_LEAVES, bhi = blo + UPS_LEAVES;
+// Leaf moments of leaf c (K particles) into the node arrays.
+__device__ __forceinline__ void leaf_moments(int64_t c, const float4* __restrict__ xm, const float* __restrict__ hs,
+                                             int64_t n, int K, int64_t L, float4* ncom, float4* nbmin, float4* nbmax,
+                                             float4* nqa, float2* nqb, float inv_theta) {
   const int64_t start = c * K, end = min(start + (int64_t)K, n);
   float m = 0.f, mx = 0.f, my = 0.f, mz = 0.f, hm = 0.f;
   float lo[3] = {1.0e30f, 1.0e30f, 1.0e30f}, hi[3] = {-1.0e30f, -1.0e30f, -1.0e30f};
@@ -383,23 +376,47 @@ __global__ __launch_bounds__(UPS_LEAVES) void k_upsweep_local(const float4* __re
   nqa[li] = qa;
   nqb[li] = qb;
   nbmin[li] = make_float4(lo[0], lo[1], lo[2], hm);
-  {
-    const float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
-    const float ex = 0.5f * (lo[0] + hi[0]) - cm.x, ey = 0.5f * (lo[1] + hi[1]) - cm.y, ez = 0.5f * (lo[2] + hi[2]) - cm.z;
-    nbmax[li] = make_float4(hi[0], hi[1], hi[2],
-                            sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta + sqrtf(ex * ex + ey * ey + ez * ez));
-  }
-  uint32_t p = parent[li];
-  while (p != IHS_NONE) {
-    const int2 rg = range[p];
-    if (rg.x < blo || rg.y >= bhi) {  // top node: phase 2 replays this arrival
-      pend[atomicAdd(npend, 1u)] = p;
-      break;
+  const float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
+  const float ex = 0.5f * (lo[0] + hi[0]) - cm.x, ey = 0.5f * (lo[1] + hi[1]) - cm.y, ez = 0.5f * (lo[2] + hi[2]) - cm.z;
+  nbmax[li] = make_float4(hi[0], hi[1], hi[2],
+                          sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta + sqrtf(ex * ex + ey * ey + ez * ez));
+}
+
+// Phase 1. Arrival counters of the block's internal nodes live in LDS (node i of the block at i - blo: a node
+// whose leaf range lies inside the block has its index there); a node's parent, children and range are
+// fetched before its counter, so each level of a climb costs one LDS atomic and one round of global loads.
+__global__ __launch_bounds__(UPS_THREADS) void k_upsweep_local(const float4* __restrict__ xm, const float* __restrict__ hs,
+                                                               int64_t n, int K, int64_t L, const int2* __restrict__ child,
+                                                               const int2* __restrict__ range,
+                                                               const uint32_t* __restrict__ parent, float4* ncom,
+                                                               float4* nbmin, float4* nbmax, float4* nqa, float2* nqb,
+                                                               float inv_theta, uint32_t* __restrict__ pend,
+                                                               uint32_t* __restrict__ npend,
+                                                               const uint32_t* __restrict__ gate) {
+  if (gate && *gate == 0u) return;
+  __shared__ uint32_t lready[UPS_LEAVES];
+  for (int i = threadIdx.x; i < UPS_LEAVES; i += UPS_THREADS) lready[i] = 0u;
+  __syncthreads();
+  const int64_t blo = (int64_t)blockIdx.x * UPS_LEAVES, bhi = blo + UPS_LEAVES;
+  for (int j = 0; j < UPS_PER_THREAD; ++j) {
+    const int64_t c = blo + (int64_t)j * UPS_THREADS + threadIdx.x;
+    if (c >= L) break;
+    leaf_moments(c, xm, hs, n, K, L, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
+    uint32_t p = parent[L - 1 + c];
+    while (p != IHS_NONE) {
+      const int2 rg = range[p];
+      const int2 ch = child[p];
+      const uint32_t pp = parent[p];
+      if (rg.x < blo || rg.y >= bhi) {  // top node: phase 2 replays this arrival
+        pend[atomicAdd(npend, 1u)] = p;
+        break;
+      }
+      const uint32_t old =
+          __hip_atomic_fetch_add(&lready[p - blo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0u) break;
+      node_merge(p, ch, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
+      p = pp;
     }
-    const uint32_t old = __hip_atomic_fetch_add(&ready[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old == 0u) break;
-    node_merge(p, child, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
-    p = parent[p];
   }
 }
 
@@ -423,10 +440,12 @@ __global__ __launch_bounds__(1024) void k_upsweep_top(const uint32_t* __restrict
   for (uint32_t e = threadIdx.x; e < np; e += blockDim.x) {
     uint32_t q = pend[e];
     while (q != IHS_NONE) {
+      const int2 ch = child[q];
+      const uint32_t qp = parent[q];
       const uint32_t old = __hip_atomic_fetch_add(&ready[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (old == 0u) break;
-      node_merge(q, child, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
-      q = parent[q];
+      node_merge(q, ch, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
+      q = qp;
     }
   }
 }
@@ -630,9 +649,9 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
   const int64_t L = (n + K - 1) / K;
   const float inv_theta = theta > 0.f ? 1.0f / theta : 3.0e38f;
   if (gate) k_tree_rearm<<<grid_for(L, 256), 256, 0, s>>>(L, tb.ready, tb.npend, gate);
-  k_upsweep_local<<<grid_for(L, UPS_LEAVES), UPS_LEAVES, 0, s>>>(xm, hs, n, K, L, tb.child, tb.range, tb.parent,
-                                                                tb.ready, tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb,
-                                                                inv_theta, tb.pend, tb.npend, gate);
+  k_upsweep_local<<<grid_for(L, UPS_LEAVES), UPS_THREADS, 0, s>>>(xm, hs, n, K, L, tb.child, tb.range, tb.parent,
+                                                                 tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb, inv_theta,
+                                                                 tb.pend, tb.npend, gate);
   if (L > UPS_LEAVES)
     k_upsweep_top<<<1, 1024, 0, s>>>(tb.pend, tb.npend, tb.child, tb.parent, tb.ready, tb.ncom, tb.nbmin, tb.nbmax,
                                      tb.nqa, tb.nqb, inv_theta, gate);
