@@ -26,6 +26,10 @@ METRIC = "SPH+gravity particle-steps/s at 1M particles, learned-h vs plain, equa
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 FLOPS_PER_INTERACTION = 20.0  # softened monopole / particle-particle interaction (standard count)
+# Monopole + traceless quadrupole cell: 50 flops for the acceleration (see interact_quad in k_gravity.hip:
+# 3 sub, 3 FMA r^2, rsqrt, 5 mul for M r^-k, 15 for q.d, 5 for d.q.d, 3 for the radial factor, 12 for the
+# accumulation) plus 4 for the potential, which only the second evaluation of a step computes: 52 on average.
+FLOPS_PER_QUAD_CELL = 52.0
 
 
 def parse():
@@ -89,7 +93,8 @@ def run_variant(ic, precision, h_mode, args, weights=None):
 
 
 def roofline(res, n, args):
-    """Dominant stage -> achieved vs peak. Gravity (FP32 VALU): 20 flops per evaluated interaction."""
+    """Dominant stage -> achieved vs peak. Gravity (FP32 VALU): 20 flops per particle-particle or monopole cell
+    interaction, 52 per quadrupole cell interaction."""
     prof = res["profile"]
     stage = max(prof, key=lambda k: prof[k][0])
     ms, calls = prof[stage]
@@ -97,7 +102,9 @@ def roofline(res, n, args):
     pst = res["profile_stats"]
     if stage == "gravity":
         inter = pst["grav_cell_interactions"] + pst["grav_pp_interactions"]
-        flops = FLOPS_PER_INTERACTION * inter / max(calls, 1)
+        cell_flops = FLOPS_PER_QUAD_CELL if args.quadrupole else FLOPS_PER_INTERACTION
+        flops = (cell_flops * pst["grav_cell_interactions"] +
+                 FLOPS_PER_INTERACTION * pst["grav_pp_interactions"]) / max(calls, 1)
         achieved = flops / (avg_ms * 1e-3) / 1e12
         return dict(kernel="k_gravity_walk", bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS,
                     unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=None, avg_ms=avg_ms,
