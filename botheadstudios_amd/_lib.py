@@ -42,6 +42,7 @@ EXPORTS = [
     "ihs_count", "ihs_sync", "ihs_stats_get", "ihs_gravity", "ihs_debug_tree", "ihs_debug_field",
     "ihs_profile", "ihs_profile_read", "ihs_mlp_set_weights", "ihs_mlp_get_weights", "ihs_mlp_record",
     "ihs_mlp_samples", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict", "ihs_direct_f64",
+    "ihs_upload_batch", "ihs_segment_clock",
 ]
 
 MLP_NPARAM = 1313
@@ -95,10 +96,13 @@ def load(path: str | None = None):
     L.ihs_mlp_gradient.argtypes = [vp, vp, vp, i64, vp]
     L.ihs_mlp_predict.argtypes = [vp, vp, i64, vp]
     L.ihs_direct_f64.argtypes = [vp, vp, i64, vp]
+    L.ihs_upload_batch.argtypes = [vp, vp, vp, i32, vp, i32, vp]
+    L.ihs_segment_clock.argtypes = [vp, i32, vp, vp, vp]
     for name in ("ihs_create", "ihs_destroy", "ihs_upload", "ihs_set_dt", "ihs_set_av", "ihs_set_omega",
                  "ihs_step_kdk", "ihs_relax", "ihs_force_eval", "ihs_sync", "ihs_stats_get", "ihs_gravity",
                  "ihs_debug_field", "ihs_profile", "ihs_profile_read", "ihs_mlp_set_weights", "ihs_mlp_get_weights",
-                 "ihs_mlp_record", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict", "ihs_direct_f64"):
+                 "ihs_mlp_record", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict", "ihs_direct_f64",
+                 "ihs_upload_batch", "ihs_segment_clock"):
         getattr(L, name).restype = C.c_int
     if path is None:
         _lib = L

@@ -35,6 +35,21 @@ struct Scratch {
   double* eout = nullptr;
   StepCtl* ctl = nullptr;
   void* eos = nullptr;
+  // segments (scenarios of a batch; one for a single upload)
+  uint32_t* segof = nullptr;    // capacity
+  uint32_t* seg_start = nullptr;  // IHS_MAX_SEGMENTS + 1
+  float* seg_soft2 = nullptr;
+  double* seg_soft2d = nullptr;  // the same in FP64, for the direct-sum probe
+  float* seg_hcap = nullptr;
+  SegClock* seg_clk = nullptr;
+  uint32_t* seg_bbox = nullptr;   // 6 IHS_MAX_SEGMENTS
+  uint32_t* seg_root = nullptr;
+  // the standalone gravity call's single segment
+  uint32_t* g_start = nullptr;  // 2
+  float* g_soft2 = nullptr;
+  float* g_hcap = nullptr;
+  SegClock* g_clk = nullptr;
+  uint32_t* g_root = nullptr;
   ihs_particle* staging = nullptr;  // device AoS for upload/readback
   // standalone gravity
   double* gpos = nullptr;
@@ -60,8 +75,9 @@ struct ihs_handle {
   TreeBufs tb;
   Scratch sc;
   MlpDev mlp;
-  float softening = 0.f;
-  float h_cap = 0.f;
+  int nseg = 1;
+  std::vector<int64_t> seg_start_host;  // nseg + 1
+  bool zero_soft = false;               // some scenario has no softening: the walk skips self pairs by index
   int64_t force_evals = 0;
   bool have_fe = false;
   // The last force evaluation solved h at the current positions (nothing has drifted since), so the next
@@ -192,6 +208,20 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &s.epart, 9 * 1024));
   HIP_TRY(h, dalloc(h, &s.eout, 16));
   HIP_TRY(h, dalloc(h, &s.ctl, 1));
+  HIP_TRY(h, dalloc(h, &s.segof, C));
+  HIP_TRY(h, dalloc(h, &s.seg_start, IHS_MAX_SEGMENTS + 1));
+  HIP_TRY(h, dalloc(h, &s.seg_soft2, IHS_MAX_SEGMENTS));
+  HIP_TRY(h, dalloc(h, &s.seg_soft2d, IHS_MAX_SEGMENTS));
+  HIP_TRY(h, dalloc(h, &s.seg_hcap, IHS_MAX_SEGMENTS));
+  HIP_TRY(h, dalloc(h, &s.seg_clk, IHS_MAX_SEGMENTS));
+  HIP_TRY(h, dalloc(h, &s.seg_bbox, 6 * IHS_MAX_SEGMENTS));
+  HIP_TRY(h, dalloc(h, &s.seg_root, IHS_MAX_SEGMENTS));
+  HIP_TRY(h, dalloc(h, &s.g_start, 2));
+  HIP_TRY(h, dalloc(h, &s.g_soft2, 1));
+  HIP_TRY(h, dalloc(h, &s.g_hcap, 1));
+  HIP_TRY(h, dalloc(h, &s.g_clk, 1));
+  HIP_TRY(h, dalloc(h, &s.g_root, 1));
+  HIP_TRY(h, dalloc(h, &t.codes0, C));
   HIP_TRY(h, alloc_raw(h, &s.eos, 16 * sizeof(ihs_eos)));
   HIP_TRY(h, dalloc(h, &s.staging, C));
   HIP_TRY(h, dalloc(h, &h->hw.pend, C));
@@ -243,6 +273,19 @@ void prof_flush(ihs_handle* h) {
 
 bool valid_leaf(int k) { return k >= 1 && k <= 64 && (k & (k - 1)) == 0; }
 
+SegView seg_view(ihs_handle* h) {
+  SegView v;
+  v.S = h->nseg;
+  v.segof = h->nseg > 1 ? h->sc.segof : nullptr;
+  v.start = h->sc.seg_start;
+  v.soft2 = h->sc.seg_soft2;
+  v.hcap = h->sc.seg_hcap;
+  v.clk = h->sc.seg_clk;
+  v.bbox = h->sc.seg_bbox;
+  v.root = h->sc.seg_root;
+  return v;
+}
+
 // Search-radius margin of an evaluation whose h was solved at unchanged positions: room for round-off only.
 constexpr float kMarginSettled = 1.02f;
 
@@ -262,48 +305,50 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   const bool seeded = h->cfg.h_mode == IHS_H_LEARNED && after_drift && h->mlp.trained;
   const bool record = h->mlp.record && after_drift && h->cfg.h_mode == IHS_H_NEWTON;
   const float eta = std::cbrt(3.0f * h->cfg.n_ngb / (4.0f * kPi));
+  const SegView sv = seg_view(h);
   prof_mark(h, -1);
-  launch_fe_init(s, sc.flags);
-  launch_bbox_morton(s, h->split, S.px, S.py, S.pz, n, h->tb);
+  launch_fe_init(s, sc.flags, sv);
+  launch_bbox_morton(s, h->split, S.px, S.py, S.pz, n, h->tb, sv);
   launch_radix_sort(s, n, 30, h->tb);
+  launch_segment_order(s, n, h->tb, sv);
   launch_fix_ties(s, n, S.id, h->tb);
   prof_mark(h, IHS_STAGE_SORT);
   float margin = 1.0f;
   if (newton) margin = (!after_drift && h->h_settled) ? kMarginSettled : h->cfg.h_margin;
-  launch_gather(s, h->split, n, h->tb.perm, S, D, sc.xm, sc.hs, margin, h->h_cap);
+  launch_gather(s, h->split, n, h->tb.perm, S, D, sc.xm, sc.hs, margin, sv);
   h->cur ^= 1;
   prof_mark(h, IHS_STAGE_GATHER);
   if (seeded) {
-    mlp_seed_h(s, h->mlp, n, D, sc.hs, sc.eos, sc.ctl, eta, h->cfg.h_margin_seeded, h->h_cap);
+    mlp_seed_h(s, h->mlp, n, D, sc.hs, sc.eos, sv, eta, h->cfg.h_margin_seeded);
     prof_mark(h, IHS_STAGE_HSOLVE);
   }
   // Walk groups are bounded in units of h, whatever margin set this evaluation's search radius.
   const float eff_margin = seeded ? h->cfg.h_margin_seeded : margin;
   const float diag_hs = newton ? GROUP_DIAG_HS * h->cfg.h_margin / eff_margin : GROUP_DIAG_HS;
-  launch_karras(s, n, K, h->tb);
+  launch_karras(s, n, K, h->tb, sv);
   launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb);
-  launch_groups(s, n, K, h->tb, nullptr, diag_hs);
-  if (h->tb.group_max != GROUP_MAX) launch_groups(s, n, K, h->tb, nullptr, diag_hs, true);
+  launch_groups(s, n, K, h->tb, sv, nullptr, diag_hs);
+  if (h->tb.group_max != GROUP_MAX) launch_groups(s, n, K, h->tb, sv, nullptr, diag_hs, true);
   prof_mark(h, IHS_STAGE_TREE);
   launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags, sc.counters);
   prof_mark(h, IHS_STAGE_NBR);
   if (newton) {
     const float eta3 = eta * eta * eta;
-    if (record) mlp_record_features(s, h->mlp, n, D, sc.eos, sc.ctl, eta);
+    if (record) mlp_record_features(s, h->mlp, n, D, sc.eos, sv, eta);
     launch_hsolve(s, n, sc.xm, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol, h->cfg.h_max_iter,
-                  h->hw, sc.flags, sc.counters, nullptr, seeded ? sc.flags + FLAG_H_RETRY : nullptr, h->h_cap);
+                  h->hw, sc.flags, sc.counters, sv, nullptr, seeded ? sc.flags + FLAG_H_RETRY : nullptr);
     if (record) mlp_record_targets(s, h->mlp, n, D);
     prof_mark(h, IHS_STAGE_HSOLVE);
     if (seeded) {
       // Wider-radius pass, gated on the device: every kernel returns at once unless a solve hit the tight radius.
       const uint32_t* gate = sc.flags + FLAG_H_RETRY;
-      launch_retry_hs(s, n, D.h, sc.hs, h->cfg.h_margin, h->h_cap, sc.flags);
+      launch_retry_hs(s, n, D.h, sc.hs, h->cfg.h_margin, sv, sc.flags);
       launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, gate);
-      launch_groups(s, n, K, h->tb, gate);
+      launch_groups(s, n, K, h->tb, sv, gate);
       launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags,
                        sc.counters, gate);
       launch_hsolve(s, n, sc.xm, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol, h->cfg.h_max_iter,
-                    h->hw, sc.flags, sc.counters, gate, nullptr, h->h_cap);
+                    h->hw, sc.flags, sc.counters, sv, gate, nullptr);
       prof_mark(h, IHS_STAGE_HSOLVE);
     }
     h->h_settled = true;
@@ -311,13 +356,12 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   // The network's inputs come from the evaluation before a drift.
   const int feat = (h->cfg.h_mode == IHS_H_LEARNED || h->mlp.record) && !after_drift;
   launch_density(s, h->split, feat, n, sc.xm, D.h, D.vx, D.vy, D.vz, D.u, D.mat, sc.eos, sc.nbr, sc.ncount, n, sc.rpc,
-                 sc.vh, D.rho, D.feat, sc.flags);
+                 sc.vh, D.rho, D.feat, sc.flags, sv);
   prof_mark(h, IHS_STAGE_DENSITY);
   launch_forces(s, kahan, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, n,
                 (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph);
   prof_mark(h, IHS_STAGE_FORCES);
-  launch_gravity_walk(s, n, K, kahan, h->softening * h->softening, sc.xm, h->tb, sc.accg, sc.flags, sc.counters,
-                      need_pot);
+  launch_gravity_walk(s, n, K, kahan, sc.xm, h->tb, sv, sc.accg, sc.flags, sc.counters, need_pot, h->zero_soft);
   prof_mark(h, IHS_STAGE_GRAVITY);
   h->force_evals++;
   h->have_fe = true;
@@ -325,11 +369,11 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
 
 void kdk_step(ihs_handle* h) {
   force_eval(h, false, false);  // the potential is read only after the step's second evaluation
-  launch_kick_drift(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, h->sc.flags);
+  launch_kick_drift(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, seg_view(h));
   h->h_settled = false;
   prof_mark(h, IHS_STAGE_KICK);
   force_eval(h, true);
-  launch_kick(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl);
+  launch_kick(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, seg_view(h));
   prof_mark(h, IHS_STAGE_KICK);
   prof_flush(h);
 }
@@ -448,37 +492,73 @@ int ihs_destroy(ihs_handle* h) {
 
 const char* ihs_last_error(const ihs_handle* h) { return h ? h->err.c_str() : "null handle"; }
 
-int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* eos, int32_t n_mat, float softening) {
+int ihs_upload_batch(ihs_handle* h, const ihs_particle* p, const int64_t* counts, int32_t nseg, const ihs_eos* eos,
+                     int32_t n_mat, const float* softening) {
   if (!h) return IHS_ERR_ARG;
-  if (n < 1 || !p) return fail(h, IHS_ERR_ARG, "upload needs at least one particle");
+  if (!p || !counts || !softening || nseg < 1 || nseg > IHS_MAX_SEGMENTS)
+    return fail(h, IHS_ERR_ARG, "a batch needs 1..%d scenarios", IHS_MAX_SEGMENTS);
+  if (!eos || n_mat < 1 || n_mat > 16) return fail(h, IHS_ERR_ARG, "need 1..16 EOS records, got %d", n_mat);
+  const int K = h->cfg.leaf_size;
+  std::vector<int64_t> start(nseg + 1, 0);
+  for (int sg = 0; sg < nseg; ++sg) {
+    if (counts[sg] < 1) return fail(h, IHS_ERR_ARG, "scenario %d has no particles", sg);
+    if (nseg > 1 && counts[sg] % K != 0)
+      return fail(h, IHS_ERR_ARG, "scenario %d: %lld particles is not a multiple of the leaf size %d (a leaf may not "
+                  "straddle two scenarios)", sg, (long long)counts[sg], K);
+    if (!(softening[sg] >= 0.f)) return fail(h, IHS_ERR_ARG, "scenario %d has negative softening", sg);
+    start[sg + 1] = start[sg] + counts[sg];
+  }
+  const int64_t n = start[nseg];
   if (n > h->cap) return fail(h, IHS_ERR_CAPACITY, "upload of %lld particles exceeds capacity %lld", (long long)n,
                               (long long)h->cap);
-  if (!eos || n_mat < 1 || n_mat > 16) return fail(h, IHS_ERR_ARG, "need 1..16 EOS records, got %d", n_mat);
-  float hmax = 0.f;
-  for (int64_t i = 0; i < n; ++i) {
-    if (p[i].mat >= (uint32_t)n_mat) return fail(h, IHS_ERR_ARG, "particle %lld has mat %u >= n_mat %d", (long long)i,
-                                                 p[i].mat, n_mat);
-    if (!(p[i].h > 0.f) || !std::isfinite(p[i].h)) return fail(h, IHS_ERR_ARG, "particle %lld has h <= 0", (long long)i);
-    hmax = std::max(hmax, p[i].h);
+  std::vector<float> hcap(nseg, 0.f), soft2(nseg);
+  std::vector<double> soft2d(nseg);
+  std::vector<uint32_t> segof(n), root(nseg), start32(nseg + 1);
+  const int64_t L = (n + K - 1) / K;
+  for (int sg = 0; sg < nseg; ++sg) {
+    float hmax = 0.f;
+    for (int64_t i = start[sg]; i < start[sg + 1]; ++i) {
+      if (p[i].mat >= (uint32_t)n_mat)
+        return fail(h, IHS_ERR_ARG, "particle %lld has mat %u >= n_mat %d", (long long)i, p[i].mat, n_mat);
+      if (!(p[i].h > 0.f) || !std::isfinite(p[i].h))
+        return fail(h, IHS_ERR_ARG, "particle %lld has h <= 0", (long long)i);
+      hmax = std::max(hmax, p[i].h);
+      segof[i] = (uint32_t)sg;
+    }
+    hcap[sg] = h->cfg.h_cap_factor * hmax;
+    soft2[sg] = softening[sg] * softening[sg];
+    soft2d[sg] = (double)softening[sg] * (double)softening[sg];
+    const int64_t leaf0 = start[sg] / K, nleaf = (start[sg + 1] - start[sg] + K - 1) / K;
+    root[sg] = nleaf >= 2 ? (uint32_t)leaf0 : (uint32_t)(L - 1 + leaf0);  // a one-leaf tree is its leaf
+    start32[sg] = (uint32_t)start[sg];
   }
+  start32[nseg] = (uint32_t)n;
   hipSetDevice(h->device);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   drop_graph(h);
   h->n = n;
+  h->nseg = nseg;
+  h->zero_soft = false;
+  for (int sg = 0; sg < nseg; ++sg) h->zero_soft |= !(soft2[sg] > 0.f);
+  h->seg_start_host = start;
   h->cur = 0;
-  h->softening = softening;
-  h->h_cap = h->cfg.h_cap_factor * hmax;
   h->have_fe = false;
   h->h_settled = false;
-  HIP_TRY(h, hipMemcpy(h->sc.eos, eos, n_mat * sizeof(ihs_eos), hipMemcpyHostToDevice));
-  HIP_TRY(h, hipMemcpy(h->sc.staging, p, n * sizeof(ihs_particle), hipMemcpyHostToDevice));
-  launch_unpack(h->stream, h->split, n, (const SphParticle48*)h->sc.staging, h->st[0]);
-  StepCtl ctl;
-  HIP_TRY(h, hipMemcpy(&ctl, h->sc.ctl, sizeof ctl, hipMemcpyDeviceToHost));
-  ctl.t = 0.0;
-  ctl.dt = 0.0;
-  ctl.steps = 0;
-  HIP_TRY(h, hipMemcpy(h->sc.ctl, &ctl, sizeof ctl, hipMemcpyHostToDevice));
+  Scratch& sc = h->sc;
+  HIP_TRY(h, hipMemcpy(sc.eos, eos, n_mat * sizeof(ihs_eos), hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(sc.segof, segof.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(sc.seg_start, start32.data(), (nseg + 1) * sizeof(uint32_t), hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(sc.seg_soft2, soft2.data(), nseg * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(sc.seg_soft2d, soft2d.data(), nseg * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(sc.seg_hcap, hcap.data(), nseg * sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(sc.seg_root, root.data(), nseg * sizeof(uint32_t), hipMemcpyHostToDevice));
+  {
+    std::vector<SegClock> clk(nseg);
+    for (auto& c : clk) c = SegClock{0.0, 0.0, 0ull, 0xffffffffu, 0u};
+    HIP_TRY(h, hipMemcpy(sc.seg_clk, clk.data(), nseg * sizeof(SegClock), hipMemcpyHostToDevice));
+  }
+  HIP_TRY(h, hipMemcpy(sc.staging, p, n * sizeof(ihs_particle), hipMemcpyHostToDevice));
+  launch_unpack(h->stream, h->split, n, (const SphParticle48*)sc.staging, h->st[0]);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   HIP_TRY(h, hipGetLastError());
   if (h->cfg.h_mode != IHS_H_FIXED) {
@@ -487,11 +567,11 @@ int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* e
     uint32_t prev = 0xffffffffu;
     for (int pass = 0; pass < 16; ++pass) {
       uint32_t before = 0, after = 0;
-      HIP_TRY(h, hipMemcpy(&before, h->sc.flags + FLAG_H_CAPPED, 4, hipMemcpyDeviceToHost));
+      HIP_TRY(h, hipMemcpy(&before, sc.flags + FLAG_H_CAPPED, 4, hipMemcpyDeviceToHost));
       h->h_settled = false;  // every pass may grow h by the full margin
       force_eval(h, false);
       HIP_TRY(h, hipStreamSynchronize(h->stream));
-      HIP_TRY(h, hipMemcpy(&after, h->sc.flags + FLAG_H_CAPPED, 4, hipMemcpyDeviceToHost));
+      HIP_TRY(h, hipMemcpy(&after, sc.flags + FLAG_H_CAPPED, 4, hipMemcpyDeviceToHost));
       const uint32_t capped = after - before;
       if (capped == 0 || capped >= prev) break;
       prev = capped;
@@ -499,13 +579,31 @@ int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* e
     HIP_TRY(h, hipGetLastError());
   }
   // Counters describe stepping only.
-  HIP_TRY(h, hipMemset(h->sc.counters, 0, CNT_SLOTS * CNT_STRIDE * sizeof(unsigned long long)));
+  HIP_TRY(h, hipMemset(sc.counters, 0, CNT_SLOTS * CNT_STRIDE * sizeof(unsigned long long)));
   {
     uint32_t fl[FLAG_COUNT] = {0};
-    fl[FLAG_SIGNAL_MIN] = 0xffffffffu;
-    HIP_TRY(h, hipMemcpy(h->sc.flags, fl, sizeof fl, hipMemcpyHostToDevice));
+    HIP_TRY(h, hipMemcpy(sc.flags, fl, sizeof fl, hipMemcpyHostToDevice));
   }
   h->force_evals = 0;
+  return IHS_OK;
+}
+
+int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* eos, int32_t n_mat, float softening) {
+  if (!h) return IHS_ERR_ARG;
+  if (n < 1 || !p) return fail(h, IHS_ERR_ARG, "upload needs at least one particle");
+  return ihs_upload_batch(h, p, &n, 1, eos, n_mat, &softening);
+}
+
+int ihs_segment_clock(ihs_handle* h, int32_t seg, double* t, double* dt, int64_t* steps) {
+  if (!h) return IHS_ERR_ARG;
+  if (seg < 0 || seg >= h->nseg) return fail(h, IHS_ERR_ARG, "scenario %d out of range (%d uploaded)", seg, h->nseg);
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  SegClock c;
+  HIP_TRY(h, hipMemcpy(&c, h->sc.seg_clk + seg, sizeof c, hipMemcpyDeviceToHost));
+  if (t) *t = c.t;
+  if (dt) *dt = c.dt;
+  if (steps) *steps = (int64_t)c.steps;
   return IHS_OK;
 }
 
@@ -596,7 +694,7 @@ int ihs_relax(ihs_handle* h, int32_t steps) {
   hipSetDevice(h->device);
   for (int i = 0; i < steps; ++i) {
     force_eval(h, false);
-    launch_relax(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl);
+    launch_relax(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, seg_view(h));
     h->h_settled = false;
   }
   HIP_TRY(h, hipGetLastError());
@@ -651,9 +749,11 @@ int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset) {
     out->p[q] = e[3 + q];
     out->L[q] = e[6 + q];
   }
-  out->t = ctl.t;
-  out->dt = ctl.dt;
-  out->steps = (int64_t)ctl.steps;
+  SegClock clk0;
+  HIP_TRY(h, hipMemcpy(&clk0, h->sc.seg_clk, sizeof clk0, hipMemcpyDeviceToHost));
+  out->t = clk0.t;
+  out->dt = clk0.dt;
+  out->steps = (int64_t)clk0.steps;
   out->force_evals = h->force_evals;
   const double solved = (double)cn[CNT_H_SOLVED];
   out->mean_h_iters = solved > 0 ? (double)cn[CNT_H_ITERS] / solved : 0.0;
@@ -731,15 +831,28 @@ int ihs_gravity(ihs_handle* h, const double* pos3, const double* mass, int64_t n
     launch_acc_out(s, nullptr, n, sc.accg, sc.gacc, sc.gpot);
   } else {
     const int K = h->cfg.leaf_size;
-    launch_fe_init(s, sc.flags);
-    launch_bbox_morton(s, 1, sc.gpx, sc.gpy, sc.gpz, n, h->tb);
+    // One segment: these bodies, their softening, a walk from node 0.
+    const uint32_t st2[2] = {0u, (uint32_t)n}, root0 = 0u;
+    HIP_TRY(h, hipMemcpy(sc.g_start, st2, sizeof st2, hipMemcpyHostToDevice));
+    HIP_TRY(h, hipMemcpy(sc.g_soft2, &soft2, sizeof soft2, hipMemcpyHostToDevice));
+    HIP_TRY(h, hipMemcpy(sc.g_root, &root0, sizeof root0, hipMemcpyHostToDevice));
+    SegView gv;
+    gv.S = 1;
+    gv.start = sc.g_start;
+    gv.soft2 = sc.g_soft2;
+    gv.hcap = sc.g_hcap;
+    gv.clk = sc.g_clk;
+    gv.bbox = h->tb.bbox;
+    gv.root = sc.g_root;
+    launch_fe_init(s, sc.flags, gv);
+    launch_bbox_morton(s, 1, sc.gpx, sc.gpy, sc.gpz, n, h->tb, gv);
     launch_radix_sort(s, n, 30, h->tb);  // identity payload: ties already ordered by index
     launch_gather_xm(s, h->tb.perm, n, sc.gpx, sc.gpy, sc.gpz, sc.gm, sc.xm);
-    launch_karras(s, n, K, h->tb);
+    launch_karras(s, n, K, h->tb, gv);
     launch_upsweep(s, n, K, (float)theta, sc.xm, nullptr, h->tb);
-    launch_groups(s, n, K, h->tb);
-    if (h->tb.group_max != GROUP_MAX) launch_groups(s, n, K, h->tb, nullptr, GROUP_DIAG_HS, true);
-    launch_gravity_walk(s, n, K, 0, soft2, sc.xm, h->tb, sc.accg, sc.flags, sc.counters);
+    launch_groups(s, n, K, h->tb, gv);
+    if (h->tb.group_max != GROUP_MAX) launch_groups(s, n, K, h->tb, gv, nullptr, GROUP_DIAG_HS, true);
+    launch_gravity_walk(s, n, K, 0, sc.xm, h->tb, gv, sc.accg, sc.flags, sc.counters, true, !(soft2 > 0.f));
     launch_acc_out(s, h->tb.perm, n, sc.accg, sc.gacc, sc.gpot);
     h->have_fe = false;  // the tree buffers now describe these bodies, not the SPH state
   }
@@ -820,9 +933,8 @@ int ihs_direct_f64(ihs_handle* h, const uint32_t* ids, int64_t nt, double* acc3)
   if (e == hipSuccess) e = hipMemcpy(d_ids, ids, nt * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) {
     const MasterBufs& S = h->st[h->cur];
-    const double soft = h->softening;
-    launch_direct_f64_targets(h->stream, h->split, S.px, S.py, S.pz, S.m, S.id, h->n, d_ids, nt, soft * soft, h->hw.pend,
-                              d_tpos, d_acc);
+    launch_direct_f64_targets(h->stream, h->split, S.px, S.py, S.pz, S.m, S.id, h->n, d_ids, nt, h->sc.seg_soft2d,
+                              h->nseg > 1 ? h->sc.segof : nullptr, h->sc.seg_start, h->hw.pend, d_tpos, d_acc);
     e = hipStreamSynchronize(h->stream);
     if (e == hipSuccess) e = hipGetLastError();
     if (e == hipSuccess) e = hipMemcpy(acc3, d_acc, nt * 3 * sizeof(double), hipMemcpyDeviceToHost);

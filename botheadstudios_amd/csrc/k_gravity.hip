@@ -137,7 +137,7 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
   const uint32_t lpc = 64u / (uint32_t)K;
   uint32_t sp = 1, chead = 0, ctail = 0, lhead = 0, ltail = 0;
   wave_sync();
-  if (lane == 0) S.stack[0] = 0u;
+  if (lane == 0) S.stack[0] = (uint32_t)grp.w;  // the group's segment root
   wave_sync();
 
   // Accepted nodes: chead stays a multiple of 64 until the final flush, so a chunk is contiguous in the
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
                                                                  const float2* __restrict__ nqb,
                                                                  const int4* __restrict__ groups,
                                                                  const uint32_t* __restrict__ ngroups,
-                                                                 uint32_t* __restrict__ wq, float soft2,
+                                                                 uint32_t* __restrict__ wq, SegView sv,
                                                                  float4* __restrict__ accg, uint32_t* __restrict__ flags,
                                                                  unsigned long long* __restrict__ counters) {
   __shared__ WalkLds<QUAD> lds[GW_WAVES];
@@ -315,7 +315,9 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
       if (lane == 0) g = atomicAdd(&wq[q], 1u);
       g = (uint32_t)__builtin_amdgcn_readfirstlane((int)g) + lo;
       if (g >= hi) break;
-      walk_group<KAHAN, SELF, T, POT, QUAD>(S, groups[g], lane, xm, n, K, L, child, ncom, nbmax, nqa, nqb, soft2, accg,
+      const int4 grp = groups[g];
+      const float soft2 = sv.soft2[sv.of(grp.x)];
+      walk_group<KAHAN, SELF, T, POT, QUAD>(S, grp, lane, xm, n, K, L, child, ncom, nbmax, nqa, nqb, soft2, accg,
                                             flags, n_cell, n_pp, n_nodes);
     }
   }
@@ -387,28 +389,37 @@ constexpr int DT_TILE = 8;
 template <class PT>
 __global__ __launch_bounds__(256) void k_direct_f64(const PT* __restrict__ px, const PT* __restrict__ py,
                                                     const PT* __restrict__ pz, const float* __restrict__ m, int64_t n,
-                                                    const uint32_t* __restrict__ tpos, int64_t nt, double soft2,
-                                                    double* __restrict__ acc3) {
+                                                    const uint32_t* __restrict__ tpos, int64_t nt,
+                                                    const double* __restrict__ seg_soft2,
+                                                    const uint32_t* __restrict__ segof,
+                                                    const uint32_t* __restrict__ seg_start, double* __restrict__ acc3) {
   __shared__ double red[256];
   const int64_t t0 = (int64_t)blockIdx.x * DT_TILE;
-  double tx[DT_TILE], ty[DT_TILE], tz[DT_TILE], ax[DT_TILE], ay[DT_TILE], az[DT_TILE];
-  int64_t ti[DT_TILE];
+  double tx[DT_TILE], ty[DT_TILE], tz[DT_TILE], ax[DT_TILE], ay[DT_TILE], az[DT_TILE], s2[DT_TILE];
+  int64_t ti[DT_TILE], lo[DT_TILE], hi[DT_TILE];
+  int64_t jlo = n, jhi = 0;
 #pragma unroll
   for (int k = 0; k < DT_TILE; ++k) {
     const int64_t q = t0 + k < nt ? (int64_t)tpos[t0 + k] : 0;
+    const uint32_t sg = segof ? segof[q] : 0u;  // sources: the target's own scenario
     ti[k] = q;
     tx[k] = (double)px[q];
     ty[k] = (double)py[q];
     tz[k] = (double)pz[q];
+    s2[k] = (double)seg_soft2[sg];
+    lo[k] = seg_start[sg];
+    hi[k] = seg_start[sg + 1];
+    jlo = min(jlo, lo[k]);
+    jhi = max(jhi, hi[k]);
     ax[k] = ay[k] = az[k] = 0.0;
   }
-  for (int64_t j = threadIdx.x; j < n; j += blockDim.x) {
+  for (int64_t j = jlo + threadIdx.x; j < jhi; j += blockDim.x) {
     const double xj = (double)px[j], yj = (double)py[j], zj = (double)pz[j], mj = (double)m[j];
 #pragma unroll
     for (int k = 0; k < DT_TILE; ++k) {
-      if (j == ti[k]) continue;
+      if (j == ti[k] || j < lo[k] || j >= hi[k]) continue;
       const double dx = xj - tx[k], dy = yj - ty[k], dz = zj - tz[k];
-      const double r2 = dx * dx + dy * dy + dz * dz + soft2;
+      const double r2 = dx * dx + dy * dy + dz * dz + s2[k];
       const double rinv = 1.0 / sqrt(r2);
       const double f = mj * rinv * rinv * rinv;
       ax[k] += f * dx;
@@ -456,10 +467,10 @@ void launch_acc_out(hipStream_t s, const uint32_t* perm, int64_t n, const float4
   k_acc_out<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(perm, n, acc4, acc3, pot);
 }
 
-void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, float soft2, const float4* xm, const TreeBufs& tb,
-                         float4* accg, uint32_t* flags, unsigned long long* counters, bool pot) {
+void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
+                         const SegView& sv, float4* accg, uint32_t* flags, unsigned long long* counters, bool pot,
+                         bool self) {
   const int64_t L = (n + K - 1) / K;
-  const bool self = !(soft2 > 0.f);
   const bool two = tb.group_max > GROUP_MAX;
   const bool quad = tb.quadrupole != 0;
   static unsigned nblk[32] = {0};
@@ -469,7 +480,7 @@ void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, float soft2
     if (!nblk[variant])                                                                                       \
       nblk[variant] = persistent_blocks((const void*)k_gravity_walk<KA, SE, T, PO, QU>, GW_WAVES * 64);       \
     k_gravity_walk<KA, SE, T, PO, QU><<<nblk[variant], GW_WAVES * 64, 0, s>>>(                                \
-        xm, n, K, L, tb.child, tb.ncom, tb.nbmax, tb.nqa, tb.nqb, tb.ggroups, tb.nggroups, tb.wq, soft2, accg,  \
+        xm, n, K, L, tb.child, tb.ncom, tb.nbmax, tb.nqa, tb.nqb, tb.ggroups, tb.nggroups, tb.wq, sv, accg,     \
         flags, counters);                                                                                     \
   } while (0)
 #define IHS_WALK_P(KA, SE, T, QU)           \
@@ -505,15 +516,16 @@ void launch_gravity_direct(hipStream_t s, int64_t n, float soft2, const float4* 
 }
 
 void launch_direct_f64_targets(hipStream_t s, int split, const void* px, const void* py, const void* pz, const float* m,
-                               const uint32_t* id, int64_t n, const uint32_t* ids, int64_t nt, double soft2,
-                               uint32_t* inv, uint32_t* tpos, double* acc3) {
+                               const uint32_t* id, int64_t n, const uint32_t* ids, int64_t nt, const double* seg_soft2,
+                               const uint32_t* segof, const uint32_t* seg_start, uint32_t* inv, uint32_t* tpos,
+                               double* acc3) {
   k_storage_index<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(id, n, inv);
   k_map_ids<<<(unsigned)((nt + 255) / 256), 256, 0, s>>>(inv, ids, nt, tpos);
   const unsigned nb = (unsigned)((nt + DT_TILE - 1) / DT_TILE);
   if (split)
     k_direct_f64<double><<<nb, 256, 0, s>>>((const double*)px, (const double*)py, (const double*)pz, m, n, tpos, nt,
-                                            soft2, acc3);
+                                            seg_soft2, segof, seg_start, acc3);
   else
-    k_direct_f64<float><<<nb, 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, m, n, tpos, nt, soft2,
-                                           acc3);
+    k_direct_f64<float><<<nb, 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, m, n, tpos, nt,
+                                           seg_soft2, segof, seg_start, acc3);
 }

@@ -21,17 +21,20 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
   return v;
 }
 
-__global__ void k_bbox_init(uint32_t* bbox) {
-  const int i = threadIdx.x;
-  if (i < 3) bbox[i] = 0xffffffffu;
-  else if (i < 6) bbox[i] = 0u;
+__global__ void k_bbox_init(uint32_t* bbox, int S) {
+  for (int i = threadIdx.x; i < 6 * S; i += blockDim.x) bbox[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
 }
 
+// bps blocks per segment; block b of segment s strides over the segment's storage range.
 template <class PT>
 __global__ __launch_bounds__(256) void k_bbox(const PT* __restrict__ px, const PT* __restrict__ py,
-                                              const PT* __restrict__ pz, int64_t n, uint32_t* __restrict__ bbox) {
+                                              const PT* __restrict__ pz, const uint32_t* __restrict__ start, int bps,
+                                              uint32_t* __restrict__ bboxes) {
+  const int sg = blockIdx.x / bps, b = blockIdx.x % bps;
+  const int64_t i0 = start[sg], i1 = start[sg + 1];
+  uint32_t* bbox = bboxes + 6 * sg;
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+  for (int64_t i = i0 + (int64_t)b * blockDim.x + threadIdx.x; i < i1; i += (int64_t)bps * blockDim.x) {
     const uint32_t k[3] = {f2key((float)px[i]), f2key((float)py[i]), f2key((float)pz[i])};
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
@@ -63,10 +66,12 @@ __global__ __launch_bounds__(256) void k_bbox(const PT* __restrict__ px, const P
 
 template <class PT>
 __global__ __launch_bounds__(256) void k_morton(const PT* __restrict__ px, const PT* __restrict__ py,
-                                                const PT* __restrict__ pz, int64_t n, const uint32_t* __restrict__ bbox,
-                                                uint32_t* __restrict__ codes, uint32_t* __restrict__ vals) {
+                                                const PT* __restrict__ pz, int64_t n, const uint32_t* __restrict__ bboxes,
+                                                const uint32_t* __restrict__ segof, uint32_t* __restrict__ codes,
+                                                uint32_t* __restrict__ vals, uint32_t* __restrict__ codes0) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  const uint32_t* bbox = bboxes + 6 * (segof ? segof[i] : 0u);
   float lo[3], ext[3];
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
@@ -83,8 +88,27 @@ __global__ __launch_bounds__(256) void k_morton(const PT* __restrict__ px, const
     f = fminf(fmaxf(f, 0.0f), 1023.0f);
     qq[q] = (uint32_t)f;
   }
-  codes[i] = expand_bits(qq[0]) * 4u + expand_bits(qq[1]) * 2u + expand_bits(qq[2]);
+  const uint32_t code = expand_bits(qq[0]) * 4u + expand_bits(qq[1]) * 2u + expand_bits(qq[2]);
+  codes[i] = code;
   vals[i] = (uint32_t)i;
+  if (codes0) codes0[i] = code;
+}
+
+// Segment pass of a batch: keys = segment of the particle at each sorted position, payload = the permutation.
+__global__ __launch_bounds__(256) void k_seg_keys(const uint32_t* __restrict__ perm, const uint32_t* __restrict__ segof,
+                                                  int64_t n, uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = perm[i];
+  keys[i] = segof[p];
+  vals[i] = p;
+}
+
+__global__ __launch_bounds__(256) void k_gather_codes(const uint32_t* __restrict__ perm,
+                                                      const uint32_t* __restrict__ codes0, int64_t n,
+                                                      uint32_t* __restrict__ skeys) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) skeys[i] = codes0[perm[i]];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -228,11 +252,17 @@ __global__ __launch_bounds__(256) void k_leaf_codes(const uint32_t* __restrict__
   if (c >= L) return;
   ccode[c] = skeys[c * K];
   gstart[c] = IHS_NONE;
-  if (c == 0) parent[0] = IHS_NONE;  // the root (an internal node, or the single leaf when L == 1)
+  // Roots (one per segment) keep NONE; k_karras overwrites the parent of every child it creates.
+  parent[L - 1 + c] = IHS_NONE;
+  if (c < L - 1) parent[c] = IHS_NONE;
 }
 
-__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ cc, int64_t L, int64_t i, int64_t j) {
+// Karras' delta; leaves of different segments never share a node (-1), which splits the build into one tree
+// per segment (lseg = segment of each sorted particle; leaves hold K particles of one segment).
+__device__ __forceinline__ int kdelta(const uint32_t* __restrict__ cc, int64_t L, int64_t i, int64_t j,
+                                      const uint32_t* __restrict__ lseg, int K) {
   if (j < 0 || j >= L) return -1;
+  if (lseg && lseg[i * K] != lseg[j * K]) return -1;
   const uint32_t a = cc[i], b = cc[j];
   if (a == b) return 32 + __clz((uint32_t)i ^ (uint32_t)j);
   return __clz(a ^ b);
@@ -240,17 +270,21 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ cc, int64_t L
 
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cc, int64_t L, int2* __restrict__ child,
                                                 int2* __restrict__ range, uint32_t* __restrict__ parent,
-                                                uint32_t* __restrict__ ready, uint32_t* __restrict__ npend) {
+                                                uint32_t* __restrict__ ready, uint32_t* __restrict__ npend,
+                                                const uint32_t* __restrict__ lseg, int K) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *npend = 0u;
   if (i >= L - 1) return;
-  const int64_t d = kdelta(cc, L, i, i + 1) > kdelta(cc, L, i, i - 1) ? 1 : -1;
-  const int dmin = kdelta(cc, L, i, i - d);
+  // In a forest the index of each segment's last leaf (but the final one) names no node: its range would
+  // duplicate the segment's root at the segment's first leaf.
+  if (lseg && lseg[i * K] != lseg[(i + 1) * K]) return;
+  const int64_t d = kdelta(cc, L, i, i + 1, lseg, K) > kdelta(cc, L, i, i - 1, lseg, K) ? 1 : -1;
+  const int dmin = kdelta(cc, L, i, i - d, lseg, K);
   int64_t lmax = 2;
-  while (kdelta(cc, L, i, i + lmax * d) > dmin) lmax *= 2;
+  while (kdelta(cc, L, i, i + lmax * d, lseg, K) > dmin) lmax *= 2;
   int64_t l = 0;
   for (int64_t t = lmax / 2; t >= 1; t /= 2)
-    if (kdelta(cc, L, i, i + (l + t) * d) > dmin) l += t;
+    if (kdelta(cc, L, i, i + (l + t) * d, lseg, K) > dmin) l += t;
   const int64_t j = i + l * d;
   const int64_t first = i < j ? i : j, last = i < j ? j : i;
   int64_t split = first;
@@ -570,7 +604,9 @@ __global__ __launch_bounds__(GRP_BLK) void k_group_offsets(uint32_t* __restrict_
 __global__ __launch_bounds__(GRP_BLK) void k_group_emit(const uint32_t* __restrict__ gstart, int64_t L, int K, int64_t n,
                                                         const int2* __restrict__ range,
                                                         const uint32_t* __restrict__ gblk, int4* __restrict__ groups,
-                                                        const uint32_t* __restrict__ gate) {
+                                                        const uint32_t* __restrict__ gate,
+                                                        const uint32_t* __restrict__ segof,
+                                                        const uint32_t* __restrict__ root) {
   if (gate && *gate == 0u) return;
   __shared__ uint32_t s_w[GRP_BLK / 64];
   const int64_t c = (int64_t)blockIdx.x * GRP_BLK + threadIdx.x;
@@ -579,12 +615,13 @@ __global__ __launch_bounds__(GRP_BLK) void k_group_emit(const uint32_t* __restri
   if (wgt == 0) return;
   const uint32_t gs = gstart[c];
   if (gs == GROUP_SPLIT) {
-    for (uint32_t q = 0; q < wgt; ++q) groups[idx + q] = make_int4((int)(c * K + q), 1, (int)(L - 1 + c), 0);
+    const int rt = (int)root[segof ? segof[c * K] : 0u];
+    for (uint32_t q = 0; q < wgt; ++q) groups[idx + q] = make_int4((int)(c * K + q), 1, (int)(L - 1 + c), rt);
     return;
   }
   int64_t first;
   const int64_t cnt = node_count(range, L, K, n, gs, first);
-  groups[idx] = make_int4((int)(first * K), (int)cnt, (int)gs, 0);
+  groups[idx] = make_int4((int)(first * K), (int)cnt, (int)gs, (int)root[segof ? segof[first * K] : 0u]);
 }
 
 inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
@@ -592,18 +629,39 @@ inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b);
 }  // namespace
 
 void launch_bbox_morton(hipStream_t s, int split, const void* px, const void* py, const void* pz, int64_t n,
-                        TreeBufs& tb) {
-  k_bbox_init<<<1, 64, 0, s>>>(tb.bbox);
-  const unsigned nbk = (unsigned)std::min<int64_t>(1024, std::max<int64_t>(1, (n + 255) / 256));
+                        TreeBufs& tb, const SegView& sv) {
+  k_bbox_init<<<1, 256, 0, s>>>(sv.bbox, sv.S);
+  const int bps = (int)std::max<int64_t>(1, std::min<int64_t>(1024 / sv.S, (n / sv.S + 255) / 256));
+  const unsigned nbk = (unsigned)(bps * sv.S);
+  uint32_t* codes0 = sv.S > 1 ? tb.codes0 : nullptr;
   if (split) {
-    k_bbox<double><<<nbk, 256, 0, s>>>((const double*)px, (const double*)py, (const double*)pz, n, tb.bbox);
+    k_bbox<double><<<nbk, 256, 0, s>>>((const double*)px, (const double*)py, (const double*)pz, sv.start, bps, sv.bbox);
     k_morton<double><<<grid_for(n, 256), 256, 0, s>>>((const double*)px, (const double*)py, (const double*)pz, n,
-                                                      tb.bbox, tb.keys[0], tb.vals[0]);
+                                                      sv.bbox, sv.segof, tb.keys[0], tb.vals[0], codes0);
   } else {
-    k_bbox<float><<<nbk, 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, n, tb.bbox);
+    k_bbox<float><<<nbk, 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, sv.start, bps, sv.bbox);
     k_morton<float><<<grid_for(n, 256), 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, n,
-                                                     tb.bbox, tb.keys[0], tb.vals[0]);
+                                                     sv.bbox, sv.segof, tb.keys[0], tb.vals[0], codes0);
   }
+}
+
+void launch_segment_order(hipStream_t s, int64_t n, TreeBufs& tb, const SegView& sv) {
+  if (sv.S <= 1) return;
+  // Stable radix passes over the segment id with the Morton permutation as payload, in the buffers the Morton
+  // passes did not end in; then the codes follow the final permutation.
+  const int nb = radix_tiles(n);
+  int cur = tb.skeys == tb.keys[0] ? 1 : 0;
+  k_seg_keys<<<grid_for(n, 256), 256, 0, s>>>(tb.perm, sv.segof, n, tb.keys[cur], tb.vals[cur]);
+  for (int shift = 0; (1 << shift) < sv.S; shift += 8) {
+    k_radix_hist<<<nb, RADIX_THREADS, 0, s>>>(tb.keys[cur], n, shift, tb.hist, nb);
+    k_radix_scan<<<256, 256, 0, s>>>(tb.hist, nb, tb.dtot);
+    k_radix_scatter<<<nb, RADIX_THREADS, 0, s>>>(tb.keys[cur], tb.vals[cur], tb.keys[cur ^ 1], tb.vals[cur ^ 1], n,
+                                                 shift, tb.hist, tb.dtot, nb);
+    cur ^= 1;
+  }
+  tb.perm = tb.vals[cur];
+  tb.skeys = tb.keys[cur ^ 1];
+  k_gather_codes<<<grid_for(n, 256), 256, 0, s>>>(tb.perm, tb.codes0, n, tb.skeys);
 }
 
 void launch_radix_sort(hipStream_t s, int64_t n, int bits, TreeBufs& tb) {
@@ -624,14 +682,16 @@ void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb
   k_fix_ties<<<grid_for(n, 256), 256, 0, s>>>(tb.skeys, tb.perm, ids, n);
 }
 
-void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb) {
+void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv) {
   const int64_t L = (n + K - 1) / K;
   k_leaf_codes<<<grid_for(L, 256), 256, 0, s>>>(tb.skeys, L, K, tb.ccode, tb.parent, tb.gstart);
   if (L > 1)
-    k_karras<<<grid_for(L - 1, 256), 256, 0, s>>>(tb.ccode, L, tb.child, tb.range, tb.parent, tb.ready, tb.npend);
+    k_karras<<<grid_for(L - 1, 256), 256, 0, s>>>(tb.ccode, L, tb.child, tb.range, tb.parent, tb.ready, tb.npend,
+                                                  sv.segof, K);
 }
 
-void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const uint32_t* gate, float diag_hs, bool gravity) {
+void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv, const uint32_t* gate,
+                   float diag_hs, bool gravity) {
   const int64_t L = (n + K - 1) / K;
   const int nblk = (int)grid_for(L, GRP_BLK);
   const int gmax = gravity ? tb.group_max : GROUP_MAX;
@@ -641,7 +701,7 @@ void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const uint32_t
                                                  diag_hs, gmax);
   k_group_count<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.gblk, gate);
   k_group_offsets<<<1, GRP_BLK, 0, s>>>(tb.gblk, nblk, nout, tb.wq, gate);
-  k_group_emit<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.range, tb.gblk, out, gate);
+  k_group_emit<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.range, tb.gblk, out, gate, sv.segof, sv.root);
 }
 
 void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb,

@@ -70,15 +70,16 @@ __device__ __forceinline__ void features(int64_t t, const MasterBufs& D, const E
 }
 
 __global__ __launch_bounds__(256) void k_mlp_seed(int64_t n, MasterBufs D, float* __restrict__ hs,
-                                                  const EosDev* __restrict__ eos, const StepCtl* __restrict__ ctl,
-                                                  float eta, const uint16_t* __restrict__ wb, float margin,
-                                                  float hcap) {
+                                                  const EosDev* __restrict__ eos, SegView sv, float eta,
+                                                  const uint16_t* __restrict__ wb, float margin) {
   __shared__ float sw[MLP_NPARAM];
   load_weights(sw, wb);
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  const uint32_t sg = sv.of(t);
+  const float hcap = sv.hcap[sg];
   float x[MLP_IN], h;
-  features(t, D, eos, ctl->dt, eta, x, h);
+  features(t, D, eos, sv.clk[sg].dt, eta, x, h);
   const float y = forward(sw, x) + MLP_PHYS * x[0];
   float seed = h * __expf(fminf(fmaxf(y, -50.f), 50.f) * (1.0f / MLP_YSCALE));
   seed = fminf(fmaxf(seed, h * (1.0f / MLP_CLAMP)), h * MLP_CLAMP);
@@ -88,12 +89,11 @@ __global__ __launch_bounds__(256) void k_mlp_seed(int64_t n, MasterBufs D, float
 }
 
 __global__ __launch_bounds__(256) void k_mlp_features(int64_t n, MasterBufs D, const EosDev* __restrict__ eos,
-                                                      const StepCtl* __restrict__ ctl, float eta,
-                                                      float* __restrict__ xbuf) {
+                                                      SegView sv, float eta, float* __restrict__ xbuf) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   float x[MLP_IN], h;
-  features(t, D, eos, ctl->dt, eta, x, h);
+  features(t, D, eos, sv.clk[sv.of(t)].dt, eta, x, h);
 #pragma unroll
   for (int j = 0; j < MLP_IN; ++j) xbuf[t * MLP_SAMPLE + j] = x[j];
   xbuf[t * MLP_SAMPLE + 6] = h;  // the previous h, to form the target after the solve
@@ -252,14 +252,14 @@ void mlp_free(MlpDev& m) {
 }
 
 void mlp_seed_h(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, float* hs, const void* eos,
-                const StepCtl* ctl, float eta, float margin, float hcap) {
+                const SegView& sv, float eta, float margin) {
   if (!m.trained) return;
-  k_mlp_seed<<<g256(n), 256, 0, s>>>(n, D, hs, (const EosDev*)eos, ctl, eta, m.wb, margin, hcap);
+  k_mlp_seed<<<g256(n), 256, 0, s>>>(n, D, hs, (const EosDev*)eos, sv, eta, m.wb, margin);
 }
 
-void mlp_record_features(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, const void* eos, const StepCtl* ctl,
+void mlp_record_features(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, const void* eos, const SegView& sv,
                          float eta) {
-  k_mlp_features<<<g256(n), 256, 0, s>>>(n, D, (const EosDev*)eos, ctl, eta, m.xbuf);
+  k_mlp_features<<<g256(n), 256, 0, s>>>(n, D, (const EosDev*)eos, sv, eta, m.xbuf);
 }
 
 void mlp_record_targets(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D) {

@@ -47,7 +47,7 @@ __device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t la
   const uint32_t lpc = 64u / (uint32_t)K;
   uint32_t sp = 1, lhead = 0, ltail = 0, cnt = 0;
   wave_sync();
-  if (lane == 0) S.stack[0] = 0u;
+  if (lane == 0) S.stack[0] = (uint32_t)grp.w;  // the group's segment root
   wave_sync();
 
   uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, nb = 0, nchunk = 0;
@@ -313,7 +313,7 @@ __device__ __forceinline__ void hsolve_loop(HsolveState& st, const float4 me, fl
 // Wave-aggregated flags and counters. A solve stopped by a search radius below the global cap asks for the
 // wider-radius pass instead of being reported capped when `retry` is given.
 __device__ __forceinline__ void hsolve_account(bool valid, const HsolveState& st, float hsi, uint32_t* __restrict__ retry,
-                                               float hcap, uint32_t* __restrict__ flags,
+                                               float hcap /* this particle's segment cap */, uint32_t* __restrict__ flags,
                                                unsigned long long* __restrict__ counters, bool first_pass) {
   const bool capped = valid && st.state == HS_CAPPED;
   const bool again = capped && retry && hsi < hcap;
@@ -346,16 +346,17 @@ __global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, 
                                                  const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
                                                  int64_t stride, int64_t n, float eta3, float tol, int max_iter,
                                                  const uint32_t* __restrict__ gate, uint32_t* __restrict__ retry,
-                                                 float hcap, uint32_t* __restrict__ pend, float2* __restrict__ bracket,
+                                                 SegView sv, uint32_t* __restrict__ pend, float2* __restrict__ bracket,
                                                  uint32_t* __restrict__ npend, uint32_t* __restrict__ flags,
                                                  unsigned long long* __restrict__ counters) {
   if (gate && *gate == 0u) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < n;
   HsolveState st{0.f, 0.f, INFINITY, 0u, 0u, HS_DONE};
-  float hsi = 0.f;
+  float hsi = 0.f, hcap = 0.f;
   if (valid) {
     hsi = hs[t];
+    hcap = sv.hcap[sv.of(t)];
     st.h = fminf(h[t], hsi);
     hsolve_loop(st, xm[t], hsi, 1u, nbr4, stride, t, ncount[t], xm, eta3, tol, max_iter);
     h[t] = st.h;
@@ -381,7 +382,7 @@ __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, 
                                                  const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
                                                  int64_t stride, float eta3, float tol, int max_iter,
                                                  const uint32_t* __restrict__ gate, uint32_t* __restrict__ retry,
-                                                 float hcap, const uint32_t* __restrict__ pend,
+                                                 SegView sv, const uint32_t* __restrict__ pend,
                                                  const float2* __restrict__ bracket, const uint32_t* __restrict__ npend,
                                                  uint32_t* __restrict__ flags,
                                                  unsigned long long* __restrict__ counters) {
@@ -394,11 +395,12 @@ __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, 
     const uint32_t k = b + (threadIdx.x & 63u);
     const bool valid = k < np;
     HsolveState st{0.f, 0.f, INFINITY, 1u, 0u, HS_DONE};
-    float hsi = 0.f;
+    float hsi = 0.f, hcap = 0.f;
     int64_t t = 0;
     if (valid) {
       t = pend[k];
       hsi = hs[t];
+      hcap = sv.hcap[sv.of(t)];
       const float2 br = bracket[t];
       st.h = h[t];
       st.lo = br.x;
@@ -422,7 +424,7 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
                                                  const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
                                                  int64_t stride, int64_t n, float4* __restrict__ rpc,
                                                  float4* __restrict__ vh, float* __restrict__ rho_out,
-                                                 float4* __restrict__ feat, uint32_t* __restrict__ flags) {
+                                                 float4* __restrict__ feat, uint32_t* __restrict__ flags, SegView sv) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < n;
   uint32_t key = 0xffffffffu;
@@ -474,14 +476,21 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
     key = f2key(sig);
     if (!(sig == sig) || !(rho == rho)) atomicAdd(&flags[FLAG_NONFINITE], 1u);
   }
-  // One atomic per block: the Courant signal is a single word every block would otherwise contend on.
+  // The Courant signal is one word per segment that every block would otherwise contend on: one atomic per
+  // block when the block lies in one segment (always, for a single scenario), else one per lane.
   __shared__ uint32_t skey[4];
-  key = wave_min(key);
-  if ((threadIdx.x & 63) == 0) skey[threadIdx.x >> 6] = key;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    key = min(min(skey[0], skey[1]), min(skey[2], skey[3]));
-    if (key != 0xffffffffu) atomicMin(&flags[FLAG_SIGNAL_MIN], key);
+  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+  const uint32_t sg0 = sv.of(b0), sg1 = sv.of(min(b0 + (int64_t)blockDim.x, n) - 1);
+  if (sg0 == sg1) {
+    key = wave_min(key);
+    if ((threadIdx.x & 63) == 0) skey[threadIdx.x >> 6] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      key = min(min(skey[0], skey[1]), min(skey[2], skey[3]));
+      if (key != 0xffffffffu) atomicMin(&sv.clk[sg0].signal, key);
+    }
+  } else if (valid && key != 0xffffffffu) {
+    atomicMin(&sv.clk[sv.of(t)].signal, key);
   }
 }
 
@@ -563,25 +572,25 @@ void launch_nbr_build(hipStream_t s, int64_t n, int K, const float4* xm, const f
 
 void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, float* h, uint8_t* iters,
                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol, int max_iter,
-                   const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters, const uint32_t* gate,
-                   uint32_t* retry, float hcap) {
+                   const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters, const SegView& sv,
+                   const uint32_t* gate, uint32_t* retry) {
   static unsigned nblk2 = 0;
   if (!nblk2) nblk2 = persistent_blocks((const void*)k_hsolve2, 256);
   k_hsolve_begin<<<1, 64, 0, s>>>(gate, hw.npend);
   k_hsolve1<<<g256(n), 256, 0, s>>>(xm, hs, h, iters, (const uint4*)nbr, ncount, stride, n, eta3, tol, max_iter, gate,
-                                    retry, hcap, hw.pend, hw.bracket, hw.npend, flags, counters);
+                                    retry, sv, hw.pend, hw.bracket, hw.npend, flags, counters);
   k_hsolve2<<<nblk2, 256, 0, s>>>(xm, hs, h, iters, (const uint4*)nbr, ncount, stride, eta3, tol, max_iter, gate, retry,
-                                  hcap, hw.pend, hw.bracket, hw.npend, flags, counters);
+                                  sv, hw.pend, hw.bracket, hw.npend, flags, counters);
 }
 
 void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const float* vx,
                     const float* vy, const float* vz, const void* u, const uint8_t* mat, const void* eos,
                     const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float4* rpc, float4* vh, float* rho,
-                    float4* featv, uint32_t* flags) {
+                    float4* featv, uint32_t* flags, const SegView& sv) {
   const EosDev* e = (const EosDev*)eos;
 #define IHS_DENS(PT, F)                                                                                              \
   k_density<PT, F><<<g256(n), 256, 0, s>>>(xm, h, vx, vy, vz, (const PT*)u, mat, e, (const uint4*)nbr, ncount, stride, n, rpc, vh, \
-                                           rho, featv, flags)
+                                           rho, featv, flags, sv)
   if (split) {
     if (feat) IHS_DENS(double, true);
     else IHS_DENS(double, false);

@@ -13,28 +13,26 @@ namespace {
 
 inline unsigned g256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
-__global__ void k_fe_init(uint32_t* flags) {
-  if (threadIdx.x == 0) {
-    flags[FLAG_SIGNAL_MIN] = 0xffffffffu;
-    flags[FLAG_H_RETRY] = 0u;
-  }
+__global__ void k_fe_init(uint32_t* flags, SegView sv) {
+  if (threadIdx.x == 0) flags[FLAG_H_RETRY] = 0u;
+  for (int sg = threadIdx.x; sg < sv.S; sg += blockDim.x) sv.clk[sg].signal = 0xffffffffu;
 }
 
 // Wider-radius pass of a seeded evaluation (runs only when the tight pass stopped some solve at its radius).
 __global__ __launch_bounds__(256) void k_retry_hs(int64_t n, const float* __restrict__ h, float* __restrict__ hs,
-                                                  float margin, float hcap, uint32_t* __restrict__ flags) {
+                                                  float margin, SegView sv, uint32_t* __restrict__ flags) {
   if (flags[FLAG_H_RETRY] == 0u) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) atomicAdd(&flags[FLAG_RETRIES], 1u);
   if (t >= n) return;
   const float hh = h[t];
-  hs[t] = fminf(hh * margin, fmaxf(hcap, hh));
+  hs[t] = fminf(hh * margin, fmaxf(sv.hcap[sv.of(t)], hh));
 }
 
 template <class PT>
 __global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ perm, int64_t n, MasterBufs S, MasterBufs D,
                                                 float4* __restrict__ xm, float* __restrict__ hs, float margin,
-                                                float hcap) {
+                                                SegView sv) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const uint32_t s = perm[t];
@@ -57,11 +55,21 @@ __global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ per
   D.feat[t] = S.feat[s];
   D.hiter[t] = S.hiter[s];
   xm[t] = make_float4((float)x, (float)y, (float)z, m);
-  hs[t] = fminf(h * margin, fmaxf(hcap, h));
+  hs[t] = fminf(h * margin, fmaxf(sv.hcap[sv.of(t)], h));
 }
 
-__device__ __forceinline__ double step_dt(const StepCtl* ctl, const uint32_t* flags) {
-  return ctl->dt_fixed > 0.0 ? ctl->dt_fixed : ctl->cfl * (double)key2f(flags[FLAG_SIGNAL_MIN]);
+// The step of segment sg: fixed, or the Courant dt from the segment's signal of this evaluation.
+__device__ __forceinline__ double step_dt(const StepCtl* ctl, const SegView& sv, uint32_t sg) {
+  return ctl->dt_fixed > 0.0 ? ctl->dt_fixed : ctl->cfl * (double)key2f(sv.clk[sg].signal);
+}
+
+// The segment's first storage position advances its clock.
+__device__ __forceinline__ void clock_tick(const SegView& sv, uint32_t sg, int64_t t, double dt) {
+  if (t == (int64_t)sv.start[sg]) {
+    sv.clk[sg].dt = dt;
+    sv.clk[sg].t += dt;
+    sv.clk[sg].steps += 1;
+  }
 }
 
 template <class PT>
@@ -71,16 +79,13 @@ __device__ __forceinline__ PT advance(PT x, float v, double dt) {
 
 template <class PT>
 __global__ __launch_bounds__(256) void k_kick_drift(int64_t n, MasterBufs S, const float4* __restrict__ asph,
-                                                    const float4* __restrict__ accg, StepCtl* ctl,
-                                                    const uint32_t* __restrict__ flags) {
+                                                    const float4* __restrict__ accg, const StepCtl* __restrict__ ctl,
+                                                    SegView sv) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const double dt = step_dt(ctl, flags);
-  if (t == 0) {
-    ctl->dt = dt;
-    ctl->t += dt;
-    ctl->steps += 1;
-  }
   if (t >= n) return;
+  const uint32_t sg = sv.of(t);
+  const double dt = step_dt(ctl, sv, sg);
+  clock_tick(sv, sg, t, dt);
   const float4 a = asph[t], g = accg[t];
   const float hdt = (float)(0.5 * dt);
   const float vx = S.vx[t] + (a.x + g.x) * hdt;
@@ -99,10 +104,10 @@ __global__ __launch_bounds__(256) void k_kick_drift(int64_t n, MasterBufs S, con
 
 template <class PT>
 __global__ __launch_bounds__(256) void k_kick(int64_t n, MasterBufs S, const float4* __restrict__ asph,
-                                              const float4* __restrict__ accg, const StepCtl* __restrict__ ctl) {
+                                              const float4* __restrict__ accg, SegView sv) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const double dt = ctl->dt;
+  const double dt = sv.clk[sv.of(t)].dt;
   const float4 a = asph[t], g = accg[t];
   const float hdt = (float)(0.5 * dt);
   S.vx[t] += (a.x + g.x) * hdt;
@@ -116,15 +121,12 @@ __global__ __launch_bounds__(256) void k_kick(int64_t n, MasterBufs S, const flo
 // ie/shaders/sph_step.wgsl:244-257 (damped relaxation in a frame rotating at omega about +z).
 template <class PT>
 __global__ __launch_bounds__(256) void k_relax(int64_t n, MasterBufs S, const float4* __restrict__ asph,
-                                               const float4* __restrict__ accg, StepCtl* ctl) {
+                                               const float4* __restrict__ accg, const StepCtl* __restrict__ ctl,
+                                               SegView sv) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const double dt = ctl->dt_fixed;
-  if (t == 0) {
-    ctl->dt = dt;
-    ctl->t += dt;
-    ctl->steps += 1;
-  }
   if (t >= n) return;
+  const double dt = ctl->dt_fixed;
+  clock_tick(sv, sv.of(t), t, dt);
   const float4 a = asph[t], g = accg[t];
   const float w2 = ctl->omega * ctl->omega;
   const float fdt = (float)dt, damp = ctl->damp;
@@ -253,36 +255,37 @@ __global__ __launch_bounds__(256) void k_pos_f64(int64_t n, MasterBufs S, double
 
 }  // namespace
 
-void launch_fe_init(hipStream_t s, uint32_t* flags) { k_fe_init<<<1, 64, 0, s>>>(flags); }
+void launch_fe_init(hipStream_t s, uint32_t* flags, const SegView& sv) { k_fe_init<<<1, 256, 0, s>>>(flags, sv); }
 
-void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float margin, float hcap, uint32_t* flags) {
-  k_retry_hs<<<std::max(1u, g256(n)), 256, 0, s>>>(n, h, hs, margin, hcap, flags);
+void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float margin, const SegView& sv,
+                     uint32_t* flags) {
+  k_retry_hs<<<std::max(1u, g256(n)), 256, 0, s>>>(n, h, hs, margin, sv, flags);
 }
 
 void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,
-                   const MasterBufs& dst, float4* xm, float* hs, float h_margin, float h_cap) {
-  if (split) k_gather<double><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, hs, h_margin, h_cap);
-  else k_gather<float><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, hs, h_margin, h_cap);
+                   const MasterBufs& dst, float4* xm, float* hs, float h_margin, const SegView& sv) {
+  if (split) k_gather<double><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, hs, h_margin, sv);
+  else k_gather<float><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, hs, h_margin, sv);
 }
 
 void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph,
-                       const float4* accg, StepCtl* ctl, const uint32_t* flags) {
+                       const float4* accg, const StepCtl* ctl, const SegView& sv) {
   const unsigned g = std::max(1u, g256(n));
-  if (split) k_kick_drift<double><<<g, 256, 0, s>>>(n, st, asph, accg, ctl, flags);
-  else k_kick_drift<float><<<g, 256, 0, s>>>(n, st, asph, accg, ctl, flags);
+  if (split) k_kick_drift<double><<<g, 256, 0, s>>>(n, st, asph, accg, ctl, sv);
+  else k_kick_drift<float><<<g, 256, 0, s>>>(n, st, asph, accg, ctl, sv);
 }
 
 void launch_kick(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
-                 const StepCtl* ctl) {
-  if (split) k_kick<double><<<g256(n), 256, 0, s>>>(n, st, asph, accg, ctl);
-  else k_kick<float><<<g256(n), 256, 0, s>>>(n, st, asph, accg, ctl);
+                 const SegView& sv) {
+  if (split) k_kick<double><<<g256(n), 256, 0, s>>>(n, st, asph, accg, sv);
+  else k_kick<float><<<g256(n), 256, 0, s>>>(n, st, asph, accg, sv);
 }
 
 void launch_relax(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
-                  StepCtl* ctl) {
+                  const StepCtl* ctl, const SegView& sv) {
   const unsigned g = std::max(1u, g256(n));
-  if (split) k_relax<double><<<g, 256, 0, s>>>(n, st, asph, accg, ctl);
-  else k_relax<float><<<g, 256, 0, s>>>(n, st, asph, accg, ctl);
+  if (split) k_relax<double><<<g, 256, 0, s>>>(n, st, asph, accg, ctl, sv);
+  else k_relax<float><<<g, 256, 0, s>>>(n, st, asph, accg, ctl, sv);
 }
 
 void launch_energy(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* accg, double* part,

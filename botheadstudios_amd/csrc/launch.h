@@ -10,7 +10,7 @@ enum FlagIndex : int {
   FLAG_WALK_OVERFLOW = 1,  // tree-walk stack overflows (interactions dropped)
   FLAG_H_CAPPED = 2,       // h-solves stopped at the searched radius (isolated particles)
   FLAG_H_UNCONVERGED = 3,  // h-solves that hit the iteration limit
-  FLAG_SIGNAL_MIN = 4,     // order-preserving key of min_i h_i/max(c_i+|v_i|,1)
+  FLAG_RESERVED4 = 4,      // (the Courant signal lives in the segment clocks)
   FLAG_NONFINITE = 5,      // particles with non-finite results
   FLAG_H_RETRY = 6,        // this evaluation: seeded solves stopped by the tight search radius (gate of the
                            // wider-radius pass)
@@ -39,6 +39,29 @@ __device__ __forceinline__ void cnt_add(unsigned long long* counters, int id, un
 }
 
 // Buffers of one Morton/sort/Karras/upsweep pipeline (sized for a capacity).
+// Per-scenario clock: a batch advances several independent impact scenarios ("segments"), each with its own
+// Courant dt. Segment s occupies storage positions [start[s], start[s+1]) in every buffer at all times (the sort
+// orders by (segment, Morton code)), so the segment of a storage position never changes.
+struct SegClock {
+  double dt;                 // dt of the segment's last step
+  double t;                  // simulated time
+  unsigned long long steps;  // steps taken
+  uint32_t signal;           // order-preserving key of min h / max(c + |v|, 1) of the current evaluation
+  uint32_t pad;
+};
+
+struct SegView {
+  int S = 1;
+  const uint32_t* segof = nullptr;  // n: segment of storage position t (null when S == 1)
+  const uint32_t* start = nullptr;  // S + 1 storage offsets
+  const float* soft2 = nullptr;     // S: softening^2
+  const float* hcap = nullptr;      // S: search-radius cap
+  SegClock* clk = nullptr;          // S
+  uint32_t* bbox = nullptr;         // 6 S order-preserving bbox keys
+  const uint32_t* root = nullptr;   // S: the segment tree's root node (walk start)
+  __device__ __forceinline__ uint32_t of(int64_t t) const { return segof ? segof[t] : 0u; }
+};
+
 struct TreeBufs {
   int64_t cap = 0;   // particle capacity
   int64_t lcap = 0;  // leaf capacity
@@ -75,6 +98,7 @@ struct TreeBufs {
   // results of the last sort: sorted keys and the permutation (pointers into keys/vals)
   uint32_t* skeys = nullptr;
   uint32_t* perm = nullptr;
+  uint32_t* codes0 = nullptr;  // n: Morton codes by storage position (the segment pass re-gathers them)
 };
 
 constexpr int GROUP_MAX = 64;            // one wavefront: the largest walk group
@@ -89,20 +113,25 @@ inline int radix_tiles(int64_t n) { return (int)((n + RADIX_TILE - 1) / RADIX_TI
 
 // --- k_lbvh.hip ---
 // bbox + Morton codes from positions (PT = float or double, SoA), vals = identity.
+// Per-segment bbox (sv.bbox) and codes normalised in each segment's box.
 void launch_bbox_morton(hipStream_t s, int split, const void* px, const void* py, const void* pz, int64_t n,
-                        TreeBufs& tb);
+                        TreeBufs& tb, const SegView& sv);
+// S > 1: stable pass by segment after the Morton passes, so the order is (segment, code); re-gathers skeys.
+void launch_segment_order(hipStream_t s, int64_t n, TreeBufs& tb, const SegView& sv);
 // Stable LSD radix sort of (keys[0], vals[0]) over `bits` low bits; sets tb.skeys / tb.perm.
 void launch_radix_sort(hipStream_t s, int64_t n, int bits, TreeBufs& tb);
 // Reorders runs of equal codes in tb.perm by ids[perm] (so the order is (code, id)).
 void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb);
 // Leaf cluster codes + Karras topology + counters reset.
-void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb);
+// With segments (sv.segof) the tree is a forest: one Karras tree per segment, rooted at sv.root[s].
+void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv);
 // Walk groups in leaf order -> tb.groups / tb.ngroups, and resets the walk work queues. A group is the
 // topmost subtree on a leaf's root path holding <= GROUP_MAX (gravity: tb.group_max) particles whose box
 // diagonal is at most
 // diag_hs times its largest search radius (no radius: no box limit); leaves without one are split into
 // single-particle groups. Needs the upsweep's boxes.
-void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const uint32_t* gate = nullptr,
+// Groups record their segment's root (int4.w), where their walks start.
+void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv, const uint32_t* gate = nullptr,
                    float diag_hs = GROUP_DIAG_HS, bool gravity = false);
 // Leaf moments and ordered upsweep; xm = sorted (x,y,z,m), hs = sorted search radius (or null). With a gate
 // the arrival counters are re-armed first (a second upsweep over the same topology).
@@ -113,14 +142,17 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
 // Wave-coherent Barnes-Hut group walk over sorted particles; accg = (a/G xyz... see kernel), sorted order.
 // Persistent: waves pull groups from per-XCD queues (tb.wq[0..8)).
 // Groups come from tb.ggroups; pot = false skips the potential (accg.w = 0).
-void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, float soft2, const float4* xm,
-                         const TreeBufs& tb, float4* accg, uint32_t* flags, unsigned long long* counters,
-                         bool pot = true);
+// self: some segment has zero softening (the walk then skips self pairs by index).
+void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
+                         const SegView& sv, float4* accg, uint32_t* flags, unsigned long long* counters,
+                         bool pot = true, bool self = false);
 // FP64 direct-sum accelerations (G included) at the particles with the given ids, from the state's positions
-// in storage order (accuracy probe of the walk). inv needs n entries, tpos nt.
+// in storage order (accuracy probe of the walk), each over its own scenario with its softening. inv needs n
+// entries, tpos nt.
 void launch_direct_f64_targets(hipStream_t s, int split, const void* px, const void* py, const void* pz, const float* m,
-                               const uint32_t* id, int64_t n, const uint32_t* ids, int64_t nt, double soft2,
-                               uint32_t* inv, uint32_t* tpos, double* acc3);
+                               const uint32_t* id, int64_t n, const uint32_t* ids, int64_t nt, const double* seg_soft2,
+                               const uint32_t* segof, const uint32_t* seg_start, uint32_t* inv, uint32_t* tpos,
+                               double* acc3);
 // Tiled direct O(N^2) sum (FP32), sorted or unsorted, acc4 = (acc xyz, potential).
 void launch_gravity_direct(hipStream_t s, int64_t n, float soft2, const float4* xm, float4* acc4);
 
@@ -138,14 +170,16 @@ struct HsolveScratch {
 // retry != null: solves stopped by a search radius below hcap increment *retry instead of FLAG_H_CAPPED.
 void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, float* h, uint8_t* iters,
                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol, int max_iter,
-                   const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters,
-                   const uint32_t* gate = nullptr, uint32_t* retry = nullptr, float hcap = 0.f);
+                   const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters, const SegView& sv,
+                   const uint32_t* gate = nullptr, uint32_t* retry = nullptr);
 // Wider-radius pass set-up (gated): hs = min(h * margin, max(hcap, h)); counts the pass in FLAG_RETRIES.
-void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float margin, float hcap, uint32_t* flags);
+void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float margin, const SegView& sv,
+                     uint32_t* flags);
+// Also reduces each segment's Courant signal into sv.clk[s].signal.
 void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const float* vx,
                     const float* vy, const float* vz, const void* u, const uint8_t* mat, const void* eos,
                     const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float4* rpc, float4* vh, float* rho,
-                    float4* featv, uint32_t* flags);
+                    float4* featv, uint32_t* flags, const SegView& sv);
 void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const float4* vh, const float4* rpc,
                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, const float* av, float4* asph);
 
@@ -172,14 +206,11 @@ struct MasterBufs {
 
 // Device-resident step control (kernels read it, so a captured step needs no host values).
 struct StepCtl {
-  double dt_fixed;  // > 0: fixed dt; otherwise cfl * min signal
+  double dt_fixed;  // > 0: fixed dt; otherwise cfl * the segment's min signal
   double cfl;
-  double dt;        // dt of the last step
-  double t;         // simulated time
   float av[2];      // artificial viscosity alpha, beta
   float damp;       // relaxation damping
   float omega;      // relaxation frame rotation rate about +z
-  uint64_t steps;
 };
 
 // 48-byte interchange record (gpu_sph.rs:25-37).
@@ -194,15 +225,15 @@ struct SphParticle48 {
   uint32_t prov;
 };
 
-void launch_fe_init(hipStream_t s, uint32_t* flags);
+void launch_fe_init(hipStream_t s, uint32_t* flags, const SegView& sv);
 void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,
-                   const MasterBufs& dst, float4* xm, float* hs, float h_margin, float h_cap);
+                   const MasterBufs& dst, float4* xm, float* hs, float h_margin, const SegView& sv);
 void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph,
-                       const float4* accg, StepCtl* ctl, const uint32_t* flags);
+                       const float4* accg, const StepCtl* ctl, const SegView& sv);
 void launch_kick(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
-                 const StepCtl* ctl);
+                 const SegView& sv);
 void launch_relax(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
-                  StepCtl* ctl);
+                  const StepCtl* ctl, const SegView& sv);
 // FP64 KE, IE, PE, p, L (9 values) into out; part needs 9 * 1024 doubles.
 void launch_energy(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* accg, double* part,
                    double* out);

@@ -49,12 +49,12 @@ struct MlpDev {
 
 int mlp_alloc(MlpDev& m, int64_t cap);
 void mlp_free(MlpDev& m);
-// Writes the Newton initial guess into D.h and the searched radius hs = min(seed * margin, max(hcap, seed)).
-// ctl gives the drift dt.
+// Writes the Newton initial guess into D.h and the searched radius hs = min(seed * margin, max(hcap, seed)),
+// with each particle's segment drift dt and cap.
 void mlp_seed_h(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, float* hs, const void* eos,
-                const StepCtl* ctl, float eta, float margin, float hcap);
+                const SegView& sv, float eta, float margin);
 // Recording: features before the solve, then the converged target after it.
-void mlp_record_features(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, const void* eos, const StepCtl* ctl,
+void mlp_record_features(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, const void* eos, const SegView& sv,
                          float eta);
 void mlp_record_targets(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D);
 // Gradient of the mean squared error over n samples into m.grad[0..NPARAM), mean loss in grad[NPARAM].

@@ -102,6 +102,24 @@ class GpuSph:
         self._check(self._L.ihs_upload(self._h, p.ctypes.data, p.shape[0], e.ctypes.data, e.shape[0],
                                        float(softening)))
 
+    def upload_batch(self, scenarios, eos, softenings):
+        """Several independent scenarios in one state (list of 48-byte particle arrays, one softening each): every
+        pass of a step covers all of them in one launch, each with its own tree and Courant dt."""
+        parts = [np.ascontiguousarray(p_, dtype=SPH_PARTICLE_DTYPE) for p_ in scenarios]
+        allp = np.ascontiguousarray(np.concatenate(parts))
+        counts = np.array([len(p_) for p_ in parts], np.int64)
+        soft = np.ascontiguousarray(softenings, np.float32)
+        assert soft.shape == counts.shape
+        e = eos_records(eos)
+        self._check(self._L.ihs_upload_batch(self._h, allp.ctypes.data, counts.ctypes.data, len(counts),
+                                             e.ctypes.data, e.shape[0], soft.ctypes.data))
+        self._counts = counts
+
+    def segment_clock(self, seg: int) -> dict:
+        t, dt, steps = C.c_double(), C.c_double(), C.c_int64()
+        self._check(self._L.ihs_segment_clock(self._h, int(seg), C.byref(t), C.byref(dt), C.byref(steps)))
+        return {"t": t.value, "dt": dt.value, "steps": steps.value}
+
     def set_dt(self, dt: float, damp: float = 1.0):
         self._check(self._L.ihs_set_dt(self._h, float(dt), float(damp)))
 

@@ -80,9 +80,9 @@ typedef struct {
   double ke, ie, pe;      /* FP64: kinetic, internal, gravitational (1/2 sum m phi, tree potential) */
   double p[3];            /* FP64 momentum */
   double L[3];            /* FP64 angular momentum about the origin */
-  double t;               /* simulated time */
-  double dt;              /* last step's dt */
-  int64_t steps;          /* integration steps taken */
+  double t;               /* simulated time (scenario 0 of a batch) */
+  double dt;              /* last step's dt (scenario 0 of a batch) */
+  int64_t steps;          /* integration steps taken (scenario 0 of a batch) */
   int64_t force_evals;    /* force evaluations since the last counter reset */
   double mean_h_iters;    /* Newton updates per particle per force evaluation */
   double mean_h_evals;    /* density evaluations per particle per force evaluation */
@@ -130,6 +130,16 @@ int ihs_destroy(ihs_handle* h);
 const char* ihs_last_error(const ihs_handle* h);
 /* GpuSph::upload (gpu_sph.rs:839-850): n beyond capacity is an error (the reference clamps silently). */
 int ihs_upload(ihs_handle* h, const ihs_particle* p, int64_t n, const ihs_eos* eos, int32_t n_mat, float softening);
+/* Batched scenarios (impact-parameter sweeps): nseg independent scenarios concatenated in p, counts[s] particles
+ * each, with their own softening. Every pass of a step covers all of them in one launch: one tree per scenario,
+ * walks confined to it, a Courant dt per scenario (ihs_segment_clock). With nseg > 1 each count must be a multiple
+ * of the leaf size. ihs_upload is the nseg = 1 case. No counterpart in the reference, which runs scenarios one at
+ * a time (impact-run:645-671). */
+#define IHS_MAX_SEGMENTS 1024
+int ihs_upload_batch(ihs_handle* h, const ihs_particle* p, const int64_t* counts, int32_t nseg, const ihs_eos* eos,
+                     int32_t n_mat, const float* softening);
+/* Simulated time, last dt and step count of scenario seg. */
+int ihs_segment_clock(ihs_handle* h, int32_t seg, double* t, double* dt, int64_t* steps);
 /* GpuSph::set_dt (gpu_sph.rs:853-857). dt <= 0 selects the adaptive Courant dt (impact-run:403-410). */
 int ihs_set_dt(ihs_handle* h, double dt, float damp);
 /* GpuSph::set_av (gpu_sph.rs:864-868). */

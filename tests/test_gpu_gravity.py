@@ -101,7 +101,7 @@ def test_direct_f64_probe_matches_oracle(precision):
         got = sph.direct_f64(ids)
         err = sph.gravity_error(512)
     m32 = ic.m.astype(np.float32).astype(np.float64)  # masses cross the boundary as f32 (gpu_sph.rs:25-37)
-    ref = oracle.direct_f64(pos, m32, np.float32(ic.softening), targets=ids.astype(np.int64))
+    ref = oracle.direct_f64(pos, m32, float(np.float32(ic.softening)), targets=ids.astype(np.int64))
     assert np.max(np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)) < 1e-12
     assert err["targets"] == 512 and err["rms"] < 2e-3, err
 
