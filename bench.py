@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=65536)
     ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--no-sweep", action="store_true", help="skip the batched 64-scenario sweep line")
+    ap.add_argument("--sweep-steps", type=int, default=10)
     return ap.parse_args()
 
 
@@ -195,6 +197,14 @@ def main():
     }
     if "plain_fp32" in variants and "learned_split" in variants:
         line["learned_vs_plain"] = variants["learned_split"]["value"] / variants["plain_fp32"]["value"]
+    if not args.no_sweep:
+        # Batched impact-parameter sweep (64 scenarios, N log-uniform in [4096, 131072]) in one launch per pass,
+        # against the same scenarios one at a time. Reported beside the headline, not as it.
+        from botheadstudios_amd import sweep as sweep_mod
+        try:
+            line["sweep"] = sweep_mod.measure(64, args.sweep_steps, 2, log=log)
+        except Exception as e:  # the headline line must still print
+            line["sweep"] = {"error": repr(e)}
     line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args)
     print(json.dumps(line), flush=True)
 
