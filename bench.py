@@ -45,8 +45,8 @@ def parse():
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--variants", default="plain,learned")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=65536)
-    ap.add_argument("--cpu-steps", type=int, default=12)
+    ap.add_argument("--cpu-n", type=int, default=16384)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched 64-scenario sweep line")
     ap.add_argument("--sweep-steps", type=int, default=10)
     return ap.parse_args()
@@ -123,14 +123,15 @@ def roofline(res, n, args):
 
 def cpu_baseline(args):
     """The oracle's FP64 restatement of HydroBody::step (hydrostatic.rs:345-359: grid neighbours, octree
-    Barnes-Hut theta 0.5 with leaf bucket 8, fixed h = 2 (m/rho0)^(1/3), Courant dt), 1 thread, on the same
-    impact geometry at a bounded particle count."""
+    Barnes-Hut theta 0.5 with leaf bucket 8, Courant dt) with the same Newton h-solve (48 neighbours, 1e-4) before
+    every density pass, 1 thread, on the same impact geometry at a bounded particle count."""
     from botheadstudios_amd import ics
     from oracle import oracle
 
     ic = ics.config_ic(4, n=args.cpu_n)
-    o = oracle.Hydro(ic.pos, ic.vel, ic.u, ic.h * (2.0 / ics.ETA_48), ic.m, ic.mat.astype(np.int32), ic.eos,
-                     ic.softening, theta=0.5, gravity_mode=0, cs_fd=0, brute=False, leaf_cap=8, nthreads=1)
+    o = oracle.Hydro(ic.pos, ic.vel, ic.u, ic.h, ic.m, ic.mat.astype(np.int32), ic.eos,
+                     ic.softening, theta=0.5, gravity_mode=0, cs_fd=0, brute=False, leaf_cap=8, nthreads=1,
+                     h_solve=True, h_eta=ics.ETA_48, h_tol=1e-4)
     t0 = time.perf_counter()
     o.kdk(args.cpu_steps, dt=0.0, cfl=0.1)
     sec = time.perf_counter() - t0
@@ -141,7 +142,7 @@ def cpu_baseline(args):
     except Exception:
         pass
     return dict(value=args.cpu_n * args.cpu_steps / sec, unit="particle-steps/s", cores=1, kind="port",
-                sample=f"oracle FP64 HydroBody KDK step (fixed h, octree leaf 8, theta 0.5) on the same impact "
+                sample=f"oracle FP64 HydroBody KDK step with the Newton h-solve (octree leaf 8, theta 0.5) on the same impact "
                        f"geometry at N={args.cpu_n}, {args.cpu_steps} steps, {sec:.1f} s, 1 thread "
                        f"({model or platform.processor()}, nproc={os.cpu_count()})")
 

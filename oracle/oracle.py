@@ -62,6 +62,10 @@ class OrState(C.Structure):
         ("brute", C.c_int32),
         ("leaf_cap", C.c_int32),
         ("nthreads", C.c_int32),
+        ("h_solve", C.c_int32),
+        ("h_pad", C.c_int32),
+        ("h_eta", C.c_double),
+        ("h_tol", C.c_double),
     ]
 
 
@@ -259,7 +263,8 @@ class Hydro:
     """FP64 SPH state for the oracle (hydrostatic.rs HydroBody / sph-verify CpuState)."""
 
     def __init__(self, xyz, vel, u, h, m, mat, eos_table, softening, av_alpha=1.0, av_beta=2.0, theta=0.5,
-                 gravity_mode=1, cs_fd=1, brute=False, leaf_cap=1, nthreads=0):
+                 gravity_mode=1, cs_fd=1, brute=False, leaf_cap=1, nthreads=0, h_solve=False, h_eta=2.2545,
+                 h_tol=1e-4):
         self.xyz = _f64(xyz).copy()
         self.vel = _f64(vel).copy()
         self.u = _f64(u).copy()
@@ -278,6 +283,7 @@ class Hydro:
         s.av_alpha, s.av_beta, s.theta = float(av_alpha), float(av_beta), float(theta)
         s.gravity_mode, s.cs_fd, s.brute = int(gravity_mode), int(cs_fd), int(bool(brute))
         s.leaf_cap, s.nthreads = int(leaf_cap), int(nthreads)
+        s.h_solve, s.h_eta, s.h_tol = int(bool(h_solve)), float(h_eta), float(h_tol)
 
     def density(self):
         rho = np.zeros(self.st.n)

@@ -704,7 +704,15 @@ double or_kdk(or_state* s, double dt_fixed, double cfl, int32_t steps) {
   const int64_t n = s->n;
   std::vector<double> rho(n), acc(3 * n), dudt(n);
   double t = 0.0;
+  // Optional Newton h-solve before each density pass; its candidate radius is 1.5x the largest current h.
+  auto solve_h = [&]() {
+    if (!s->h_solve) return;
+    double hmax = 0.0;
+    for (int64_t i = 0; i < n; ++i) hmax = std::max(hmax, s->h[i]);
+    or_hsolve(s->xyz, s->m, n, s->h_eta, s->h_tol, 40, 1.5 * hmax, s->h, nullptr, nullptr, s->nthreads);
+  };
   for (int32_t st = 0; st < steps; ++st) {
+    solve_h();
     or_density(s, rho.data());
     const double dt = dt_fixed > 0 ? dt_fixed : or_courant_dt(s, rho.data(), cfl);
     or_forces(s, rho.data(), acc.data(), dudt.data());
@@ -713,6 +721,7 @@ double or_kdk(or_state* s, double dt_fixed, double cfl, int32_t steps) {
       s->u[i] = std::max(s->u[i] + dudt[i] * (0.5 * dt), 0.0);
       for (int q = 0; q < 3; ++q) s->xyz[3 * i + q] += s->vel[3 * i + q] * dt;
     }
+    solve_h();
     or_density(s, rho.data());
     or_forces(s, rho.data(), acc.data(), dudt.data());
     for (int64_t i = 0; i < n; ++i) {

@@ -101,6 +101,11 @@ typedef struct {
   int32_t brute;    /* 1 = O(N^2) pair sweep (sph-verify cpu_forces), 0 = neighbour grid */
   int32_t leaf_cap; /* octree leaf bucket (1 = reference) */
   int32_t nthreads;
+  int32_t h_solve;  /* 1: Newton-solve h (or_hsolve) before every density pass of or_kdk (new; the reference keeps h
+                       fixed) */
+  int32_t h_pad;
+  double h_eta;     /* eta of the h-solve (48 neighbours: 2.2545) */
+  double h_tol;     /* its |f|/rho tolerance */
 } or_state;
 
 void or_density(const or_state* s, double* rho);
