@@ -1,0 +1,78 @@
+"""Energy and angular-momentum drift of a long impact run (FP64 reductions on the device).
+
+The reference reports total energy conserved to 0.3-0.5% over an impact run (ie/JOURNAL.md:3257, via
+impact-run:452-472) and checks angular momentum with a sorted-Kahan sum (impact-run:195-205,528-536). Here E = KE + IE +
+PE is reduced on the device in FP64 after the last force evaluation of each block of steps: PE is 1/2 sum m phi of
+the tree walk. L and p are taken about the origin of the centre-of-mass frame.
+
+    python -m botheadstudios_amd.conservation --cfg 3 --steps 1000
+"""
+from __future__ import annotations
+
+import json
+import time
+
+import numpy as np
+
+
+def run(ic, steps: int, every: int, precision: str, h_mode: str = "newton", params=None, weights=None) -> dict:
+    from .gpu_sph import GpuSph
+
+    params = params or {"leaf": 16, "group": 64, "theta": 0.5, "quadrupole": True}
+    with GpuSph(ic.n, leaf_size=params["leaf"], theta=params["theta"], group_max=params["group"],
+                quadrupole=params["quadrupole"], precision=precision, h_mode=h_mode) as sph:
+        if weights is not None and h_mode == "learned":
+            sph.load_mlp(weights)
+        sph.upload(ic.particles48(), ic.eos, ic.softening)
+        sph.set_dt(0.0)
+        sph.encode_kdk(1)
+        s0 = sph.stats(reset=True)
+        rows = [(s0["t"], s0["etot"], s0["ke"], s0["ie"], s0["pe"], *s0["L"], *s0["p"])]
+        t0 = time.perf_counter()
+        done = 0
+        while done < steps:
+            k = min(every, steps - done)
+            sph.encode_kdk(k)
+            done += k
+            st = sph.stats()
+            rows.append((st["t"], st["etot"], st["ke"], st["ie"], st["pe"], *st["L"], *st["p"]))
+            if st["nonfinite"]:
+                raise RuntimeError(f"non-finite state after {done} steps")
+        sec = time.perf_counter() - t0
+    a = np.array(rows)
+    e0 = a[0, 1]
+    L0 = np.linalg.norm(a[0, 5:8])
+    scale_p = np.sqrt(2.0 * np.sum(ic.m) * abs(a[0, 2]))  # |p| scale: total mass x rms speed
+    return {
+        "precision": precision, "h_mode": h_mode, "steps": steps, "t_end": float(a[-1, 0]), "seconds": sec,
+        "energy_drift_max": float(np.max(np.abs(a[:, 1] - e0)) / abs(e0)),
+        "energy_drift_end": float((a[-1, 1] - e0) / abs(e0)),
+        "ke_to_ie": float((a[-1, 3] - a[0, 3]) / abs(e0)),
+        "angmom_drift_max": float(np.max(np.linalg.norm(a[:, 5:8] - a[0, 5:8], axis=1)) / L0),
+        "momentum_max": float(np.max(np.linalg.norm(a[:, 8:11], axis=1)) / scale_p),
+    }
+
+
+def main(argv=None):
+    import argparse
+
+    from . import autotune, ics
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cfg", type=int, default=3)
+    ap.add_argument("--n", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--every", type=int, default=50)
+    a = ap.parse_args(argv)
+    ic = ics.config_ic(a.cfg, n=a.n)
+    params = autotune.tuned_params(ic.n)
+    out = {"n": ic.n, "cfg": a.cfg, "walk": params, "runs": []}
+    for prec in ("fp32", "split"):
+        r = run(ic, a.steps, a.every, prec, params=params)
+        print(json.dumps(r), flush=True)
+        out["runs"].append(r)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
