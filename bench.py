@@ -43,7 +43,9 @@ def parse():
     ap.add_argument("--group", type=int, default=None, help="default: the autotuned value for this N")
     ap.add_argument("--quadrupole", type=int, default=None, help="default: the autotuned value for this N")
     ap.add_argument("--profile-steps", type=int, default=3)
-    ap.add_argument("--variants", default="plain,learned")
+    ap.add_argument("--variants", default="plain,learned,reuse",
+                    help="plain: Newton FP32; learned: learned seed + split precision; reuse: both again with the "
+                         "first evaluation of each step reusing the previous one's position-only work")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=16384)
     ap.add_argument("--cpu-steps", type=int, default=6)
@@ -56,11 +58,11 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_variant(ic, precision, h_mode, args, weights=None):
+def run_variant(ic, precision, h_mode, args, weights=None, reuse=False):
     from botheadstudios_amd.gpu_sph import GpuSph
 
     sph = GpuSph(ic.n, leaf_size=args.leaf, precision=precision, h_mode=h_mode, theta=args.theta,
-                 group_max=args.group, quadrupole=args.quadrupole)
+                 group_max=args.group, quadrupole=args.quadrupole, reuse_positions=reuse)
     try:
         if weights is not None:
             sph.load_mlp(weights)
@@ -174,6 +176,14 @@ def main():
         log("learned: MLP-seeded Newton h, split precision ...")
         variants["learned_split"] = run_variant(ic, "split", "learned", args, weights=weights)
         log(f"  {variants['learned_split']['ms_per_step']:.3f} ms/step")
+    if "reuse" in wanted:
+        log("plain, reusing position-only work ...")
+        variants["plain_fp32_reuse"] = run_variant(ic, "fp32", "newton", args, reuse=True)
+        log(f"  {variants['plain_fp32_reuse']['ms_per_step']:.3f} ms/step")
+        if weights is not None:
+            log("learned + split, reusing position-only work ...")
+            variants["learned_split_reuse"] = run_variant(ic, "split", "learned", args, weights=weights, reuse=True)
+            log(f"  {variants['learned_split_reuse']['ms_per_step']:.3f} ms/step")
     head = "learned_split" if "learned_split" in variants else "plain_fp32"
     res = variants[head]
     line = {
@@ -198,6 +208,9 @@ def main():
     }
     if "plain_fp32" in variants and "learned_split" in variants:
         line["learned_vs_plain"] = variants["learned_split"]["value"] / variants["plain_fp32"]["value"]
+    if "plain_fp32_reuse" in variants and "learned_split_reuse" in variants:
+        line["learned_vs_plain_reuse"] = (variants["learned_split_reuse"]["value"] /
+                                          variants["plain_fp32_reuse"]["value"])
     if not args.no_sweep:
         # Batched impact-parameter sweep (64 scenarios, N log-uniform in [4096, 131072]) in one launch per pass,
         # against the same scenarios one at a time. Reported beside the headline, not as it.

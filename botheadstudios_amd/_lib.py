@@ -16,7 +16,8 @@ class IhsConfig(C.Structure):
         ("theta", C.c_float), ("cfl", C.c_float), ("n_ngb", C.c_float), ("h_tol", C.c_float),
         ("h_max_iter", C.c_int32), ("h_margin", C.c_float), ("h_cap_factor", C.c_float),
         ("nbr_capacity", C.c_int32), ("use_graph", C.c_int32), ("h_margin_seeded", C.c_float),
-        ("group_max", C.c_int32), ("quadrupole", C.c_int32), ("reserved", C.c_int32 * 4),
+        ("group_max", C.c_int32), ("quadrupole", C.c_int32), ("reuse_positions", C.c_int32),
+        ("reserved", C.c_int32 * 3),
     ]
 
 

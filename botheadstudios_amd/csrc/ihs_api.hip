@@ -83,10 +83,14 @@ struct ihs_handle {
   // The last force evaluation solved h at the current positions (nothing has drifted since), so the next
   // evaluation's search radius only needs room for round-off.
   bool h_settled = false;
+  // The last full force evaluation saw the current positions (no drift since): its tree, neighbour lists,
+  // density and gravity still hold.
+  bool pos_fresh = false;
   HsolveScratch hw;
   // One captured KDK step per starting parity of the ping-pong state buffers.
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
+  int graph_next[2] = {0, 1};  // the buffer a captured step leaves the state in
   // stage profiling
   bool prof_on = false;
   std::vector<hipEvent_t> prof_ev;
@@ -353,8 +357,9 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
     }
     h->h_settled = true;
   }
-  // The network's inputs come from the evaluation before a drift.
-  const int feat = (h->cfg.h_mode == IHS_H_LEARNED || h->mlp.record) && !after_drift;
+  // The network's inputs come from the evaluation before a drift. With reuse_positions that evaluation is
+  // velocity-only (it refreshes div v), so the position-dependent inputs are taken here, at the same positions.
+  const int feat = (h->cfg.h_mode == IHS_H_LEARNED || h->mlp.record) && (!after_drift || h->cfg.reuse_positions);
   launch_density(s, h->split, feat, n, sc.xm, D.h, D.vx, D.vy, D.vz, D.u, D.mat, sc.eos, sc.nbr, sc.ncount, n, sc.rpc,
                  sc.vh, D.rho, D.feat, sc.flags, sv);
   prof_mark(h, IHS_STAGE_DENSITY);
@@ -365,12 +370,35 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   prof_mark(h, IHS_STAGE_GRAVITY);
   h->force_evals++;
   h->have_fe = true;
+  h->pos_fresh = true;
+}
+
+// Velocity-only force evaluation (configuration reuse_positions): the first evaluation of a KDK step sees the
+// positions of the previous step's second evaluation, so its sort, tree, neighbour lists, h, density and gravity
+// are those of that evaluation; only the EOS (u was kicked) and the velocity-dependent SPH forces are recomputed.
+void force_eval_velocity(ihs_handle* h) {
+  hipStream_t s = h->stream;
+  const int64_t n = h->n;
+  MasterBufs& S = h->st[h->cur];
+  Scratch& sc = h->sc;
+  const SegView sv = seg_view(h);
+  const bool learned = h->cfg.h_mode == IHS_H_LEARNED;
+  prof_mark(h, -1);
+  launch_fe_init(s, sc.flags, sv);
+  launch_eos(s, h->split, n, S.rho, S.h, S.vx, S.vy, S.vz, S.u, S.mat, sc.eos, sc.rpc, sc.vh, sc.flags, sv);
+  prof_mark(h, IHS_STAGE_DENSITY);
+  launch_forces(s, h->split != 0, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, n,
+                (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph, learned ? S.feat : nullptr);
+  prof_mark(h, IHS_STAGE_FORCES);
+  h->force_evals++;
 }
 
 void kdk_step(ihs_handle* h) {
-  force_eval(h, false, false);  // the potential is read only after the step's second evaluation
+  if (h->cfg.reuse_positions && h->pos_fresh) force_eval_velocity(h);
+  else force_eval(h, false, false);  // the potential is read only after the step's second evaluation
   launch_kick_drift(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, seg_view(h));
   h->h_settled = false;
+  h->pos_fresh = false;
   prof_mark(h, IHS_STAGE_KICK);
   force_eval(h, true);
   launch_kick(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, seg_view(h));
@@ -443,6 +471,7 @@ void ihs_default_config(ihs_config* c) {
   c->h_margin_seeded = 1.05f;
   c->group_max = 64;
   c->quadrupole = 1;
+  c->reuse_positions = 0;
 }
 
 int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
@@ -454,7 +483,7 @@ int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
   if (c.capacity < 1 || !valid_leaf(c.leaf_size) || c.nbr_capacity < 1 || c.h_mode < 0 || c.h_mode > 2 ||
       c.precision < 0 || c.precision > 1 || !(c.theta >= 0.f) || !(c.h_margin >= 1.0f) ||
       !(c.h_margin_seeded >= 1.0f) || c.group_max < c.leaf_size || c.group_max > 2 * GROUP_MAX ||
-      c.quadrupole < 0 || c.quadrupole > 1)
+      c.quadrupole < 0 || c.quadrupole > 1 || c.reuse_positions < 0 || c.reuse_positions > 1)
     return IHS_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IHS_ERR_HIP;
@@ -544,6 +573,7 @@ int ihs_upload_batch(ihs_handle* h, const ihs_particle* p, const int64_t* counts
   h->cur = 0;
   h->have_fe = false;
   h->h_settled = false;
+  h->pos_fresh = false;
   Scratch& sc = h->sc;
   HIP_TRY(h, hipMemcpy(sc.eos, eos, n_mat * sizeof(ihs_eos), hipMemcpyHostToDevice));
   HIP_TRY(h, hipMemcpy(sc.segof, segof.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -659,9 +689,10 @@ int ihs_step_kdk(ihs_handle* h, int32_t substeps) {
   if (substeps < 0) return fail(h, IHS_ERR_ARG, "negative substeps");
   hipSetDevice(h->device);
   const bool graph = h->cfg.use_graph && !h->mlp.record && !h->prof_on;
-  if (graph && substeps > 0 && h->cfg.h_mode != IHS_H_FIXED && !h->h_settled) {
-    // Captured steps assume h was solved at the current positions (the state after any step); a step after
-    // an upload without a solve or after relaxation runs directly once.
+  const bool fresh_needed = h->cfg.reuse_positions && !h->pos_fresh;
+  if (graph && substeps > 0 && ((h->cfg.h_mode != IHS_H_FIXED && !h->h_settled) || fresh_needed)) {
+    // Captured steps assume h was solved (and, reusing positions, the forces evaluated) at the current
+    // positions, as after any step; a step after an upload without a solve or after relaxation runs directly.
     kdk_step(h);
     --substeps;
   }
@@ -670,18 +701,23 @@ int ihs_step_kdk(ihs_handle* h, int32_t substeps) {
     HIP_TRY(h, hipGetLastError());
     return IHS_OK;
   }
-  const int par = h->cur;  // a KDK step flips the buffers twice, so the parity is preserved
-  if (!h->graph_exec[par]) {
-    const int64_t fe0 = h->force_evals;
-    HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    kdk_step(h);
-    hipError_t e = hipStreamEndCapture(h->stream, &h->graph[par]);
-    h->force_evals = fe0;
-    if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
-    HIP_TRY(h, hipGraphInstantiate(&h->graph_exec[par], h->graph[par], nullptr, nullptr, 0));
-  }
+  // Each full evaluation flips the double-buffered state, so a step flips it twice, or once when its first
+  // evaluation reuses positions: one graph per starting buffer, each leaving the state where its capture did.
   for (int i = 0; i < substeps; ++i) {
+    const int par = h->cur;
+    if (!h->graph_exec[par]) {
+      const int64_t fe0 = h->force_evals;
+      HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+      kdk_step(h);
+      hipError_t e = hipStreamEndCapture(h->stream, &h->graph[par]);
+      h->force_evals = fe0;
+      h->graph_next[par] = h->cur;
+      h->cur = par;
+      if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
+      HIP_TRY(h, hipGraphInstantiate(&h->graph_exec[par], h->graph[par], nullptr, nullptr, 0));
+    }
     HIP_TRY(h, hipGraphLaunch(h->graph_exec[par], h->stream));
+    h->cur = h->graph_next[par];
     h->force_evals += 2;
     h->have_fe = true;
   }
@@ -696,6 +732,7 @@ int ihs_relax(ihs_handle* h, int32_t steps) {
     force_eval(h, false);
     launch_relax(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, seg_view(h));
     h->h_settled = false;
+    h->pos_fresh = false;
   }
   HIP_TRY(h, hipGetLastError());
   return IHS_OK;

@@ -416,6 +416,54 @@ __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, 
 
 // Density with h_ij, EOS, Courant signal; optional MLP features (neighbour count within h_i,
 // div v and |grad rho|, both with the h_ij kernel gradient).
+// The Courant signal is one word per segment that every block would otherwise contend on: one atomic per
+// block when the block lies in one segment (always, for a single scenario), else one per lane. Blocks of 256.
+__device__ __forceinline__ void signal_reduce(uint32_t key, int64_t t, int64_t n, bool valid, const SegView& sv) {
+  __shared__ uint32_t skey[4];
+  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+  const uint32_t sg0 = sv.of(b0), sg1 = sv.of(min(b0 + (int64_t)blockDim.x, n) - 1);
+  if (sg0 == sg1) {
+    key = wave_min(key);
+    if ((threadIdx.x & 63) == 0) skey[threadIdx.x >> 6] = key;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      key = min(min(skey[0], skey[1]), min(skey[2], skey[3]));
+      if (key != 0xffffffffu) atomicMin(&sv.clk[sg0].signal, key);
+    }
+  } else if (valid && key != 0xffffffffu) {
+    atomicMin(&sv.clk[sv.of(t)].signal, key);
+  }
+}
+
+// Velocity-only evaluation at unchanged positions: the density of the last evaluation stands (positions and h
+// are the same), so only the EOS (u changed with the kick), the particle velocities and the Courant signal are
+// refreshed before the forces.
+template <class PT>
+__global__ __launch_bounds__(256) void k_eos(int64_t n, const float* __restrict__ rho_in, const float* __restrict__ h,
+                                             const float* __restrict__ vx, const float* __restrict__ vy,
+                                             const float* __restrict__ vz, const PT* __restrict__ u,
+                                             const uint8_t* __restrict__ mat, const EosDev* __restrict__ eos,
+                                             float4* __restrict__ rpc, float4* __restrict__ vh,
+                                             uint32_t* __restrict__ flags, SegView sv) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = t < n;
+  uint32_t key = 0xffffffffu;
+  if (valid) {
+    const float rho = rho_in[t], hi = h[t];
+    const float vxi = vx[t], vyi = vy[t], vzi = vz[t];
+    const EosDev e = eos[mat[t]];
+    const float uu = (float)u[t];
+    const float p = till_pressure(e, rho, uu);
+    const float c = till_sound_speed(e, rho, uu, p);
+    rpc[t] = make_float4(rho, p / (rho * rho), c, p);
+    vh[t] = make_float4(vxi, vyi, vzi, hi);
+    const float sig = hi / fmaxf(c + sqrtf(vxi * vxi + vyi * vyi + vzi * vzi), 1.0f);
+    key = f2key(sig);
+    if (!(sig == sig)) atomicAdd(&flags[FLAG_NONFINITE], 1u);
+  }
+  signal_reduce(key, t, n, valid, sv);
+}
+
 template <class PT, bool FEAT>
 __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, const float* __restrict__ h,
                                                  const float* __restrict__ vx, const float* __restrict__ vy,
@@ -476,29 +524,17 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
     key = f2key(sig);
     if (!(sig == sig) || !(rho == rho)) atomicAdd(&flags[FLAG_NONFINITE], 1u);
   }
-  // The Courant signal is one word per segment that every block would otherwise contend on: one atomic per
-  // block when the block lies in one segment (always, for a single scenario), else one per lane.
-  __shared__ uint32_t skey[4];
-  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
-  const uint32_t sg0 = sv.of(b0), sg1 = sv.of(min(b0 + (int64_t)blockDim.x, n) - 1);
-  if (sg0 == sg1) {
-    key = wave_min(key);
-    if ((threadIdx.x & 63) == 0) skey[threadIdx.x >> 6] = key;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      key = min(min(skey[0], skey[1]), min(skey[2], skey[3]));
-      if (key != 0xffffffffu) atomicMin(&sv.clk[sg0].signal, key);
-    }
-  } else if (valid && key != 0xffffffffu) {
-    atomicMin(&sv.clk[sv.of(t)].signal, key);
-  }
+  signal_reduce(key, t, n, valid, sv);
 }
 
-template <bool KAHAN>
+// DIVV: also form the SPH velocity divergence (the network's first input) from the same pair terms, for an
+// evaluation whose density pass was skipped.
+template <bool KAHAN, bool DIVV>
 __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, const float4* __restrict__ vh,
                                                 const float4* __restrict__ rpc, const uint4* __restrict__ nbr4,
                                                 const uint32_t* __restrict__ ncount, int64_t stride, int64_t n,
-                                                const float* __restrict__ av, float4* __restrict__ asph) {
+                                                const float* __restrict__ av, float4* __restrict__ asph,
+                                                float4* __restrict__ feat) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const float alpha = av[0], beta = av[1];
@@ -506,7 +542,7 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
   const float4 vi = vh[t];
   const float4 ri = rpc[t];
   const uint32_t cnt = ncount[t];
-  float ax = 0.f, ay = 0.f, az = 0.f, de = 0.f;
+  float ax = 0.f, ay = 0.f, az = 0.f, de = 0.f, dv = 0.f;
   Kahan3 ka;
   ka.zero();
   struct Src {
@@ -545,7 +581,9 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
       ay = fmaf(fy, s, ay);
       az = fmaf(fz, s, az);
     }
-    de += 0.5f * q.w * coeff * (dvx * fx + dvy * fy + dvz * fz);
+    const float vdw = dvx * fx + dvy * fy + dvz * fz;
+    de += 0.5f * q.w * coeff * vdw;
+    if (DIVV) dv += q.w * vdw;
   });
   if (KAHAN) {
     ax = ka.s[0];
@@ -553,6 +591,7 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
     az = ka.s[2];
   }
   asph[t] = make_float4(ax, ay, az, de);
+  if (DIVV) feat[t].y = -dv / ri.x;
 }
 
 inline unsigned g256(int64_t n) { return (unsigned)((n + 255) / 256); }
@@ -602,8 +641,26 @@ void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4*
 }
 
 void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const float4* vh, const float4* rpc,
-                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, const float* av, float4* asph) {
+                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, const float* av, float4* asph,
+                   float4* divv_feat) {
   const uint4* nbr4 = (const uint4*)nbr;
-  if (kahan) k_forces<true><<<g256(n), 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph);
-  else k_forces<false><<<g256(n), 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph);
+  const unsigned g = g256(n);
+  if (divv_feat) {
+    if (kahan) k_forces<true, true><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph, divv_feat);
+    else k_forces<false, true><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph, divv_feat);
+  } else {
+    if (kahan) k_forces<true, false><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph, nullptr);
+    else k_forces<false, false><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph, nullptr);
+  }
+}
+
+void launch_eos(hipStream_t s, int split, int64_t n, const float* rho, const float* h, const float* vx, const float* vy,
+                const float* vz, const void* u, const uint8_t* mat, const void* eos, float4* rpc, float4* vh,
+                uint32_t* flags, const SegView& sv) {
+  const EosDev* e = (const EosDev*)eos;
+  const unsigned g = std::max(1u, g256(n));
+  if (split)
+    k_eos<double><<<g, 256, 0, s>>>(n, rho, h, vx, vy, vz, (const double*)u, mat, e, rpc, vh, flags, sv);
+  else
+    k_eos<float><<<g, 256, 0, s>>>(n, rho, h, vx, vy, vz, (const float*)u, mat, e, rpc, vh, flags, sv);
 }

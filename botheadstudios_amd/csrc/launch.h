@@ -180,8 +180,15 @@ void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4*
                     const float* vy, const float* vz, const void* u, const uint8_t* mat, const void* eos,
                     const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float4* rpc, float4* vh, float* rho,
                     float4* featv, uint32_t* flags, const SegView& sv);
+// divv_feat != null: also write the velocity divergence into divv_feat[t].y.
 void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const float4* vh, const float4* rpc,
-                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, const float* av, float4* asph);
+                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, const float* av, float4* asph,
+                   float4* divv_feat = nullptr);
+// Velocity-only evaluation at unchanged positions: EOS from the stored density and the current u, rpc / vh for
+// the forces, and the segments' Courant signals.
+void launch_eos(hipStream_t s, int split, int64_t n, const float* rho, const float* h, const float* vx, const float* vy,
+                const float* vz, const void* u, const uint8_t* mat, const void* eos, float4* rpc, float4* vh,
+                uint32_t* flags, const SegView& sv);
 
 // --- k_step.hip ---
 // Master particle state (SoA, current storage order). Positions and u are double in split precision,

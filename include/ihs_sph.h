@@ -71,7 +71,11 @@ typedef struct {
                             each lane owning one (<= 64) or two (<= 128) particles */
   int32_t quadrupole;    /* 1: accepted cells act with monopole + traceless quadrupole (default); 0: monopole only,
                             as the reference (bh_gravity.wgsl:375-386) */
-  int32_t reserved[4];
+  int32_t reuse_positions; /* 1: the first evaluation of a KDK step, at the positions the previous step's second
+                              evaluation saw, keeps that evaluation's tree, neighbours, h, density and gravity and
+                              recomputes only the EOS and the velocity-dependent SPH forces (0: recompute all, as
+                              the reference does, sph_step.wgsl:213-237) */
+  int32_t reserved[3];
 } ihs_config;
 
 typedef struct ihs_handle ihs_handle;
