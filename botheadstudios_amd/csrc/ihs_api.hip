@@ -29,6 +29,7 @@ struct Scratch {
   float4* accg = nullptr;
   uint32_t* nbr = nullptr;
   uint32_t* ncount = nullptr;
+  uint32_t* nmask = nullptr;  // support masks of the lists (null when lists exceed NMASK_WORDS * 32 entries)
   uint32_t* flags = nullptr;
   unsigned long long* counters = nullptr;
   double* epart = nullptr;
@@ -207,6 +208,7 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &s.accg, C));
   HIP_TRY(h, dalloc(h, &s.nbr, (size_t)C * h->cfg.nbr_capacity));
   HIP_TRY(h, dalloc(h, &s.ncount, C));
+  if (h->cfg.nbr_capacity <= NMASK_WORDS * 32) HIP_TRY(h, dalloc(h, &s.nmask, (size_t)NMASK_WORDS * C));
   HIP_TRY(h, dalloc(h, &s.flags, FLAG_COUNT));
   HIP_TRY(h, dalloc(h, &s.counters, CNT_SLOTS * CNT_STRIDE));
   HIP_TRY(h, dalloc(h, &s.epart, 9 * 1024));
@@ -319,7 +321,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   prof_mark(h, IHS_STAGE_SORT);
   float margin = 1.0f;
   if (newton) margin = (!after_drift && h->h_settled) ? kMarginSettled : h->cfg.h_margin;
-  launch_gather(s, h->split, n, h->tb.perm, S, D, sc.xm, sc.hs, margin, sv);
+  launch_gather(s, h->split, n, h->tb.perm, S, D, sc.xm, sc.vh, sc.hs, margin, sv);
   h->cur ^= 1;
   prof_mark(h, IHS_STAGE_GATHER);
   if (seeded) {
@@ -360,13 +362,14 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   // The network's inputs come from the evaluation before a drift. With reuse_positions that evaluation is
   // velocity-only (it refreshes div v), so the position-dependent inputs are taken here, at the same positions.
   const int feat = (h->cfg.h_mode == IHS_H_LEARNED || h->mlp.record) && (!after_drift || h->cfg.reuse_positions);
-  launch_density(s, h->split, feat, n, sc.xm, D.h, D.vx, D.vy, D.vz, D.u, D.mat, sc.eos, sc.nbr, sc.ncount, n, sc.rpc,
-                 sc.vh, D.rho, D.feat, sc.flags, sv);
+  launch_density(s, h->split, feat, n, sc.xm, D.h, D.u, D.mat, sc.eos, sc.nbr, sc.ncount, n, sc.rpc, sc.vh, D.rho,
+                 D.feat, sc.nmask, sc.flags, sv);
   prof_mark(h, IHS_STAGE_DENSITY);
-  launch_forces(s, kahan, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, n,
+  launch_forces(s, kahan, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, sc.nmask, n,
                 (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph);
   prof_mark(h, IHS_STAGE_FORCES);
-  launch_gravity_walk(s, n, K, kahan, sc.xm, h->tb, sv, sc.accg, sc.flags, sc.counters, need_pot, h->zero_soft);
+  launch_gravity_walk(s, n, K, kahan, sc.xm, h->tb, h->tb.ggroups, h->tb.nggroups, sv, sc.accg, sc.flags,
+                      sc.counters, need_pot, h->zero_soft);
   prof_mark(h, IHS_STAGE_GRAVITY);
   h->force_evals++;
   h->have_fe = true;
@@ -387,7 +390,7 @@ void force_eval_velocity(ihs_handle* h) {
   launch_fe_init(s, sc.flags, sv);
   launch_eos(s, h->split, n, S.rho, S.h, S.vx, S.vy, S.vz, S.u, S.mat, sc.eos, sc.rpc, sc.vh, sc.flags, sv);
   prof_mark(h, IHS_STAGE_DENSITY);
-  launch_forces(s, h->split != 0, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, n,
+  launch_forces(s, h->split != 0, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, sc.nmask, n,
                 (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph, learned ? S.feat : nullptr);
   prof_mark(h, IHS_STAGE_FORCES);
   h->force_evals++;
@@ -889,7 +892,8 @@ int ihs_gravity(ihs_handle* h, const double* pos3, const double* mass, int64_t n
     launch_upsweep(s, n, K, (float)theta, sc.xm, nullptr, h->tb);
     launch_groups(s, n, K, h->tb, gv);
     if (h->tb.group_max != GROUP_MAX) launch_groups(s, n, K, h->tb, gv, nullptr, GROUP_DIAG_HS, true);
-    launch_gravity_walk(s, n, K, 0, sc.xm, h->tb, gv, sc.accg, sc.flags, sc.counters, true, !(soft2 > 0.f));
+    launch_gravity_walk(s, n, K, 0, sc.xm, h->tb, h->tb.ggroups, h->tb.nggroups, gv, sc.accg, sc.flags, sc.counters,
+                        true, !(soft2 > 0.f));
     launch_acc_out(s, h->tb.perm, n, sc.accg, sc.gacc, sc.gpot);
     h->have_fe = false;  // the tree buffers now describe these bodies, not the SPH state
   }

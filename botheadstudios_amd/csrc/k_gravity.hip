@@ -25,10 +25,15 @@ struct WalkLds {
   float4 cell[GW_RING];
   float4 qa[QUAD ? GW_RING : 1];  // accepted cells' quadrupoles (xx, yy, zz, xy), (xz, yz)
   float2 qb[QUAD ? GW_RING : 1];
+  float4 mono[QUAD ? GW_RING : 1];  // accepted one-particle leaves: monopoles, evaluated without the quadrupole
   uint32_t leaf[GW_RING];
   float4 part[64];
   uint32_t pidx[64];
 };
+
+// Padding entry of the source rings: massless and far from every particle, so that 0 * 1/r stays 0 even without
+// softening.
+constexpr float4 kPadSource = {1.0e18f, 0.f, 0.f, 0.f};
 
 struct Acc {
   float a[3], pot;
@@ -135,7 +140,9 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
 
   const uint32_t leaf0 = (uint32_t)(L - 1);
   const uint32_t lpc = 64u / (uint32_t)K;
-  uint32_t sp = 1, chead = 0, ctail = 0, lhead = 0, ltail = 0;
+  uint32_t sp = 1, chead = 0, ctail = 0, lhead = 0, ltail = 0, mhead = 0, mtail = 0;
+  // With one particle per leaf a leaf's quadrupole is zero: accepted leaves go to a monopole ring.
+  const bool mono_leaves = QUAD && K == 1;
   wave_sync();
   if (lane == 0) S.stack[0] = (uint32_t)grp.w;  // the group's segment root
   wave_sync();
@@ -170,6 +177,24 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
     chead += cnt;
     n_cell += nvalid * real;
   };
+  auto eval_mono = [&](uint32_t cnt, uint32_t real) {
+    const float4* src = &S.mono[mhead & (GW_RING - 1)];
+    float ax[T], ay[T], az[T], pp[T];
+#pragma unroll
+    for (int q = 0; q < T; ++q) ax[q] = ay[q] = az[q] = pp[q] = 0.f;
+    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+#pragma unroll
+      for (uint32_t e = 0; e < 4; ++e) {
+        const float4 c = src[k0 + e];
+#pragma unroll
+        for (int q = 0; q < T; ++q) interact<POT>(c, me[q], soft2, false, ax[q], ay[q], az[q], pp[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < T; ++q) acc_add<KAHAN>(A[q], ax[q], ay[q], az[q], pp[q]);
+    mhead += cnt;
+    n_pp += nvalid * real;
+  };
   auto eval_leaves = [&](uint32_t nl) {
     const uint32_t li = lane / (uint32_t)K;
     const bool have = li < nl;
@@ -177,7 +202,7 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
     const int64_t src = (int64_t)leaf * K + (lane % (uint32_t)K);
     const bool ok = have && src < n;
     wave_sync();
-    S.part[lane] = ok ? xm[src] : make_float4(0.f, 0.f, 0.f, 0.f);
+    S.part[lane] = ok ? xm[src] : kPadSource;
     if (SELF) S.pidx[lane] = ok ? (uint32_t)src : IHS_NONE;
     wave_sync();
     float ax[T], ay[T], az[T], pp[T];
@@ -225,7 +250,11 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
     }
     const bool isleaf = live && !acc && node >= leaf0;
     const bool open = live && !acc && node < leaf0;
-    const uint64_t ma = __ballot(acc), mo = __ballot(open), ml = __ballot(isleaf);
+    const bool accm = acc && mono_leaves && node >= leaf0;
+    acc = acc && !accm;
+    const uint64_t ma = __ballot(acc), mo = __ballot(open), ml = __ballot(isleaf), mm = __ballot(accm);
+    if (accm) S.mono[(mtail + lanes_below(mm)) & (GW_RING - 1)] = c;
+    mtail += (uint32_t)__popcll(mm);
     if (acc) {
       const uint32_t slot = (ctail + lanes_below(ma)) & (GW_RING - 1);
       S.cell[slot] = c;
@@ -251,14 +280,23 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
     }
     wave_sync();
     while (ctail - chead >= 64u) eval_cells(64u, 64u);
+    while (mtail - mhead >= 64u) eval_mono(64u, 64u);
     while (ltail - lhead >= lpc) eval_leaves(lpc);
+  }
+  if (mtail != mhead) {
+    const uint32_t real = mtail - mhead;
+    const uint32_t pad = (4u - (real & 3u)) & 3u;
+    if (lane < pad) S.mono[(mtail + lane) & (GW_RING - 1)] = kPadSource;
+    wave_sync();
+    mtail += pad;
+    eval_mono(mtail - mhead, real);
   }
   if (ctail != chead) {
     const uint32_t real = ctail - chead;
     const uint32_t pad = (4u - (real & 3u)) & 3u;
     if (lane < pad) {
       const uint32_t slot = (ctail + lane) & (GW_RING - 1);
-      S.cell[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+      S.cell[slot] = kPadSource;
       if (QUAD) {
         S.qa[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
         S.qb[slot] = make_float2(0.f, 0.f);
@@ -468,8 +506,8 @@ void launch_acc_out(hipStream_t s, const uint32_t* perm, int64_t n, const float4
 }
 
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
-                         const SegView& sv, float4* accg, uint32_t* flags, unsigned long long* counters, bool pot,
-                         bool self) {
+                         const int4* groups, const uint32_t* ngroups, const SegView& sv, float4* accg, uint32_t* flags,
+                         unsigned long long* counters, bool pot, bool self) {
   const int64_t L = (n + K - 1) / K;
   const bool two = tb.group_max > GROUP_MAX;
   const bool quad = tb.quadrupole != 0;
@@ -480,7 +518,7 @@ void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float
     if (!nblk[variant])                                                                                       \
       nblk[variant] = persistent_blocks((const void*)k_gravity_walk<KA, SE, T, PO, QU>, GW_WAVES * 64);       \
     k_gravity_walk<KA, SE, T, PO, QU><<<nblk[variant], GW_WAVES * 64, 0, s>>>(                                \
-        xm, n, K, L, tb.child, tb.ncom, tb.nbmax, tb.nqa, tb.nqb, tb.ggroups, tb.nggroups, tb.wq, sv, accg,     \
+        xm, n, K, L, tb.child, tb.ncom, tb.nbmax, tb.nqa, tb.nqb, groups, ngroups, tb.wq, sv, accg,             \
         flags, counters);                                                                                     \
   } while (0)
 #define IHS_WALK_P(KA, SE, T, QU)           \

@@ -20,8 +20,7 @@ constexpr int NB_RING = 128;
 struct NbrLds {
   uint32_t stack[NB_STACK];
   uint32_t leaf[NB_RING];
-  float4 part[64];
-  float phs[64];
+  float4 part[64];  // candidate x, y, z and its searched radius squared
   uint32_t pidx[64];
 };
 
@@ -39,6 +38,7 @@ __device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t la
   const bool valid = lane < (uint32_t)grp.y;
   const float4 me = xm[valid ? t : t0];
   const float myhs = valid ? hs[t] : 0.f;
+  const float myhs2 = myhs * myhs;
   const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
   const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
   const float ghs = wave_max(myhs);
@@ -61,32 +61,38 @@ __device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t la
     const bool ok = have && src < n;
     wave_sync();
     // Empty slots sit at 1e30 with a zero radius: they can never pass the distance test.
-    S.part[lane] = ok ? xm[src] : make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.f);
-    S.phs[lane] = ok ? hs[src] : 0.f;
+    const float ph = ok ? hs[src] : 0.f;
+    const float4 pp = ok ? xm[src] : make_float4(1.0e30f, 1.0e30f, 1.0e30f, 0.f);
+    S.part[lane] = make_float4(pp.x, pp.y, pp.z, ph * ph);
     S.pidx[lane] = ok ? (uint32_t)src : IHS_NONE;
     wave_sync();
     const uint32_t m = (min(64u, nl * (uint32_t)K) + 3u) & ~3u;
     ncand += nvalid * m;
-    // Branch-free test of every candidate into a per-lane hit mask, then visit only the hits: a
-    // divergent branch inside the candidate loop would serialise the LDS loads.
-    uint64_t hits = 0;
-    for (uint32_t k0 = 0; k0 < m; k0 += 4) {
+    // Branch-free test of every candidate into a per-lane hit mask (r < hs_i or r < hs_j), then visit only the
+    // hits: a divergent branch inside the candidate loop would serialise the LDS loads. The loop is unrolled
+    // so that each candidate's mask bit is a constant; the particle itself is dropped among the hits.
+    uint32_t hlo = 0, hhi = 0;
 #pragma unroll
-      for (uint32_t e = 0; e < 4; ++e) {
-        const float4 p = S.part[k0 + e];
-        const uint32_t j = S.pidx[k0 + e];
-        const float R = fmaxf(myhs, S.phs[k0 + e]);
-        const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
-        const float r2 = dx * dx + dy * dy + dz * dz;
-        const uint64_t hit = (uint64_t)((uint32_t)(r2 < R * R) & (uint32_t)(j != tt));  // no short-circuit
-        hits |= hit << (k0 + e);
+    for (uint32_t k0 = 0; k0 < 64; k0 += 4) {
+      if (k0 < m) {
+#pragma unroll
+        for (uint32_t e = 0; e < 4; ++e) {
+          const uint32_t k = k0 + e;
+          const float4 p = S.part[k];
+          const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
+          const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+          const bool hit = (r2 < myhs2) | (r2 < p.w);
+          if (k < 32) hlo |= hit ? (1u << k) : 0u;
+          else hhi |= hit ? (1u << (k - 32)) : 0u;
+        }
       }
     }
-    if (!valid) hits = 0;
+    uint64_t hits = valid ? ((uint64_t)hhi << 32) | hlo : 0ull;
     while (hits) {
       const uint32_t k = (uint32_t)__builtin_ctzll(hits);
       hits &= hits - 1;
       const uint32_t j = S.pidx[k];
+      if (j == tt) continue;
       // Neighbours are staged four at a time and written as one 16-byte chunk.
       b0 = nb == 0 ? j : b0;
       b1 = nb == 1 ? j : b1;
@@ -208,6 +214,34 @@ __device__ __forceinline__ void for_neighbours(const uint4* __restrict__ nbr4, i
     use(q.z, c, k + 2 < cnt);
     use(q.w, d, k + 3 < cnt);
     q = qn;
+  }
+}
+
+// As for_neighbours, visiting only the entries whose bit is set in the particle's support mask (bit k of word
+// k / 32 at (k / 32) * stride + t): the sources of the other entries are not fetched.
+template <class Fetch, class Use>
+__device__ __forceinline__ void for_support(const uint4* __restrict__ nbr4, const uint32_t* __restrict__ mask,
+                                            int64_t stride, int64_t t, uint32_t cnt, Fetch&& fetch, Use&& use) {
+  if (cnt == 0) return;
+  uint4 q = nbr4[t];
+  uint32_t mw = mask[t];
+  for (uint32_t k = 0; k < cnt; k += 4) {
+    const uint32_t kn = k + 4;
+    const uint4 qn = kn < cnt ? nbr4[(int64_t)(kn >> 2) * stride + t] : q;
+    const uint32_t mwn = (kn & 31u) == 0u && kn < cnt ? mask[(int64_t)(kn >> 5) * stride + t] : mw;
+    const uint32_t bits = mw >> (k & 31u);
+    using S = decltype(fetch(0u));
+    S a{}, b{}, c{}, d{};
+    if (bits & 1u) a = fetch(q.x);
+    if (bits & 2u) b = fetch(q.y);
+    if (bits & 4u) c = fetch(q.z);
+    if (bits & 8u) d = fetch(q.w);
+    use(q.x, a, (bits & 1u) != 0u);
+    use(q.y, b, (bits & 2u) != 0u);
+    use(q.z, c, (bits & 4u) != 0u);
+    use(q.w, d, (bits & 8u) != 0u);
+    q = qn;
+    mw = mwn;
   }
 }
 
@@ -464,33 +498,45 @@ __global__ __launch_bounds__(256) void k_eos(int64_t n, const float* __restrict_
   signal_reduce(key, t, n, valid, sv);
 }
 
+// Density, EOS and the Courant signal, one thread per particle.
+// vh holds (v, h_prev) from the permutation pass on entry (the feature pass reads the neighbours' velocities
+// from it, one 16-byte load each); its h is set here to the solved h.
+// mask (optional): the support mask of each list, bit k set when entry k is inside the kernel support
+// (r < h_ij), the pairs the force pass sums; entries between h_ij and the searched radius are then not fetched
+// again. The words are assembled in LDS (thread-private) and written once, after the loop.
 template <class PT, bool FEAT>
 __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, const float* __restrict__ h,
-                                                 const float* __restrict__ vx, const float* __restrict__ vy,
-                                                 const float* __restrict__ vz, const PT* __restrict__ u,
-                                                 const uint8_t* __restrict__ mat, const EosDev* __restrict__ eos,
-                                                 const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
-                                                 int64_t stride, int64_t n, float4* __restrict__ rpc,
-                                                 float4* __restrict__ vh, float* __restrict__ rho_out,
-                                                 float4* __restrict__ feat, uint32_t* __restrict__ flags, SegView sv) {
+                                                 const PT* __restrict__ u, const uint8_t* __restrict__ mat,
+                                                 const EosDev* __restrict__ eos, const uint4* __restrict__ nbr4,
+                                                 const uint32_t* __restrict__ ncount, int64_t stride, int64_t n,
+                                                 float4* __restrict__ rpc, float4* vh, float* __restrict__ rho_out,
+                                                 float4* __restrict__ feat, uint32_t* __restrict__ mask,
+                                                 uint32_t* __restrict__ flags, SegView sv) {
+  __shared__ uint32_t smask[NMASK_WORDS * 256];
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = t < n;
   uint32_t key = 0xffffffffu;
   if (valid) {
     const float4 me = xm[t];
     const float hi = h[t];
-    const float vxi = vx[t], vyi = vy[t], vzi = vz[t];
+    const float4 vi = vh[t];
     const uint32_t cnt = ncount[t];
     const float hinv_i = 1.0f / hi;
     float rho = me.w * (8.0f / kPI32) * hinv_i * hinv_i * hinv_i;
     float dv = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
-    uint32_t nn = 0;
+    uint32_t nn = 0, ent = 0;
+    const uint32_t nwords = (cnt + 31u) >> 5;
+    uint32_t* my = &smask[threadIdx.x];
+    if (mask)
+      for (uint32_t w = 0; w < nwords; ++w) my[w * 256] = 0u;
     struct Src {
       float4 q;
       float hj;
+      float4 vj;
     };
-    for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return Src{xm[j], h[j]}; },
-                   [&](uint32_t j, const Src src, bool ok) {
+    for_neighbours(nbr4, stride, t, cnt,
+                   [&](uint32_t j) { return Src{xm[j], h[j], FEAT ? vh[j] : make_float4(0.f, 0.f, 0.f, 0.f)}; },
+                   [&](uint32_t, const Src src, bool ok) {
       const float4 q = src.q;
       const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
       const float r2 = dx * dx + dy * dy + dz * dz;
@@ -508,19 +554,23 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
           gx += q.w * ddx;
           gy += q.w * ddy;
           gz += q.w * ddz;
-          dv += q.w * ((vxi - vx[j]) * ddx + (vyi - vy[j]) * ddy + (vzi - vz[j]) * ddz);
+          dv += q.w * ((vi.x - src.vj.x) * ddx + (vi.y - src.vj.y) * ddy + (vi.z - src.vj.z) * ddz);
         }
+        if (mask) atomicOr(&my[(ent >> 5) * 256], 1u << (ent & 31u));
       }
+      ++ent;
     });
+    if (mask)
+      for (uint32_t w = 0; w < nwords; ++w) mask[(int64_t)w * stride + t] = my[w * 256];
     const EosDev e = eos[mat[t]];
     const float uu = (float)u[t];
     const float p = till_pressure(e, rho, uu);
     const float c = till_sound_speed(e, rho, uu, p);
     rpc[t] = make_float4(rho, p / (rho * rho), c, p);
-    vh[t] = make_float4(vxi, vyi, vzi, hi);
+    vh[t].w = hi;
     rho_out[t] = rho;
     if (FEAT) feat[t] = make_float4((float)nn, -dv / rho, sqrtf(gx * gx + gy * gy + gz * gz), 0.f);
-    const float sig = hi / fmaxf(c + sqrtf(vxi * vxi + vyi * vyi + vzi * vzi), 1.0f);
+    const float sig = hi / fmaxf(c + sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z), 1.0f);
     key = f2key(sig);
     if (!(sig == sig) || !(rho == rho)) atomicAdd(&flags[FLAG_NONFINITE], 1u);
   }
@@ -532,7 +582,8 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
 template <bool KAHAN, bool DIVV>
 __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, const float4* __restrict__ vh,
                                                 const float4* __restrict__ rpc, const uint4* __restrict__ nbr4,
-                                                const uint32_t* __restrict__ ncount, int64_t stride, int64_t n,
+                                                const uint32_t* __restrict__ ncount,
+                                                const uint32_t* __restrict__ mask, int64_t stride, int64_t n,
                                                 const float* __restrict__ av, float4* __restrict__ asph,
                                                 float4* __restrict__ feat) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -548,8 +599,8 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
   struct Src {
     float4 q, vj, rj;
   };
-  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return Src{xm[j], vh[j], rpc[j]}; },
-                 [&](uint32_t, const Src src, bool ok) {
+  auto fetch = [&](uint32_t j) { return Src{xm[j], vh[j], rpc[j]}; };
+  auto use = [&](uint32_t, const Src& src, bool ok) {
     const float4 q = src.q, vj = src.vj, rj = src.rj;
     const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
@@ -584,7 +635,9 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
     const float vdw = dvx * fx + dvy * fy + dvz * fz;
     de += 0.5f * q.w * coeff * vdw;
     if (DIVV) dv += q.w * vdw;
-  });
+  };
+  if (mask) for_support(nbr4, mask, stride, t, cnt, fetch, use);
+  else for_neighbours(nbr4, stride, t, cnt, fetch, use);
   if (KAHAN) {
     ax = ka.s[0];
     ay = ka.s[1];
@@ -622,14 +675,14 @@ void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, 
                                   sv, hw.pend, hw.bracket, hw.npend, flags, counters);
 }
 
-void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const float* vx,
-                    const float* vy, const float* vz, const void* u, const uint8_t* mat, const void* eos,
-                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float4* rpc, float4* vh, float* rho,
-                    float4* featv, uint32_t* flags, const SegView& sv) {
+void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const void* u,
+                    const uint8_t* mat, const void* eos, const uint32_t* nbr, const uint32_t* ncount, int64_t stride,
+                    float4* rpc, float4* vh, float* rho, float4* featv, uint32_t* mask, uint32_t* flags,
+                    const SegView& sv) {
   const EosDev* e = (const EosDev*)eos;
 #define IHS_DENS(PT, F)                                                                                              \
-  k_density<PT, F><<<g256(n), 256, 0, s>>>(xm, h, vx, vy, vz, (const PT*)u, mat, e, (const uint4*)nbr, ncount, stride, n, rpc, vh, \
-                                           rho, featv, flags, sv)
+  k_density<PT, F><<<g256(n), 256, 0, s>>>(xm, h, (const PT*)u, mat, e, (const uint4*)nbr, ncount, stride, n, rpc, \
+                                           vh, rho, featv, mask, flags, sv)
   if (split) {
     if (feat) IHS_DENS(double, true);
     else IHS_DENS(double, false);
@@ -641,16 +694,16 @@ void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4*
 }
 
 void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const float4* vh, const float4* rpc,
-                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, const float* av, float4* asph,
-                   float4* divv_feat) {
+                   const uint32_t* nbr, const uint32_t* ncount, const uint32_t* mask, int64_t stride, const float* av,
+                   float4* asph, float4* divv_feat) {
   const uint4* nbr4 = (const uint4*)nbr;
   const unsigned g = g256(n);
   if (divv_feat) {
-    if (kahan) k_forces<true, true><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph, divv_feat);
-    else k_forces<false, true><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph, divv_feat);
+    if (kahan) k_forces<true, true><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, mask, stride, n, av, asph, divv_feat);
+    else k_forces<false, true><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, mask, stride, n, av, asph, divv_feat);
   } else {
-    if (kahan) k_forces<true, false><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph, nullptr);
-    else k_forces<false, false><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, stride, n, av, asph, nullptr);
+    if (kahan) k_forces<true, false><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, mask, stride, n, av, asph, nullptr);
+    else k_forces<false, false><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, mask, stride, n, av, asph, nullptr);
   }
 }
 

@@ -31,8 +31,8 @@ __global__ __launch_bounds__(256) void k_retry_hs(int64_t n, const float* __rest
 
 template <class PT>
 __global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ perm, int64_t n, MasterBufs S, MasterBufs D,
-                                                float4* __restrict__ xm, float* __restrict__ hs, float margin,
-                                                SegView sv) {
+                                                float4* __restrict__ xm, float4* __restrict__ vh,
+                                                float* __restrict__ hs, float margin, SegView sv) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const uint32_t s = perm[t];
@@ -40,9 +40,10 @@ __global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ per
   ((PT*)D.px)[t] = x;
   ((PT*)D.py)[t] = y;
   ((PT*)D.pz)[t] = z;
-  D.vx[t] = S.vx[s];
-  D.vy[t] = S.vy[s];
-  D.vz[t] = S.vz[s];
+  const float vx = S.vx[s], vy = S.vy[s], vz = S.vz[s];
+  D.vx[t] = vx;
+  D.vy[t] = vy;
+  D.vz[t] = vz;
   ((PT*)D.u)[t] = ((const PT*)S.u)[s];
   const float h = S.h[s];
   D.h[t] = h;
@@ -55,6 +56,7 @@ __global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ per
   D.feat[t] = S.feat[s];
   D.hiter[t] = S.hiter[s];
   xm[t] = make_float4((float)x, (float)y, (float)z, m);
+  vh[t] = make_float4(vx, vy, vz, h);
   hs[t] = fminf(h * margin, fmaxf(sv.hcap[sv.of(t)], h));
 }
 
@@ -263,9 +265,9 @@ void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float 
 }
 
 void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,
-                   const MasterBufs& dst, float4* xm, float* hs, float h_margin, const SegView& sv) {
-  if (split) k_gather<double><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, hs, h_margin, sv);
-  else k_gather<float><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, hs, h_margin, sv);
+                   const MasterBufs& dst, float4* xm, float4* vh, float* hs, float h_margin, const SegView& sv) {
+  if (split) k_gather<double><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, vh, hs, h_margin, sv);
+  else k_gather<float><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, vh, hs, h_margin, sv);
 }
 
 void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph,

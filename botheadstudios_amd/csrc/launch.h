@@ -101,6 +101,7 @@ struct TreeBufs {
   uint32_t* codes0 = nullptr;  // n: Morton codes by storage position (the segment pass re-gathers them)
 };
 
+constexpr int NMASK_WORDS = 8;           // support masks cover neighbour lists of up to 256 entries
 constexpr int GROUP_MAX = 64;            // one wavefront: the largest walk group
 constexpr float GROUP_DIAG_HS = 6.0f;     // a group's box diagonal is at most this many search radii (at the
                                           // full search margin; scaled so the bound stays fixed in units of h)
@@ -141,11 +142,11 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
 // --- k_gravity.hip ---
 // Wave-coherent Barnes-Hut group walk over sorted particles; accg = (a/G xyz... see kernel), sorted order.
 // Persistent: waves pull groups from per-XCD queues (tb.wq[0..8)).
-// Groups come from tb.ggroups; pot = false skips the potential (accg.w = 0).
+// Groups come from groups/ngroups (tb.ggroups); pot = false skips the potential (accg.w = 0).
 // self: some segment has zero softening (the walk then skips self pairs by index).
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
-                         const SegView& sv, float4* accg, uint32_t* flags, unsigned long long* counters,
-                         bool pot = true, bool self = false);
+                         const int4* groups, const uint32_t* ngroups, const SegView& sv, float4* accg, uint32_t* flags,
+                         unsigned long long* counters, bool pot, bool self);
 // FP64 direct-sum accelerations (G included) at the particles with the given ids, from the state's positions
 // in storage order (accuracy probe of the walk), each over its own scenario with its softening. inv needs n
 // entries, tpos nt.
@@ -175,15 +176,17 @@ void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, 
 // Wider-radius pass set-up (gated): hs = min(h * margin, max(hcap, h)); counts the pass in FLAG_RETRIES.
 void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float margin, const SegView& sv,
                      uint32_t* flags);
-// Also reduces each segment's Courant signal into sv.clk[s].signal.
-void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const float* vx,
-                    const float* vy, const float* vz, const void* u, const uint8_t* mat, const void* eos,
-                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float4* rpc, float4* vh, float* rho,
-                    float4* featv, uint32_t* flags, const SegView& sv);
+// Also reduces each segment's Courant signal into sv.clk[s].signal. vh enters as (v, h_prev) and leaves as (v, h).
+// mask (optional, lists of at most NMASK_WORDS * 32 entries): receives each list's support mask, bit k of word
+// k / 32 at (k / 32) * stride + t set when entry k lies inside the kernel support r < h_ij.
+void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const void* u,
+                    const uint8_t* mat, const void* eos, const uint32_t* nbr, const uint32_t* ncount, int64_t stride,
+                    float4* rpc, float4* vh, float* rho, float4* featv, uint32_t* mask, uint32_t* flags,
+                    const SegView& sv);
 // divv_feat != null: also write the velocity divergence into divv_feat[t].y.
 void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const float4* vh, const float4* rpc,
-                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, const float* av, float4* asph,
-                   float4* divv_feat = nullptr);
+                   const uint32_t* nbr, const uint32_t* ncount, const uint32_t* mask, int64_t stride, const float* av,
+                   float4* asph, float4* divv_feat = nullptr);
 // Velocity-only evaluation at unchanged positions: EOS from the stored density and the current u, rpc / vh for
 // the forces, and the segments' Courant signals.
 void launch_eos(hipStream_t s, int split, int64_t n, const float* rho, const float* h, const float* vx, const float* vy,
@@ -233,8 +236,9 @@ struct SphParticle48 {
 };
 
 void launch_fe_init(hipStream_t s, uint32_t* flags, const SegView& sv);
+// Permutes the state into the new order and writes the working copies xm (x, m), vh (v, h) and hs.
 void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,
-                   const MasterBufs& dst, float4* xm, float* hs, float h_margin, const SegView& sv);
+                   const MasterBufs& dst, float4* xm, float4* vh, float* hs, float h_margin, const SegView& sv);
 void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph,
                        const float4* accg, const StepCtl* ctl, const SegView& sv);
 void launch_kick(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
