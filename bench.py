@@ -51,7 +51,45 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched 64-scenario sweep line")
     ap.add_argument("--sweep-steps", type=int, default=10)
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="first copy MIB MiB device to device (hipMemcpy): the known-size kernel that calibrates "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
     return ap.parse_args()
+
+
+def copy_probe(mib: int):
+    """One device-to-device copy of mib MiB (the runtime's copy kernel reads and writes exactly that much)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    size = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), size) or hip.hipMalloc(C.byref(b), size):
+        raise RuntimeError("hipMalloc failed for the copy probe")
+    try:
+        if hip.hipMemset(a, 1, size) or hip.hipMemcpy(b, a, size, 3) or hip.hipDeviceSynchronize():
+            raise RuntimeError("copy probe failed")
+    finally:
+        hip.hipFree(a)
+        hip.hipFree(b)
+    log(f"copy probe: {mib} MiB device to device")
+
+
+def measured_traffic(n, args, variant):
+    """HBM bytes per k_gravity_walk launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when that
+    summary was taken on this workload, walk setting and variant (else None)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_gravity_walk_1M_traffic.md")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        line = next((l for l in f if l.startswith("traffic: ")), None)
+    if line is None:
+        return None
+    rec = json.loads(line[len("traffic: "):])
+    same = (rec.get("n") == n and rec.get("leaf") == args.leaf and rec.get("group") == args.group and
+            abs(rec.get("theta", -1) - args.theta) < 1e-6 and rec.get("quadrupole") == bool(args.quadrupole) and
+            rec.get("variant") == variant)
+    return rec["bytes_per_launch"] if same else None
 
 
 def log(msg):
@@ -96,7 +134,7 @@ def run_variant(ic, precision, h_mode, args, weights=None, reuse=False):
         sph.close()
 
 
-def roofline(res, n, args):
+def roofline(res, n, args, variant):
     """Dominant stage -> achieved vs peak. Gravity (FP32 VALU): 20 flops per particle-particle or monopole cell
     interaction, 52 per quadrupole cell interaction."""
     prof = res["profile"]
@@ -111,7 +149,8 @@ def roofline(res, n, args):
                  FLOPS_PER_INTERACTION * pst["grav_pp_interactions"]) / max(calls, 1)
         achieved = flops / (avg_ms * 1e-3) / 1e12
         return dict(kernel="k_gravity_walk", bound="mfma", achieved=achieved, peak=FP32_PEAK_TFLOPS,
-                    unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=None, avg_ms=avg_ms,
+                    unit="TFLOP/s", frac=achieved / FP32_PEAK_TFLOPS, traffic=measured_traffic(n, args, variant),
+                    avg_ms=avg_ms,
                     per_launch=dict(interactions=inter / max(calls, 1), flops=flops,
                                     cell_per_particle=pst["grav_cell_interactions"] / max(calls, 1) / n,
                                     pp_per_particle=pst["grav_pp_interactions"] / max(calls, 1) / n))
@@ -165,6 +204,8 @@ def main():
     args.group = tuned["group"] if args.group is None else args.group
     args.quadrupole = tuned["quadrupole"] if args.quadrupole is None else bool(args.quadrupole)
     log(f"walk parameters: leaf {args.leaf}, group {args.group}, theta {args.theta}, quadrupole {args.quadrupole}")
+    if args.copy_probe > 0:
+        copy_probe(args.copy_probe)
     variants = {}
     wanted = args.variants.split(",")
     if "plain" in wanted:
@@ -194,7 +235,7 @@ def main():
         "config": {"workload": f"two-body giant impact, N={ic.n}, 0.9+0.1 Earth masses, 45 deg, 1.1 v_esc; "
                                f"KDK step = 2 force evaluations; variant={head}",
                    "n": ic.n, "leaf": args.leaf, "group": args.group, "theta": args.theta, "quadrupole": args.quadrupole, "n_ngb": 48, "h_tol": 1e-4, "variant": head},
-        "roofline": roofline(res, ic.n, args),
+        "roofline": roofline(res, ic.n, args, head),
         "variants": {k: {"value": v["value"], "ms_per_step": v["ms_per_step"],
                          "mean_h_evals": v["stats"]["mean_h_evals"], "mean_h_iters": v["stats"]["mean_h_iters"],
                          "mean_neighbours": v["stats"]["mean_neighbours"],
