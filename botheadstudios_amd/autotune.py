@@ -30,7 +30,7 @@ import time
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(HERE, "data", "tuned.json")
 DEFAULT = {"leaf": 16, "group": 64, "theta": 0.5, "quadrupole": True}
-QUAD_THETAS = (1.2, 1.1, 1.0, 0.9, 0.85, 0.8, 0.75, 0.7, 0.6, 0.5, 0.45, 0.4)
+QUAD_THETAS = (1.2, 1.1, 1.0, 0.95, 0.9, 0.875, 0.85, 0.825, 0.8, 0.75, 0.7, 0.6, 0.5, 0.45, 0.4)
 MONO_THETAS = (0.5, 0.45, 0.42, 0.4, 0.375, 0.35)
 
 

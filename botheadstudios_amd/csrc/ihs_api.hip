@@ -191,7 +191,7 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &t.gblk, Lc / 1024 + 2));
   HIP_TRY(h, dalloc(h, &t.groups, C + 1));
   HIP_TRY(h, dalloc(h, &t.ngroups, 1));
-  if (t.group_max != GROUP_MAX) {
+  if (t.group_max != GROUP_MAX || NBR_DIAG_HS != GROUP_DIAG_HS) {
     HIP_TRY(h, dalloc(h, &t.ggroups, C + 1));
     HIP_TRY(h, dalloc(h, &t.nggroups, 1));
   } else {
@@ -330,11 +330,11 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   }
   // Walk groups are bounded in units of h, whatever margin set this evaluation's search radius.
   const float eff_margin = seeded ? h->cfg.h_margin_seeded : margin;
-  const float diag_hs = newton ? GROUP_DIAG_HS * h->cfg.h_margin / eff_margin : GROUP_DIAG_HS;
+  const float diag_scale = newton ? h->cfg.h_margin / eff_margin : 1.0f;
   launch_karras(s, n, K, h->tb, sv);
   launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb);
-  launch_groups(s, n, K, h->tb, sv, nullptr, diag_hs);
-  if (h->tb.group_max != GROUP_MAX) launch_groups(s, n, K, h->tb, sv, nullptr, diag_hs, true);
+  launch_groups(s, n, K, h->tb, sv, nullptr, NBR_DIAG_HS * diag_scale);
+  if (h->tb.ggroups != h->tb.groups) launch_groups(s, n, K, h->tb, sv, nullptr, GROUP_DIAG_HS * diag_scale, true);
   prof_mark(h, IHS_STAGE_TREE);
   launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags, sc.counters);
   prof_mark(h, IHS_STAGE_NBR);
@@ -350,7 +350,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
       const uint32_t* gate = sc.flags + FLAG_H_RETRY;
       launch_retry_hs(s, n, D.h, sc.hs, h->cfg.h_margin, sv, sc.flags);
       launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, gate);
-      launch_groups(s, n, K, h->tb, sv, gate);
+      launch_groups(s, n, K, h->tb, sv, gate, NBR_DIAG_HS);
       launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags,
                        sc.counters, gate);
       launch_hsolve(s, n, sc.xm, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol, h->cfg.h_max_iter,
@@ -890,8 +890,8 @@ int ihs_gravity(ihs_handle* h, const double* pos3, const double* mass, int64_t n
     launch_gather_xm(s, h->tb.perm, n, sc.gpx, sc.gpy, sc.gpz, sc.gm, sc.xm);
     launch_karras(s, n, K, h->tb, gv);
     launch_upsweep(s, n, K, (float)theta, sc.xm, nullptr, h->tb);
-    launch_groups(s, n, K, h->tb, gv);
-    if (h->tb.group_max != GROUP_MAX) launch_groups(s, n, K, h->tb, gv, nullptr, GROUP_DIAG_HS, true);
+    if (h->tb.ggroups != h->tb.groups) launch_groups(s, n, K, h->tb, gv, nullptr, GROUP_DIAG_HS, true);
+    else launch_groups(s, n, K, h->tb, gv);
     launch_gravity_walk(s, n, K, 0, sc.xm, h->tb, h->tb.ggroups, h->tb.nggroups, gv, sc.accg, sc.flags, sc.counters,
                         true, !(soft2 > 0.f));
     launch_acc_out(s, h->tb.perm, n, sc.accg, sc.gacc, sc.gpot);

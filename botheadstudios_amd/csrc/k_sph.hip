@@ -8,6 +8,9 @@
 //                  the Tillotson P and c per particle (sph_step.wgsl:78-100) and the Courant signal
 //                  (sph_step.wgsl:262-269), fused;
 //   k_forces     — pressure + Monaghan AV + du/dt (ie/shaders/sph_step.wgsl:170-208), gather form.
+// The one-thread-per-particle kernels that gather neighbours map their blocks to particle ranges XCD by XCD
+// (xcd_block): blocks go round-robin to the eight XCDs, each with its own L2, and a contiguous Morton range per
+// XCD keeps a particle's neighbours in the L2 that reads them.
 #include "common.h"
 #include "launch.h"
 
@@ -384,7 +387,7 @@ __global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, 
                                                  uint32_t* __restrict__ npend, uint32_t* __restrict__ flags,
                                                  unsigned long long* __restrict__ counters) {
   if (gate && *gate == 0u) return;
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const bool valid = t < n;
   HsolveState st{0.f, 0.f, INFINITY, 0u, 0u, HS_DONE};
   float hsi = 0.f, hcap = 0.f;
@@ -454,7 +457,7 @@ __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, 
 // block when the block lies in one segment (always, for a single scenario), else one per lane. Blocks of 256.
 __device__ __forceinline__ void signal_reduce(uint32_t key, int64_t t, int64_t n, bool valid, const SegView& sv) {
   __shared__ uint32_t skey[4];
-  const int64_t b0 = (int64_t)blockIdx.x * blockDim.x;
+  const int64_t b0 = t - threadIdx.x;  // the block's first particle, whatever the block-to-range mapping
   const uint32_t sg0 = sv.of(b0), sg1 = sv.of(min(b0 + (int64_t)blockDim.x, n) - 1);
   if (sg0 == sg1) {
     key = wave_min(key);
@@ -513,7 +516,7 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
                                                  float4* __restrict__ feat, uint32_t* __restrict__ mask,
                                                  uint32_t* __restrict__ flags, SegView sv) {
   __shared__ uint32_t smask[NMASK_WORDS * 256];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const bool valid = t < n;
   uint32_t key = 0xffffffffu;
   if (valid) {
@@ -586,7 +589,7 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
                                                 const uint32_t* __restrict__ mask, int64_t stride, int64_t n,
                                                 const float* __restrict__ av, float4* __restrict__ asph,
                                                 float4* __restrict__ feat) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const float alpha = av[0], beta = av[1];
   const float4 me = xm[t];

@@ -103,8 +103,9 @@ struct TreeBufs {
 
 constexpr int NMASK_WORDS = 8;           // support masks cover neighbour lists of up to 256 entries
 constexpr int GROUP_MAX = 64;            // one wavefront: the largest walk group
-constexpr float GROUP_DIAG_HS = 6.0f;     // a group's box diagonal is at most this many search radii (at the
-                                          // full search margin; scaled so the bound stays fixed in units of h)
+constexpr float GROUP_DIAG_HS = 6.0f;     // a gravity group's box diagonal is at most this many search radii (at
+                                          // the full search margin; scaled so the bound stays fixed in units of h)
+constexpr float NBR_DIAG_HS = 6.0f;       // the same bound for neighbour-search groups (separate lists if unequal)
 constexpr uint32_t GROUP_SPLIT = 0xfffffffeu;
 
 constexpr int RADIX_THREADS = 256;

@@ -83,6 +83,22 @@ def test_newton_h_residual_and_neighbours():
     assert abs(bulk.mean() - 47.0) < 3.0, bulk.mean()  # 48 incl. self -> ~47 others
 
 
+def test_support_masks_do_not_change_the_forces():
+    """The density pass marks each list's entries inside the kernel support and the force pass fetches only those.
+    Lists too long for the masks (nbr_capacity > 256) take the unmasked path, which visits every entry and skips
+    the same ones by distance: the SPH accelerations and du/dt agree bit for bit."""
+    ic = ics.config_ic(1, n=16384)
+    out = []
+    for cap in (192, 320):
+        with GpuSph(ic.n, leaf_size=1, h_mode="newton", nbr_capacity=cap) as sph:
+            sph.upload(ic.particles48(), ic.eos, ic.softening)
+            sph.set_dt(5.0)
+            sph.encode_kdk(3)
+            sph.force_eval()
+            out.append(sph.field("asph"))
+    assert np.array_equal(out[0], out[1])
+
+
 def test_newton_matches_oracle_solution():
     """GPU Newton h equals the oracle's FP64 safeguarded Newton root to the tolerance-implied precision."""
     ic = ics.config_ic(1, n=4096)
