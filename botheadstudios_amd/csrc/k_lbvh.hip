@@ -377,23 +377,27 @@ __device__ __forceinline__ void leaf_moments(int64_t c, const float4* __restrict
                                              int64_t n, int K, int64_t L, float4* ncom, float4* nbmin, float4* nbmax,
                                              float4* nqa, float2* nqb, float inv_theta) {
   const int64_t start = c * K, end = min(start + (int64_t)K, n);
+  // The centre of mass is formed relative to the leaf's first particle: a one-particle leaf's is then exactly
+  // that particle's position (m x / m is not, in FP32), so a walk never sees a particle's own leaf as a source
+  // an ulp away from it.
+  const float4 b0 = xm[start];
   float m = 0.f, mx = 0.f, my = 0.f, mz = 0.f, hm = 0.f;
   float lo[3] = {1.0e30f, 1.0e30f, 1.0e30f}, hi[3] = {-1.0e30f, -1.0e30f, -1.0e30f};
   for (int64_t s = start; s < end; ++s) {
     const float4 b = xm[s];
     m += b.w;
-    mx += b.x * b.w;
-    my += b.y * b.w;
-    mz += b.z * b.w;
+    mx += (b.x - b0.x) * b.w;
+    my += (b.y - b0.y) * b.w;
+    mz += (b.z - b0.z) * b.w;
     lo[0] = fminf(lo[0], b.x); lo[1] = fminf(lo[1], b.y); lo[2] = fminf(lo[2], b.z);
     hi[0] = fmaxf(hi[0], b.x); hi[1] = fmaxf(hi[1], b.y); hi[2] = fmaxf(hi[2], b.z);
     if (hs) hm = fmaxf(hm, hs[s]);
   }
-  float4 cm = make_float4(0.f, 0.f, 0.f, m);
+  float4 cm = make_float4(b0.x, b0.y, b0.z, m);
   if (m > 0.f) {
-    cm.x = mx / m;
-    cm.y = my / m;
-    cm.z = mz / m;
+    cm.x += mx / m;
+    cm.y += my / m;
+    cm.z += mz / m;
   }
   // Second pass for the quadrupole about the leaf's centre of mass (no large-coordinate cancellation).
   float4 qa = make_float4(0.f, 0.f, 0.f, 0.f);

@@ -41,7 +41,7 @@ __device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t la
   const bool valid = lane < (uint32_t)grp.y;
   const float4 me = xm[valid ? t : t0];
   const float myhs = valid ? hs[t] : 0.f;
-  const float myhs2 = myhs * myhs;
+  const uint32_t myhs2u = __float_as_uint(myhs * myhs);
   const float gx0 = wave_min(me.x), gy0 = wave_min(me.y), gz0 = wave_min(me.z);
   const float gx1 = wave_max(me.x), gy1 = wave_max(me.y), gz1 = wave_max(me.z);
   const float ghs = wave_max(myhs);
@@ -84,7 +84,9 @@ __device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t la
           const float4 p = S.part[k];
           const float dx = p.x - me.x, dy = p.y - me.y, dz = p.z - me.z;
           const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-          const bool hit = (r2 < myhs2) | (r2 < p.w);
+          // r < max(hs_i, hs_j): non-negative floats order as their bit patterns, so the maximum and the
+          // comparison are integer ones (no NaN canonicalisation).
+          const bool hit = __float_as_uint(r2) < max(myhs2u, __float_as_uint(p.w));
           if (k < 32) hlo |= hit ? (1u << k) : 0u;
           else hhi |= hit ? (1u << (k - 32)) : 0u;
         }
@@ -96,11 +98,11 @@ __device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t la
       hits &= hits - 1;
       const uint32_t j = S.pidx[k];
       if (j == tt) continue;
-      // Neighbours are staged four at a time and written as one 16-byte chunk.
-      b0 = nb == 0 ? j : b0;
-      b1 = nb == 1 ? j : b1;
-      b2 = nb == 2 ? j : b2;
-      b3 = nb == 3 ? j : b3;
+      // Neighbours are staged four at a time (shifting in at b3) and written as one 16-byte chunk.
+      b0 = b1;
+      b1 = b2;
+      b2 = b3;
+      b3 = j;
       ++cnt;
       if (++nb == 4) {
         if (nchunk < cap4) nbr4[(int64_t)nchunk * stride + t] = make_uint4(b0, b1, b2, b3);
@@ -149,7 +151,10 @@ __device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t la
     while (ltail - lhead >= lpc) flush(lpc);
   }
   while (ltail != lhead) flush(min(lpc, ltail - lhead));
-  if (valid && nb > 0 && nchunk < cap4) nbr4[(int64_t)nchunk * stride + t] = make_uint4(b0, b1, b2, b3);
+  // A partial last chunk holds its nb entries in b[4-nb..3]: move them to the front, padding with the last.
+  if (valid && nb > 0 && nchunk < cap4)
+    nbr4[(int64_t)nchunk * stride + t] = nb == 1 ? make_uint4(b3, b3, b3, b3)
+                                      : nb == 2 ? make_uint4(b2, b3, b3, b3) : make_uint4(b1, b2, b3, b3);
   const uint32_t capn = cap4 * 4u;
   if (valid) ncount[t] = min(cnt, capn);
   const uint64_t over = __ballot(valid && cnt > capn);
