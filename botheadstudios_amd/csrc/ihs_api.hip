@@ -471,7 +471,7 @@ void ihs_default_config(ihs_config* c) {
   c->h_cap_factor = 4.0f;
   c->nbr_capacity = 192;
   c->use_graph = 1;
-  c->h_margin_seeded = 1.05f;
+  c->h_margin_seeded = 1.03f;
   c->group_max = 64;
   c->quadrupole = 1;
   c->reuse_positions = 0;

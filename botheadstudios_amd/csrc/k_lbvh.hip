@@ -328,6 +328,7 @@ constexpr int UPS_LEAVES = UPS_THREADS * UPS_PER_THREAD;  // leaves per block of
 // s = com_c - com. w = m_c / M.
 __device__ __forceinline__ void quad_shift(float4& qa, float2& qb, const float4 cq, const float4 qca, const float2 qcb,
                                            const float4 c, float w) {
+#pragma clang fp contract(off)
   const float sx = cq.x - c.x, sy = cq.y - c.y, sz = cq.z - c.z;
   const float s2 = sx * sx + sy * sy + sz * sz;
   qa.x += w * (qca.x + 3.0f * sx * sx - s2);
@@ -338,8 +339,11 @@ __device__ __forceinline__ void quad_shift(float4& qa, float2& qb, const float4 
   qb.y += w * (qcb.y + 3.0f * sy * sz);
 }
 
+// The moment arithmetic is not contracted into FMAs, so a node gets the same bits whichever kernel merges it
+// (batched and single scenarios cut the leaves into blocks at different places).
 __device__ __forceinline__ void node_merge(uint32_t p, const int2 ch, float4* ncom, float4* nbmin, float4* nbmax,
                                            float4* nqa, float2* nqb, float inv_theta) {
+#pragma clang fp contract(off)
   const float4 cl = ncom[ch.x], cr = ncom[ch.y];
   const float4 bl0 = nbmin[ch.x], br0 = nbmin[ch.y];
   const float4 bl1 = nbmax[ch.x], br1 = nbmax[ch.y];

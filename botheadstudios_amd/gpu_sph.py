@@ -50,7 +50,7 @@ class GpuSph:
     def __init__(self, capacity: int, device: int = 0, leaf_size: int = 16, precision: str = "fp32",
                  h_mode: str = "newton", theta: float = 0.5, cfl: float = 0.1, n_ngb: float = 48.0,
                  h_tol: float = 1e-4, h_max_iter: int = 40, h_margin: float = 1.25, h_cap_factor: float = 4.0,
-                 nbr_capacity: int = 192, use_graph: bool = True, h_margin_seeded: float = 1.05,
+                 nbr_capacity: int = 192, use_graph: bool = True, h_margin_seeded: float = 1.03,
                  group_max: int = 64, quadrupole: bool = True, reuse_positions: bool = False):
         self._L = _lib.load()
         cfg = default_config()

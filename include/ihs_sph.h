@@ -65,7 +65,7 @@ typedef struct {
   float h_cap_factor;    /* search radius never exceeds h_cap_factor * max uploaded h (4) */
   int32_t nbr_capacity;  /* neighbour-list entries per particle (192) */
   int32_t use_graph;     /* capture the KDK step in a hipGraph (1) */
-  float h_margin_seeded; /* LEARNED: search radius = h_margin_seeded * seed after a drift (1.05); a wider-radius
+  float h_margin_seeded; /* LEARNED: search radius = h_margin_seeded * seed after a drift (1.03); a wider-radius
                             pass (h_margin) runs only when some solve reaches it */
   int32_t group_max;     /* particles per gravity-walk group, leaf_size..128 (64): one wavefront walks one group,
                             each lane owning one (<= 64) or two (<= 128) particles */
