@@ -53,6 +53,18 @@ __device__ __forceinline__ T wave_max(T v) {
   }
   return v;
 }
+// Exclusive prefix sum over the 64 lanes of a wave (lane order).
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
+  const uint32_t lane = lane_id();
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= (uint32_t)o) x += y;
+  }
+  return x - v;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

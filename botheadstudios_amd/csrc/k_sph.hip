@@ -17,8 +17,9 @@
 namespace {
 
 constexpr int NB_WAVES = 4;
-constexpr int NB_STACK = 1024;
-constexpr int NB_RING = 128;
+constexpr int NB_STACK = 512;
+constexpr int NB_SMALL = 8;     // a subtree of at most this many leaves that passes the box test is taken whole
+constexpr int NB_RING = 1024;   // candidate leaves: < 64 pending + up to 64 x NB_SMALL per walk step
 
 struct NbrLds {
   uint32_t stack[NB_STACK];
@@ -29,9 +30,13 @@ struct NbrLds {
 
 // Neighbour list of one group (one wave): LBVH walk with box-vs-box tests against the larger search radius,
 // candidate leaves staged through LDS 64 particles at a time, each lane testing r < max(hs_i, hs_j).
+// The walk stops at subtrees of at most NB_SMALL leaves: one that passes the box test hands all its leaves (a
+// contiguous index range) to the candidate ring instead of being opened level by level. The lists then come out
+// in runs of consecutive indices, which the neighbour loops of the later passes gather with better locality.
 __device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t lane, const float4* __restrict__ xm,
                                           const float* __restrict__ hs, int64_t n, int K, int64_t L,
-                                          const int2* __restrict__ child, const float4* __restrict__ nbmin,
+                                          const int2* __restrict__ child, const int2* __restrict__ range,
+                                          const float4* __restrict__ nbmin,
                                           const float4* __restrict__ nbmax, uint4* __restrict__ nbr4,
                                           uint32_t* __restrict__ ncount, int cap, int64_t stride,
                                           uint32_t* __restrict__ flags, uint64_t& ncand, uint64_t& nnodes,
@@ -122,25 +127,33 @@ __device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t la
     nnodes += take;
     wave_sync();
     bool pass = false;
+    int2 rg = make_int2(0, 0), ch = make_int2(0, 0);
     if (node != IHS_NONE) {
       const float4 c0 = nbmin[node], c1 = nbmax[node];
+      if (node < leaf0) {  // fetched with the box: no second dependent load for an opened node
+        rg = range[node];
+        ch = child[node];
+      }
       const float dx = fmaxf(fmaxf(c0.x - gx1, gx0 - c1.x), 0.f);
       const float dy = fmaxf(fmaxf(c0.y - gy1, gy0 - c1.y), 0.f);
       const float dz = fmaxf(fmaxf(c0.z - gz1, gz0 - c1.z), 0.f);
       const float R = fmaxf(ghs, c0.w);
       pass = dx * dx + dy * dy + dz * dz < R * R;
     }
-    const bool isleaf = pass && node >= leaf0;
-    const bool open = pass && node < leaf0;
-    const uint64_t mo = __ballot(open), ml = __ballot(isleaf);
-    if (isleaf) S.leaf[(ltail + lanes_below(ml)) & (NB_RING - 1)] = node - leaf0;
-    ltail += (uint32_t)__popcll(ml);
+    const uint32_t nleaf = node >= leaf0 ? 1u : (uint32_t)(rg.y - rg.x + 1);
+    const bool whole = pass && nleaf <= (uint32_t)NB_SMALL;
+    const bool open = pass && !whole;
+    const uint32_t first = node >= leaf0 ? node - leaf0 : (uint32_t)rg.x;
+    const uint32_t take_l = whole ? nleaf : 0u;
+    const uint32_t off = wave_excl_scan(take_l);
+    for (uint32_t k = 0; k < take_l; ++k) S.leaf[(ltail + off + k) & (NB_RING - 1)] = first + k;
+    ltail += wave_sum(take_l);
+    const uint64_t mo = __ballot(open);
     const uint32_t nopen = (uint32_t)__popcll(mo);
     if (sp + 2u * nopen > (uint32_t)NB_STACK) {
       if (lane == 0) atomicAdd(&flags[FLAG_WALK_OVERFLOW], 1u);
     } else {
       if (open) {
-        const int2 ch = child[node];
         const uint32_t pos = sp + 2u * lanes_below(mo);
         S.stack[pos] = (uint32_t)ch.x;
         S.stack[pos + 1] = (uint32_t)ch.y;
@@ -165,6 +178,7 @@ __device__ __forceinline__ void nbr_group(NbrLds& S, const int4 grp, uint32_t la
 // Persistent: waves pull groups from per-XCD queues (wq[8..16)), then help the other queues.
 __global__ __launch_bounds__(NB_WAVES * 64) void k_nbr_build(const float4* __restrict__ xm, const float* __restrict__ hs,
                                                              int64_t n, int K, int64_t L, const int2* __restrict__ child,
+                                                             const int2* __restrict__ range,
                                                              const float4* __restrict__ nbmin,
                                                              const float4* __restrict__ nbmax,
                                                              const int4* __restrict__ groups,
@@ -190,8 +204,8 @@ __global__ __launch_bounds__(NB_WAVES * 64) void k_nbr_build(const float4* __res
       if (lane == 0) g = atomicAdd(&wq[8 + q], 1u);
       g = (uint32_t)__builtin_amdgcn_readfirstlane((int)g) + lo;
       if (g >= hi) break;
-      nbr_group(S, groups[g], lane, xm, hs, n, K, L, child, nbmin, nbmax, nbr4, ncount, cap, stride, flags, ncand,
-                nnodes, npairs);
+      nbr_group(S, groups[g], lane, xm, hs, n, K, L, child, range, nbmin, nbmax, nbr4, ncount, cap, stride, flags,
+                ncand, nnodes, npairs);
     }
   }
   if (lane == 0) {
@@ -666,8 +680,8 @@ void launch_nbr_build(hipStream_t s, int64_t n, int K, const float4* xm, const f
   const int64_t L = (n + K - 1) / K;
   static unsigned nblk = 0;
   if (!nblk) nblk = persistent_blocks((const void*)k_nbr_build, NB_WAVES * 64);
-  k_nbr_build<<<nblk, NB_WAVES * 64, 0, s>>>(xm, hs, n, K, L, tb.child, tb.nbmin, tb.nbmax, tb.groups, tb.ngroups,
-                                             tb.wq, nbr4, ncount, cap, stride, flags, counters, gate);
+  k_nbr_build<<<nblk, NB_WAVES * 64, 0, s>>>(xm, hs, n, K, L, tb.child, tb.range, tb.nbmin, tb.nbmax, tb.groups,
+                                             tb.ngroups, tb.wq, nbr4, ncount, cap, stride, flags, counters, gate);
 }
 
 void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, float* h, uint8_t* iters,
