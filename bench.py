@@ -191,7 +191,7 @@ def cpu_baseline(args):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
+        print(json.dumps({"error": "the step runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "design-only)"}))
         sys.exit(2)
     from botheadstudios_amd import autotune, ics
