@@ -33,6 +33,34 @@ class IhsStats(C.Structure):
     ]
 
 
+class IhsClump(C.Structure):
+    _fields_ = [
+        ("first", C.c_int64), ("n_members", C.c_int64), ("mass", C.c_double), ("com_pos", C.c_double * 3),
+        ("com_vel", C.c_double * 3), ("rho", C.c_double), ("radius", C.c_double), ("internal_ke", C.c_double),
+        ("self_pe", C.c_double), ("earth_mass", C.c_double), ("bound", C.c_int32), ("outside_roche", C.c_int32),
+    ]
+
+
+class IhsMeasure(C.Structure):
+    _fields_ = [
+        ("earth_frac", C.c_double), ("disk_kg", C.c_double), ("e_disk", C.c_double), ("t_disk", C.c_double),
+        ("e_rem", C.c_double), ("t_rem", C.c_double), ("e_esc", C.c_double), ("t_esc", C.c_double),
+        ("r_remnant", C.c_double), ("clump_kg", C.c_double), ("clump_earth_frac", C.c_double),
+        ("clump_n", C.c_int64), ("n_clumps", C.c_int64), ("n_bound", C.c_int64),
+        ("m_total", C.c_double), ("m_remnant", C.c_double), ("com", C.c_double * 3), ("v_com", C.c_double * 3),
+        ("link", C.c_double), ("n_disk", C.c_int64),
+    ]
+
+
+class IhsDiskSummary(C.Structure):
+    _fields_ = [
+        ("disk", C.c_double), ("earth_pct", C.c_double), ("remnant_km", C.c_double), ("escaped", C.c_double),
+        ("moon", C.c_double), ("e_disk", C.c_double), ("t_disk", C.c_double), ("esc", C.c_double),
+        ("r_remnant", C.c_double), ("m_remnant", C.c_double), ("biggest", C.c_double), ("link", C.c_double),
+        ("n_disk", C.c_int64), ("n_clumps", C.c_int64),
+    ]
+
+
 STAGES = ["sort", "gather", "tree", "nbr", "hsolve", "density", "forces", "gravity", "kick"]
 
 
@@ -43,7 +71,8 @@ EXPORTS = [
     "ihs_count", "ihs_sync", "ihs_stats_get", "ihs_gravity", "ihs_debug_tree", "ihs_debug_field",
     "ihs_profile", "ihs_profile_read", "ihs_mlp_set_weights", "ihs_mlp_get_weights", "ihs_mlp_record",
     "ihs_mlp_samples", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict", "ihs_direct_f64",
-    "ihs_upload_batch", "ihs_segment_clock",
+    "ihs_upload_batch", "ihs_segment_clock", "ihs_measure_impact", "ihs_disk_stats", "ihs_disk_clumps",
+    "ihs_find_clumps",
 ]
 
 MLP_NPARAM = 1313
@@ -99,11 +128,16 @@ def load(path: str | None = None):
     L.ihs_direct_f64.argtypes = [vp, vp, i64, vp]
     L.ihs_upload_batch.argtypes = [vp, vp, vp, i32, vp, i32, vp]
     L.ihs_segment_clock.argtypes = [vp, i32, vp, vp, vp]
+    L.ihs_measure_impact.argtypes = [vp, i32, i64, C.POINTER(IhsMeasure)]
+    L.ihs_disk_stats.argtypes = [vp, i32, C.POINTER(IhsDiskSummary)]
+    L.ihs_disk_clumps.argtypes = [vp, i32, i32, vp, vp, i64, vp]
+    L.ihs_find_clumps.argtypes = [vp, i64, vp, vp, vp, vp, f64, f64, f64, vp, f64, f64, vp, i64, vp, vp]
     for name in ("ihs_create", "ihs_destroy", "ihs_upload", "ihs_set_dt", "ihs_set_av", "ihs_set_omega",
                  "ihs_step_kdk", "ihs_relax", "ihs_force_eval", "ihs_sync", "ihs_stats_get", "ihs_gravity",
                  "ihs_debug_field", "ihs_profile", "ihs_profile_read", "ihs_mlp_set_weights", "ihs_mlp_get_weights",
                  "ihs_mlp_record", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict", "ihs_direct_f64",
-                 "ihs_upload_batch", "ihs_segment_clock"):
+                 "ihs_upload_batch", "ihs_segment_clock", "ihs_measure_impact", "ihs_disk_stats", "ihs_disk_clumps",
+                 "ihs_find_clumps"):
         getattr(L, name).restype = C.c_int
     if path is None:
         _lib = L

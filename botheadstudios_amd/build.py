@@ -16,7 +16,10 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libihs_sph.so")
 ARCH = os.environ.get("IHS_OFFLOAD_ARCH", "gfx950")
 
-SOURCES = ["k_lbvh.hip", "k_gravity.hip", "k_sph.hip", "k_step.hip", "k_mlp.hip", "ihs_api.hip"]
+SOURCES = ["k_lbvh.hip", "k_gravity.hip", "k_sph.hip", "k_step.hip", "k_mlp.hip", "k_measure.hip", "ihs_api.hip"]
+
+# Per-file flags: the measurement keeps the reference's f64 rounding (no fused multiply-add).
+EXTRA = {"k_measure.hip": ["-ffp-contract=off"]}
 
 # Packed FP32 issue is disabled for all device code; hipcc's host pass warns that it does not know
 # the feature, which is harmless.
@@ -36,7 +39,7 @@ def hipcc() -> str:
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, src.replace(".hip", ".o"))
-    cmd = [hipcc(), *CFLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    cmd = [hipcc(), *CFLAGS, *EXTRA.get(src, []), "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

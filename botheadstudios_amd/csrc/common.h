@@ -65,6 +65,29 @@ __device__ __forceinline__ uint32_t wave_excl_scan(uint32_t v) {
   return x - v;
 }
 
+// Exclusive scan of 256 values held one per thread (block of 256); returns this thread's prefix.
+__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_tmp /*[4]*/, uint32_t* total) {
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  uint32_t x = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) s_tmp[w] = x;
+  __syncthreads();
+  uint32_t wbase = 0, tot = 0;
+#pragma unroll
+  for (int ww = 0; ww < 4; ++ww) {
+    const uint32_t t = s_tmp[ww];
+    if (ww < w) wbase += t;
+    tot += t;
+  }
+  __syncthreads();
+  if (total) *total = tot;
+  return wbase + x - v;
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

@@ -13,6 +13,7 @@
 
 #include "../../include/ihs_sph.h"
 #include "launch.h"
+#include "measure.h"
 #include "mlp.h"
 
 namespace {
@@ -100,6 +101,7 @@ struct ihs_handle {
   double prof_ms[IHS_STAGE_COUNT] = {0};
   int64_t prof_calls[IHS_STAGE_COUNT] = {0};
   std::vector<void*> allocs;
+  MeasureBufs mb;  // disk / remnant measurement and clump buffers, allocated on first use
   std::string err;
 };
 
@@ -515,6 +517,7 @@ int ihs_destroy(ihs_handle* h) {
   if (h->stream) hipStreamSynchronize(h->stream);
   drop_graph(h);
   mlp_free(h->mlp);
+  measure_release(h->mb);
   for (hipEvent_t e : h->prof_ev) hipEventDestroy(e);
   for (void* p : h->allocs) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -1132,6 +1135,196 @@ int ihs_mlp_predict(ihs_handle* h, const float* x, int64_t n, float* y) {
   }
   hipFree(dx);
   if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "predict: %s", hipGetErrorString(e));
+  return IHS_OK;
+}
+
+// ---- Disk / remnant measurement (k_measure.hip) ----
+
+// Packs the current state into the staging records in upload order (what ihs_readback copies out) and points at
+// one scenario's particles.
+static int measure_snapshot(ihs_handle* h, int32_t segment, const SphParticle48** P, int64_t* n) {
+  if (h->n < 1) return fail(h, IHS_ERR_STATE, "measurement needs uploaded particles");
+  if (segment < 0 || segment >= h->nseg)
+    return fail(h, IHS_ERR_ARG, "segment %d outside [0, %d)", segment, h->nseg);
+  HIP_TRY(h, measure_reserve(h->mb, h->cap));
+  launch_pack(h->stream, h->split, h->n, h->st[h->cur], (SphParticle48*)h->sc.staging);
+  *P = (const SphParticle48*)h->sc.staging + h->seg_start_host[segment];
+  *n = h->seg_start_host[segment + 1] - h->seg_start_host[segment];
+  return IHS_OK;
+}
+
+int ihs_measure_impact(ihs_handle* h, int32_t segment, int64_t n_earth, ihs_measure* out) {
+  if (!h || !out) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  const SphParticle48* P = nullptr;
+  int64_t n = 0;
+  if (int rc = measure_snapshot(h, segment, &P, &n)) return rc;
+  if (n_earth < 0 || n_earth > n) return fail(h, IHS_ERR_ARG, "n_earth %lld outside [0, %lld]", (long long)n_earth,
+                                              (long long)n);
+  MeasureFrame fr;
+  HIP_TRY(h, measure_frame(h->stream, h->mb, P, n, MM_IMPACT_RUN, n_earth, &fr));
+  ihs_measure m;
+  std::memset(&m, 0, sizeof m);
+  m.e_disk = fr.cat[CAT_DISK_E];
+  m.t_disk = fr.cat[CAT_DISK_T];
+  m.disk_kg = m.e_disk + m.t_disk;
+  m.earth_frac = m.disk_kg > 0.0 ? 100.0 * m.e_disk / m.disk_kg : 0.0;
+  m.e_rem = fr.cat[CAT_REM_E];
+  m.t_rem = fr.cat[CAT_REM_T];
+  m.e_esc = fr.cat[CAT_ESC_E];
+  m.t_esc = fr.cat[CAT_ESC_T];
+  m.r_remnant = fr.r_remnant;
+  m.m_total = fr.m_total;
+  m.m_remnant = fr.m_remnant;
+  for (int q = 0; q < 3; ++q) {
+    m.com[q] = fr.com[q];
+    m.v_com[q] = fr.v_com[q];
+  }
+  m.n_disk = fr.n_disk;
+  if (fr.n_disk >= 2) {  // main.rs:803-805
+    ClumpQuery q;
+    q.impact_run = 1;
+    q.link = 2.0 * (fr.cat[CAT_DISK_H] / (double)fr.n_disk);  // main.rs:807-808
+    q.g = 6.674e-11;
+    for (int a = 0; a < 3; ++a) q.central[a] = fr.com[a];
+    q.central_mass = fr.m_remnant;
+    q.central_radius = fr.r_remnant;
+    ClumpSummary sum;
+    ihs_clump best;
+    HIP_TRY(h, clump_set(h->stream, h->mb, fr.n_disk, q, nullptr, nullptr, &sum, &best));
+    m.link = q.link;
+    m.n_clumps = sum.n2;
+    m.n_bound = sum.n_bound;
+    if (sum.best >= 0) {
+      m.clump_kg = best.mass;
+      m.clump_earth_frac = 100.0 * best.earth_mass / best.mass;
+      m.clump_n = best.n_members;
+    }
+  }
+  *out = m;
+  return IHS_OK;
+}
+
+// The engine's disk and its find_clumps query (gpu_sph.rs:399-640): softening 1e4 m, the remnant as the
+// central body, linking length twice the mean h of all particles.
+static int engine_disk(ihs_handle* h, int32_t segment, int mode, MeasureFrame* fr, ClumpQuery* q, int64_t* n_out) {
+  const SphParticle48* P = nullptr;
+  int64_t n = 0;
+  if (int rc = measure_snapshot(h, segment, &P, &n)) return rc;
+  HIP_TRY(h, measure_frame(h->stream, h->mb, P, n, mode, 0, fr));
+  q->impact_run = 0;
+  q->link = 2.0 * (fr->sum_h / (double)n);
+  q->g = 6.674e-11;
+  q->soft2 = 1.0e4 * 1.0e4;
+  for (int a = 0; a < 3; ++a) q->central[a] = fr->com[a];
+  q->central_mass = fr->m_remnant;
+  q->central_radius = fr->r_remnant;
+  *n_out = n;
+  return IHS_OK;
+}
+
+int ihs_disk_stats(ihs_handle* h, int32_t segment, ihs_disk_summary* out) {
+  if (!h || !out) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  MeasureFrame fr;
+  ClumpQuery q;
+  int64_t n = 0;
+  if (int rc = engine_disk(h, segment, MM_ENGINE_BOUND, &fr, &q, &n)) return rc;
+  constexpr double kMoon = 7.342e22;  // gpu_sph.rs:401
+  ihs_disk_summary d;
+  std::memset(&d, 0, sizeof d);
+  d.e_disk = fr.cat[CAT_DISK_E];
+  d.t_disk = fr.cat[CAT_DISK_T];
+  d.esc = fr.cat[CAT_ESC_ALL];
+  d.r_remnant = fr.r_remnant;
+  d.m_remnant = fr.m_remnant;
+  d.n_disk = fr.n_disk;
+  const double disk = d.e_disk + d.t_disk;
+  if (fr.n_disk >= 2) {  // gpu_sph.rs:456-460
+    ClumpSummary sum;
+    ihs_clump best;
+    HIP_TRY(h, clump_set(h->stream, h->mb, fr.n_disk, q, nullptr, nullptr, &sum, &best));
+    d.link = q.link;
+    d.n_clumps = sum.n2;
+    if (sum.best >= 0) d.biggest = best.mass;
+  }
+  d.disk = disk / kMoon;
+  d.earth_pct = disk > 0.0 ? 100.0 * d.e_disk / disk : 0.0;
+  d.remnant_km = fr.r_remnant / 1e3;
+  d.escaped = d.esc / kMoon;
+  d.moon = d.biggest / kMoon;
+  *out = d;
+  return IHS_OK;
+}
+
+int ihs_disk_clumps(ihs_handle* h, int32_t segment, int32_t require_bound, double* frame, ihs_clump* clumps,
+                    int64_t cap, int64_t* n_clumps) {
+  if (!h || !frame || !n_clumps || (cap > 0 && !clumps)) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  MeasureFrame fr;
+  ClumpQuery q;
+  int64_t n = 0;
+  if (int rc = engine_disk(h, segment, require_bound ? MM_ENGINE_BOUND : MM_ENGINE_ANY, &fr, &q, &n)) return rc;
+  frame[0] = fr.m_total;
+  for (int a = 0; a < 3; ++a) {
+    frame[1 + a] = fr.com[a];
+    frame[4 + a] = fr.v_com[a];
+  }
+  frame[7] = fr.r_remnant;
+  frame[8] = fr.m_remnant;
+  frame[9] = 6.674e-11 * fr.m_remnant;
+  frame[10] = fr.cat[CAT_DISK_E] + fr.cat[CAT_DISK_T];
+  frame[11] = q.link;
+  frame[12] = (double)fr.n_disk;
+  *n_clumps = 0;
+  if (n < 2 || fr.n_disk < 2) return IHS_OK;  // gpu_sph.rs:481-483, 500-502
+  std::vector<ihs_clump> all;
+  HIP_TRY(h, clump_set(h->stream, h->mb, fr.n_disk, q, &all, nullptr, nullptr, nullptr));
+  int64_t kept = 0;
+  for (const ihs_clump& c : all) {
+    if (!(c.mass > 0.0)) continue;  // accretion.rs:127-129
+    if (kept < cap) clumps[kept] = c;
+    ++kept;
+  }
+  *n_clumps = kept;
+  return IHS_OK;
+}
+
+int ihs_find_clumps(ihs_handle* h, int64_t n, const double* pos3, const double* vel3, const double* mass,
+                    const double* rho, double linking_length, double g, double softening, const double* central3,
+                    double central_mass, double central_radius, ihs_clump* clumps, int64_t cap, int64_t* n_clumps,
+                    int64_t* label) {
+  if (!h || !n_clumps || n < 0 || (n > 0 && (!pos3 || !vel3 || !mass || !rho)) || !central3 || (cap > 0 && !clumps))
+    return IHS_ERR_ARG;
+  if (!(linking_length >= 0.0) || !std::isfinite(linking_length))
+    return fail(h, IHS_ERR_ARG, "linking length must be finite and >= 0");
+  if (n > (int64_t)0x7fffffff) return fail(h, IHS_ERR_CAPACITY, "find_clumps takes at most 2^31 - 1 bodies");
+  *n_clumps = 0;
+  if (n == 0) return IHS_OK;
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, measure_reserve(h->mb, std::max<int64_t>(n, h->mb.cap)));
+  HIP_TRY(h, clump_upload(h->stream, h->mb, n, pos3, vel3, mass, rho));
+  ClumpQuery q;
+  q.link = linking_length;
+  q.g = g;
+  q.soft2 = softening * softening;
+  for (int a = 0; a < 3; ++a) q.central[a] = central3[a];
+  q.central_mass = central_mass;
+  q.central_radius = central_radius;
+  std::vector<ihs_clump> all;
+  std::vector<int32_t> lab;
+  HIP_TRY(h, clump_set(h->stream, h->mb, n, q, &all, label ? &lab : nullptr, nullptr, nullptr));
+  std::vector<int64_t> remap(all.size(), -1);
+  int64_t kept = 0;
+  for (size_t c = 0; c < all.size(); ++c) {
+    if (!(all[c].mass > 0.0)) continue;  // accretion.rs:127-129
+    if (kept < cap) clumps[kept] = all[c];
+    remap[c] = kept++;
+  }
+  *n_clumps = kept;
+  if (label)
+    for (int64_t i = 0; i < n; ++i) label[i] = remap[(size_t)lab[i]];
   return IHS_OK;
 }
 

@@ -129,29 +129,6 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_hist(const uint32_t* __
   hist[(int64_t)threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-// Exclusive scan of 256 values held one per thread (block of 256); returns this thread's prefix.
-__device__ __forceinline__ uint32_t block_excl_scan_256(uint32_t v, uint32_t* s_tmp /*[4]*/, uint32_t* total) {
-  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  uint32_t x = v;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const uint32_t y = __shfl_up(x, o, 64);
-    if (lane >= o) x += y;
-  }
-  if (lane == 63) s_tmp[w] = x;
-  __syncthreads();
-  uint32_t wbase = 0, tot = 0;
-#pragma unroll
-  for (int ww = 0; ww < 4; ++ww) {
-    const uint32_t t = s_tmp[ww];
-    if (ww < w) wbase += t;
-    tot += t;
-  }
-  __syncthreads();
-  if (total) *total = tot;
-  return wbase + x - v;
-}
-
 // One block per digit: exclusive scan over the nb tile counts of that digit; dtot[d] = digit total.
 __global__ __launch_bounds__(256) void k_radix_scan(uint32_t* __restrict__ hist, int nb, uint32_t* __restrict__ dtot) {
   __shared__ uint32_t s_tmp[4];
@@ -684,6 +661,19 @@ void launch_radix_sort(hipStream_t s, int64_t n, int bits, TreeBufs& tb) {
   }
   tb.skeys = tb.keys[cur];
   tb.perm = tb.vals[cur];
+}
+
+int launch_radix_pairs(hipStream_t s, int64_t n, int bits, uint32_t* keys[2], uint32_t* vals[2], int cur,
+                       uint32_t* hist, uint32_t* dtot) {
+  const int nb = radix_tiles(n);
+  for (int shift = 0; shift < bits; shift += 8) {
+    k_radix_hist<<<nb, RADIX_THREADS, 0, s>>>(keys[cur], n, shift, hist, nb);
+    k_radix_scan<<<256, 256, 0, s>>>(hist, nb, dtot);
+    k_radix_scatter<<<nb, RADIX_THREADS, 0, s>>>(keys[cur], vals[cur], keys[cur ^ 1], vals[cur ^ 1], n, shift, hist,
+                                                 dtot, nb);
+    cur ^= 1;
+  }
+  return cur;
 }
 
 void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb) {

@@ -122,6 +122,10 @@ void launch_bbox_morton(hipStream_t s, int split, const void* px, const void* py
 void launch_segment_order(hipStream_t s, int64_t n, TreeBufs& tb, const SegView& sv);
 // Stable LSD radix sort of (keys[0], vals[0]) over `bits` low bits; sets tb.skeys / tb.perm.
 void launch_radix_sort(hipStream_t s, int64_t n, int bits, TreeBufs& tb);
+// The same stable sort over caller buffers: (keys[cur], vals[cur]) sorted on `bits` low key bits, ping-ponging
+// through keys/vals[cur ^ 1]; returns the index of the buffers holding the result. hist holds 256 * radix_tiles(n).
+int launch_radix_pairs(hipStream_t s, int64_t n, int bits, uint32_t* keys[2], uint32_t* vals[2], int cur,
+                       uint32_t* hist, uint32_t* dtot);
 // Reorders runs of equal codes in tb.perm by ids[perm] (so the order is (code, id)).
 void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb);
 // Leaf cluster codes + Karras topology + counters reset.

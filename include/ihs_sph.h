@@ -217,6 +217,74 @@ int ihs_mlp_gradient(ihs_handle* h, const float* x, const float* y, int64_t n, f
 /* Network output for n host feature rows (x: n*6) into y (n). */
 int ihs_mlp_predict(ihs_handle* h, const float* x, int64_t n, float* y);
 
+/* ---- Disk / remnant measurement and accretion clumps, on the device-resident state ----
+ * All of these read the particles of one scenario (segment, 0 for a single upload) as the readback would hand
+ * them over (FP32 fields, upload order) without copying them to the host. Sums are double-double (about 104
+ * bits) in a fixed order, so a result does not depend on the storage order of the particles. */
+
+/* One friends-of-friends clump (accretion.rs:29-41 Clump). `first` is the clump's lowest member index (members
+ * are reported by label arrays, not lists); clumps are ordered by `first`. */
+typedef struct {
+  int64_t first;
+  int64_t n_members;
+  double mass;
+  double com_pos[3];
+  double com_vel[3];
+  double rho;          /* mass / sum(m_i / rho_i) */
+  double radius;       /* (3 mass / (4 pi rho))^(1/3) */
+  double internal_ke;  /* sum 1/2 m_i |v_i - v_com|^2 */
+  double self_pe;      /* -sum_{i<j} G m_i m_j / r_ij */
+  double earth_mass;   /* members from the target body (index < n_earth, or prov 0) */
+  int32_t bound;       /* internal_ke + self_pe < 0 */
+  int32_t outside_roche;
+} ihs_clump;
+
+/* impact-run `Measure` (ie/tools/impact-run/src/main.rs:715-731) plus the intermediate values it is built from. */
+typedef struct {
+  double earth_frac; /* % of the orbiting disk from the target */
+  double disk_kg, e_disk, t_disk, e_rem, t_rem, e_esc, t_esc, r_remnant;
+  double clump_kg, clump_earth_frac;
+  int64_t clump_n, n_clumps, n_bound;
+  double m_total, m_remnant, com[3], v_com[3], link;
+  int64_t n_disk;
+} ihs_measure;
+
+/* impact-run `measure(body, n_earth)` (main.rs:732-778) with `moon_candidate` (main.rs:802-878): remnant = the
+ * smallest radius about the centre of mass enclosing 85% of the mass; a particle escapes when its orbit about the
+ * remnant is unbound, is disk when its perigee lies above the remnant radius, remnant otherwise; target particles
+ * are those with index < n_earth. The Moon candidate is the most massive friends-of-friends clump of the disk
+ * (linking length twice the disk's mean h) that is self-bound and outside the fluid Roche limit; ties in mass go
+ * to the clump with the lowest member index (the reference leaves them to hash-map order). Synchronous. */
+int ihs_measure_impact(ihs_handle* h, int32_t segment, int64_t n_earth, ihs_measure* out);
+
+/* gpu_sph.rs `disk_stats_json` (gpu_sph.rs:399-469) as numbers: disk (M_moon), earth_pct, remnant_km,
+ * escaped (M_moon), moon (M_moon) are the JSON fields; the rest are the same quantities in SI units. */
+typedef struct {
+  double disk, earth_pct, remnant_km, escaped, moon;
+  double e_disk, t_disk, esc, r_remnant, m_remnant, biggest, link;
+  int64_t n_disk, n_clumps;
+} ihs_disk_summary;
+int ihs_disk_stats(ihs_handle* h, int32_t segment, ihs_disk_summary* out);
+
+/* The orbiting disk and its clumps as gpu_sph.rs moonlet_bodies / largest_moonlet_orbit / disk_moonlets select
+ * them (gpu_sph.rs:478-640): disk = perigee above the remnant radius (require_bound = 0: hyperbolic passes
+ * included, as those functions do; 1: bound orbits only, as disk_stats_json does); clumps by
+ * accretion::find_clumps (linking length twice the mean h of all particles, G, softening 1e4 m, the remnant as
+ * the central body). frame receives m_total, com[3], v_com[3], r_remnant, m_remnant, mu, disk_mass, link, n_disk
+ * (13 doubles). Writes up to cap clumps; *n_clumps receives the total. */
+int ihs_disk_clumps(ihs_handle* h, int32_t segment, int32_t require_bound, double* frame, ihs_clump* clumps,
+                    int64_t cap, int64_t* n_clumps);
+
+/* accretion::find_clumps (accretion.rs:73-155) on n host bodies (pos3, vel3: n*3 doubles; mass, rho: n):
+ * friends-of-friends within linking_length, then each clump's mass, centre, density, radius, internal kinetic
+ * and softened self-potential energy, boundness and the Roche gate against the central body. Clumps of zero
+ * mass are dropped. Writes up to cap clumps (ordered by lowest member), *n_clumps receives the total, and
+ * label (may be NULL; n entries) receives each body's clump index (-1 for a dropped clump). Synchronous. */
+int ihs_find_clumps(ihs_handle* h, int64_t n, const double* pos3, const double* vel3, const double* mass,
+                    const double* rho, double linking_length, double g, double softening, const double* central3,
+                    double central_mass, double central_radius, ihs_clump* clumps, int64_t cap, int64_t* n_clumps,
+                    int64_t* label);
+
 #ifdef __cplusplus
 }
 #endif
