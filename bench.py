@@ -51,6 +51,10 @@ def parse():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-sweep", action="store_true", help="skip the batched 64-scenario sweep line")
     ap.add_argument("--sweep-steps", type=int, default=10)
+    ap.add_argument("--no-measure", action="store_true", help="skip the disk / remnant measurement line")
+    ap.add_argument("--measure-only", action="store_true",
+                    help="run only the disk / remnant measurement leg (prints its object as the JSON line)")
+    ap.add_argument("--measure-reps", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="first copy MIB MiB device to device (hipMemcpy): the known-size kernel that calibrates "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE readings")
@@ -188,8 +192,59 @@ def cpu_baseline(args):
                        f"({model or platform.processor()}, nproc={os.cpu_count()})")
 
 
+def measure_leg(args):
+    """Disk / remnant measurement after an impact: impact-run `measure` with its Moon candidate
+    (ie/tools/impact-run/src/main.rs:732-878) and gpu_sph.rs `disk_stats_json` (gpu_sph.rs:399-469), on a synthetic
+    post-impact snapshot of 1,000,000 particles resident in HBM (a remnant of 900,000, a 60,000-body disk, a cold
+    3,000-body moonlet, a hot 2,000-body clump, 35,000 escaping), against the oracle's restatement of the
+    reference's CPU code on the same snapshot."""
+    from botheadstudios_amd import ics
+    from botheadstudios_amd import measure as M
+    from botheadstudios_amd.gpu_sph import GpuSph
+    from botheadstudios_amd.materials import eos_table
+
+    m = 1.0e20 * 30000 / 900000
+    p, nt, cls = ics.disk_scene(n_ball=900000, n_disk=60000, n_blob=3000, n_hot=2000, n_esc=35000, seed=9,
+                                h=1.2e5, m=m)
+    out = {"workload": f"synthetic post-impact snapshot, N={len(p)}, {int((cls == 1).sum())} disk bodies"}
+    with GpuSph(len(p), leaf_size=16, h_mode="fixed") as s:
+        s.upload(p, eos_table(["granite", "iron"]), 1.0e4)
+        for _ in range(2):
+            g = M.measure(s, nt)
+            d = M.disk_stats(s)
+        s.sync()
+        t0 = time.perf_counter()
+        for _ in range(args.measure_reps):
+            g = M.measure(s, nt)
+        t1 = time.perf_counter()
+        for _ in range(args.measure_reps):
+            d = M.disk_stats(s)
+        t2 = time.perf_counter()
+    out["ms_measure"] = (t1 - t0) * 1e3 / args.measure_reps
+    out["ms_disk_stats"] = (t2 - t1) * 1e3 / args.measure_reps
+    out["result"] = {"n_disk": g.n_disk, "disk_kg": g.disk_kg, "earth_frac": g.earth_frac, "r_remnant": g.r_remnant,
+                     "n_clumps": g.n_clumps, "n_bound": g.n_bound, "clump_n": g.clump_n, "clump_kg": g.clump_kg,
+                     "disk_stats_moon": d.moon}
+    if not args.no_cpu_baseline:
+        from oracle import measure_oracle as mo
+        t0 = time.perf_counter()
+        o = mo.measure_ir(p, nt)
+        sec = time.perf_counter() - t0
+        out["matches_oracle"] = all(getattr(g, k) == o[k] for k in ("n_disk", "n_clumps", "n_bound", "clump_n")) and \
+            all(abs(getattr(g, k) - o[k]) <= 1e-12 * abs(o[k]) for k in ("disk_kg", "r_remnant", "clump_kg"))
+        out["cpu_baseline"] = dict(value=sec * 1e3, unit="ms per measure", cores=1, kind="port",
+                                   sample="oracle restatement of impact-run measure + moon_candidate (sorted-Kahan "
+                                          "sums, O(k^2) friends-of-friends) on the same snapshot, 1 thread")
+        out["speedup_vs_cpu"] = sec * 1e3 / out["ms_measure"]
+    return out
+
+
 def main():
     args = parse()
+    if args.measure_only:
+        log("disk / remnant measurement ...")
+        print(json.dumps({"measure": measure_leg(args)}), flush=True)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "the step runs on one GPU; --gpus must be 1 (multi-GPU sharding is "
                                    "design-only)"}))
@@ -260,6 +315,12 @@ def main():
             line["sweep"] = sweep_mod.measure(64, args.sweep_steps, 2, log=log)
         except Exception as e:  # the headline line must still print
             line["sweep"] = {"error": repr(e)}
+    if not args.no_measure:
+        log("disk / remnant measurement ...")
+        try:
+            line["measure"] = measure_leg(args)
+        except Exception as e:  # the headline line must still print
+            line["measure"] = {"error": repr(e)}
     line["cpu_baseline"] = None if args.no_cpu_baseline else cpu_baseline(args)
     print(json.dumps(line), flush=True)
 

@@ -273,25 +273,55 @@ __global__ __launch_bounds__(256) void k_remnant(const SphParticle48* __restrict
   __shared__ double s_hi[256], s_lo[256];
   __shared__ int64_t s_best;
   const double thr = 0.85 * fr->m_total;
-  if (threadIdx.x == 0) {
-    dd c{0.0, 0.0};
-    int tile = -1;
-    dd base{0.0, 0.0};
-    for (int t = 0; t < ntile; ++t) {
-      const dd nx = dd_add(c, tsum[t]);
+  // Inclusive prefix over threads of the tile sums each thread owns, Hillis-Steele in LDS.
+  auto block_inclusive = [&](dd v) -> dd {
+    s_hi[threadIdx.x] = v.hi;
+    s_lo[threadIdx.x] = v.lo;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      dd w{s_hi[threadIdx.x], s_lo[threadIdx.x]};
+      if ((int)threadIdx.x >= o) w = dd_add(dd{s_hi[threadIdx.x - o], s_lo[threadIdx.x - o]}, w);
+      __syncthreads();
+      s_hi[threadIdx.x] = w.hi;
+      s_lo[threadIdx.x] = w.lo;
+      __syncthreads();
+    }
+    dd ex{0.0, 0.0};
+    if (threadIdx.x > 0) ex = dd{s_hi[threadIdx.x - 1], s_lo[threadIdx.x - 1]};
+    __syncthreads();
+    return ex;  // exclusive prefix of this thread
+  };
+  {
+    if (threadIdx.x == 0) {
+      s_tile = INT32_MAX;
+      s_best = INT64_MAX;
+    }
+    const int per = (ntile + 255) / 256;
+    const int q0 = min(ntile, (int)threadIdx.x * per), q1 = min(ntile, q0 + per);
+    dd own{0.0, 0.0};
+    for (int q = q0; q < q1; ++q) own = dd_add(own, tsum[q]);
+    dd c = block_inclusive(own);
+    int found = INT32_MAX;
+    dd fbase{0.0, 0.0};
+    for (int q = q0; q < q1; ++q) {
+      const dd nx = dd_add(c, tsum[q]);
       if (nx.hi >= thr) {
-        tile = t;
-        base = c;
+        found = q;
+        fbase = c;
         break;
       }
       c = nx;
     }
-    s_tile = tile;
-    s_base_hi = base.hi;
-    s_base_lo = base.lo;
-    s_best = INT64_MAX;
+    if (found != INT32_MAX) atomicMin(&s_tile, found);
+    __syncthreads();
+    if (found != INT32_MAX && found == s_tile) {
+      s_base_hi = fbase.hi;
+      s_base_lo = fbase.lo;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_tile == INT32_MAX) s_tile = -1;
+    __syncthreads();
   }
-  __syncthreads();
   const int tile = s_tile;
   if (tile < 0) {
     if (threadIdx.x == 0) {
@@ -300,24 +330,13 @@ __global__ __launch_bounds__(256) void k_remnant(const SphParticle48* __restrict
     }
     return;
   }
-  // Inclusive prefix of the per-thread sums (Hillis-Steele over 256 threads), then each thread walks its 8.
+  // Inside the tile: the same prefix over each thread's 8 sorted masses, then each thread walks its 8.
   const int64_t base = (int64_t)tile * MB_TILE + threadIdx.x * 8;
   dd own{0.0, 0.0};
   for (int j = 0; j < 8; ++j)
     if (base + j < n) own = dd_add1(own, (double)P[perm[base + j]].mass);
-  s_hi[threadIdx.x] = own.hi;
-  s_lo[threadIdx.x] = own.lo;
-  __syncthreads();
-  for (int o = 1; o < 256; o <<= 1) {
-    dd v{s_hi[threadIdx.x], s_lo[threadIdx.x]};
-    if ((int)threadIdx.x >= o) v = dd_add(dd{s_hi[threadIdx.x - o], s_lo[threadIdx.x - o]}, v);
-    __syncthreads();
-    s_hi[threadIdx.x] = v.hi;
-    s_lo[threadIdx.x] = v.lo;
-    __syncthreads();
-  }
-  dd c{s_base_hi, s_base_lo};
-  if (threadIdx.x > 0) c = dd_add(c, dd{s_hi[threadIdx.x - 1], s_lo[threadIdx.x - 1]});
+  const dd ex = block_inclusive(own);
+  dd c = dd_add(dd{s_base_hi, s_base_lo}, ex);
   int64_t found = INT64_MAX;
   double mfound = 0.0;
   for (int j = 0; j < 8 && base + j < n; ++j) {
@@ -494,15 +513,25 @@ __global__ __launch_bounds__(256) void k_bbox64(const double* __restrict__ x, co
       hi[q] = max(hi[q], c[q]);
     }
   }
+  __shared__ uint64_t s_b[4][6];
+  const int w = threadIdx.x >> 6;
   for (int q = 0; q < 3; ++q) {
     lo[q] = wave_min(lo[q]);
     hi[q] = wave_max(hi[q]);
   }
   if ((threadIdx.x & 63) == 0)
     for (int q = 0; q < 3; ++q) {
-      atomicMin((unsigned long long*)&bb[q], (unsigned long long)lo[q]);
-      atomicMax((unsigned long long*)&bb[3 + q], (unsigned long long)hi[q]);
+      s_b[w][q] = lo[q];
+      s_b[w][3 + q] = hi[q];
     }
+  __syncthreads();
+  if (threadIdx.x < 6) {
+    const int q = threadIdx.x;
+    uint64_t v = s_b[0][q];
+    for (int ww = 1; ww < 4; ++ww) v = q < 3 ? min(v, s_b[ww][q]) : max(v, s_b[ww][q]);
+    if (q < 3) atomicMin((unsigned long long*)&bb[q], (unsigned long long)v);
+    else atomicMax((unsigned long long*)&bb[q], (unsigned long long)v);
+  }
 }
 
 struct Grid {
@@ -584,45 +613,97 @@ __device__ __forceinline__ int64_t lower_bound64(const uint64_t* __restrict__ s,
   return lo;
 }
 
-// One thread per sorted body: every body of the 3x3x3 neighbour cells with a higher index (the reference's
-// j > i, neighbors.rs:69-84 / main.rs:826-835) is linked when d2 <= link^2. FP32D: impact-run's FP32 position
-// differences widened to f64 (main.rs:829); otherwise f64 differences (accretion.rs:107).
-template <bool FP32D>
-__global__ __launch_bounds__(256) void k_fof_link(int64_t k, const uint64_t* __restrict__ skey,
-                                                  const uint32_t* __restrict__ perm, Grid g, MeasureDev B, double ll2,
-                                                  uint32_t* __restrict__ par) {
+// Positions in cell-sorted order (the clump-order arrays are free until the clumps are formed), so that lanes
+// scanning the same cells read the same addresses.
+__global__ __launch_bounds__(256) void k_sorted_pos(int64_t k, const uint32_t* __restrict__ perm, MeasureDev B) {
   const int64_t t = (int64_t)blockIdx.x * MB_THREADS + threadIdx.x;
   if (t >= k) return;
   const uint32_t j = perm[t];
+  B.bx[t] = B.ax[j];
+  B.by[t] = B.ay[j];
+  B.bz[t] = B.az[j];
+  B.bxf[t] = B.axf[j];
+  B.byf[t] = B.ayf[j];
+  B.bzf[t] = B.azf[j];
+}
+
+// One wave per body (sorted position t): calls f(u) on the lanes for every body at sorted position u in the 3x3x3
+// cells around it whose link test passes, d2 <= link^2 (neighbors.rs:69-84 / main.rs:826-835). Lanes 0-8 find
+// the nine rows of three cells (one key range each, x fastest) and the wave strides over each row, so a body
+// in a dense clump spreads its thousands of candidates over 64 lanes. FP32D: impact-run's FP32 position
+// differences widened to f64 (main.rs:829); otherwise f64 differences (accretion.rs:107). The test is symmetric
+// bit for bit (a - b = -(b - a)), so both bodies of a pair see the same answer.
+template <bool FP32D, class F>
+__device__ __forceinline__ void for_links_wave(int64_t k, const uint64_t* __restrict__ skey, int64_t t,
+                                               const Grid& g, const MeasureDev& B, double ll2, F&& f) {
+  const uint32_t lane = lane_id();
   const uint64_t key = skey[t];
   const uint32_t mx = (1u << g.bits[0]) - 1u, my = (1u << g.bits[1]) - 1u;
   const uint32_t cx = (uint32_t)(key & mx), cy = (uint32_t)((key >> g.bits[0]) & my),
                  cz = (uint32_t)(key >> (g.bits[0] + g.bits[1]));
-  const double xj = B.ax[j], yj = B.ay[j], zj = B.az[j];
-  const float xfj = B.axf[j], yfj = B.ayf[j], zfj = B.azf[j];
-  const uint32_t x0 = cx > 0 ? cx - 1 : 0, x1 = min(cx + 1, g.nc[0] - 1);
-  for (int dz = -1; dz <= 1; ++dz) {
-    const int64_t zz = (int64_t)cz + dz;
-    if (zz < 0 || zz >= g.nc[2]) continue;
-    for (int dy = -1; dy <= 1; ++dy) {
-      const int64_t yy = (int64_t)cy + dy;
-      if (yy < 0 || yy >= g.nc[1]) continue;
-      const uint64_t klo = g.key(x0, (uint32_t)yy, (uint32_t)zz), khi = g.key(x1, (uint32_t)yy, (uint32_t)zz);
-      for (int64_t u = lower_bound64(skey, k, klo); u < k && skey[u] <= khi; ++u) {
-        const uint32_t l = perm[u];
-        if (l <= j) continue;
-        double d2;
-        if (FP32D) {
-          const double dx = (double)(xfj - B.axf[l]), dy2 = (double)(yfj - B.ayf[l]), dz2 = (double)(zfj - B.azf[l]);
-          d2 = dx * dx + dy2 * dy2 + dz2 * dz2;
-        } else {
-          const double dx = xj - B.ax[l], dy2 = yj - B.ay[l], dz2 = zj - B.az[l];
-          d2 = dx * dx + dy2 * dy2 + dz2 * dz2;
-        }
-        if (d2 <= ll2) uf_unite(par, j, l);
-      }
+  int64_t rs = 0, rn = 0;
+  if (lane < 9) {
+    const int64_t zz = (int64_t)cz + (int64_t)(lane / 3) - 1, yy = (int64_t)cy + (int64_t)(lane % 3) - 1;
+    if (zz >= 0 && zz < g.nc[2] && yy >= 0 && yy < g.nc[1]) {
+      const uint32_t x0 = cx > 0 ? cx - 1 : 0, x1 = min(cx + 1, g.nc[0] - 1);
+      rs = lower_bound64(skey, k, g.key(x0, (uint32_t)yy, (uint32_t)zz));
+      rn = lower_bound64(skey, k, g.key(x1, (uint32_t)yy, (uint32_t)zz) + 1) - rs;
     }
   }
+  const double xj = B.bx[t], yj = B.by[t], zj = B.bz[t];
+  const float xfj = B.bxf[t], yfj = B.byf[t], zfj = B.bzf[t];
+  for (int r = 0; r < 9; ++r) {
+    const int64_t s0 = __shfl(rs, r, 64), n0 = __shfl(rn, r, 64);
+    for (int64_t i = lane; i < n0; i += 64) {
+      const int64_t u = s0 + i;
+      double d2;
+      if (FP32D) {
+        const double dx = (double)(xfj - B.bxf[u]), dy2 = (double)(yfj - B.byf[u]), dz2 = (double)(zfj - B.bzf[u]);
+        d2 = dx * dx + dy2 * dy2 + dz2 * dz2;
+      } else {
+        const double dx = xj - B.bx[u], dy2 = yj - B.by[u], dz2 = zj - B.bz[u];
+        d2 = dx * dx + dy2 * dy2 + dz2 * dz2;
+      }
+      if (d2 <= ll2) f(u);
+    }
+  }
+}
+
+// Pass 1 (plain stores): each body's parent is its lowest-index linked body (itself if none is lower). The
+// parents form a forest inside the components (par[j] <= j).
+template <bool FP32D>
+__global__ __launch_bounds__(256) void k_fof_minlink(int64_t k, const uint64_t* __restrict__ skey,
+                                                     const uint32_t* __restrict__ perm, Grid g, MeasureDev B,
+                                                     double ll2, uint32_t* __restrict__ par) {
+  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= k) return;
+  const uint32_t j = perm[t];
+  uint32_t m = j;
+  for_links_wave<FP32D>(k, skey, t, g, B, ll2, [&](int64_t u) { m = min(m, perm[u]); });
+  m = wave_min(m);
+  if (lane_id() == 0) par[j] = m;
+}
+
+// Pass 2: the forest root of every body (a plain array the next pass reads through the cache).
+__global__ __launch_bounds__(256) void k_fof_froot(int64_t k, uint32_t* __restrict__ par, uint32_t* __restrict__ fr) {
+  const int64_t j = (int64_t)blockIdx.x * MB_THREADS + threadIdx.x;
+  if (j < k) fr[j] = uf_find(par, (uint32_t)j);
+}
+
+// Pass 3: links between different forest trees are united (lock-free union-find on the same parents). Inside
+// a dense clump almost every link joins bodies of one tree and ends at a cached load.
+template <bool FP32D>
+__global__ __launch_bounds__(256) void k_fof_link(int64_t k, const uint64_t* __restrict__ skey,
+                                                  const uint32_t* __restrict__ perm, Grid g, MeasureDev B, double ll2,
+                                                  const uint32_t* __restrict__ fr, uint32_t* __restrict__ par) {
+  const int64_t t = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (t >= k) return;
+  const uint32_t j = perm[t];
+  const uint32_t rj = fr[j];
+  for_links_wave<FP32D>(k, skey, t, g, B, ll2, [&](int64_t u) {
+    const uint32_t l = perm[u];
+    if (l > j && fr[l] != rj) uf_unite(par, rj, fr[l]);
+  });
 }
 
 __global__ __launch_bounds__(256) void k_fof_roots(int64_t k, uint32_t* __restrict__ par, uint32_t* __restrict__ keys,
@@ -664,13 +745,15 @@ __global__ __launch_bounds__(256) void k_clump_index(int64_t k, const uint32_t* 
   B.bsrc[t] = B.asrc[j];
 }
 
-// Pair-potential work items per clump: ceil(n / 256) for clumps of >= 2 members.
+// Pair-potential work items per clump of >= 2 members: the T (T + 1) / 2 tiles (row tile <= column tile) of
+// 256 x 256 members, T = ceil(n / 256).
 __global__ __launch_bounds__(256) void k_clump_items(const uint32_t* __restrict__ cstart,
                                                      const uint32_t* __restrict__ nclump, uint32_t* __restrict__ items) {
   const int64_t c = (int64_t)blockIdx.x * MB_THREADS + threadIdx.x;
   if (c >= *nclump) return;
   const uint32_t n = cstart[c + 1] - cstart[c];
-  items[c] = n >= 2 ? (n + PE_ROWS - 1) / PE_ROWS : 0u;
+  const uint32_t T = (n + PE_ROWS - 1) / PE_ROWS;
+  items[c] = n >= 2 ? T * (T + 1) / 2 : 0u;
 }
 
 __global__ void k_items_end(const uint32_t* __restrict__ nclump, const uint32_t* __restrict__ total,
@@ -752,7 +835,14 @@ __global__ __launch_bounds__(256) void k_clump_pe(int64_t C, int64_t nitems, Mea
   const int64_t c = lo;
   const uint32_t s0 = B.cstart[c];
   const int64_t nc = (int64_t)(B.cstart[c + 1] - s0);
-  const int64_t rt = item - B.ioff[c];
+  // (row tile, column tile) of this item, rows first: row tile r holds T - r items.
+  const int64_t T = (nc + PE_ROWS - 1) / PE_ROWS;
+  int64_t idx = item - B.ioff[c], rt = 0;
+  while (idx >= T - rt) {
+    idx -= T - rt;
+    ++rt;
+  }
+  const int64_t ct = rt + idx;
   const int64_t a = rt * PE_ROWS + threadIdx.x;
   double xa = 0.0, ya = 0.0, za = 0.0, ma = 0.0;
   float xfa = 0.f, yfa = 0.f, zfa = 0.f;
@@ -766,8 +856,7 @@ __global__ __launch_bounds__(256) void k_clump_pe(int64_t C, int64_t nitems, Mea
     zfa = B.bzf[s0 + a];
   }
   dd acc{0.0, 0.0};
-  for (int64_t ct = rt; ct * PE_ROWS < nc; ++ct) {
-    __syncthreads();
+  {
     const int64_t b = ct * PE_ROWS + threadIdx.x;
     if (b < nc) {
       sx[threadIdx.x] = B.bx[s0 + b];
@@ -817,49 +906,51 @@ __global__ __launch_bounds__(256) void k_clump_final(int64_t C, MeasureDev B, do
   B.crec[c] = r;
 }
 
-// One block: counts over clumps of >= 2 members and the most massive bound clump outside Roche (main.rs:838-877,
-// Clump::accretes; the lowest first member wins a tie).
-__global__ __launch_bounds__(256) void k_clump_best(int64_t C, const ihs_clump* __restrict__ rec,
-                                                    ClumpSummary* __restrict__ out) {
-  uint32_t n2 = 0, nb = 0;
-  double bm = 0.0;
-  int64_t bi = -1;
-  for (int64_t c = threadIdx.x; c < C; c += MB_THREADS) {
-    const ihs_clump& r = rec[c];
-    if (r.n_members < 2) continue;
-    ++n2;
-    if (!r.bound) continue;
-    ++nb;
-    if (r.outside_roche && r.mass > bm) {  // c ascends, so the first of equal masses is kept
-      bm = r.mass;
-      bi = c;
-    }
-  }
-  __shared__ uint32_t s_n2[256], s_nb[256];
-  __shared__ double s_bm[256];
-  __shared__ int64_t s_bi[256];
-  s_n2[threadIdx.x] = n2;
-  s_nb[threadIdx.x] = nb;
-  s_bm[threadIdx.x] = bm;
-  s_bi[threadIdx.x] = bi;
-  __syncthreads();
+// Counts over clumps of >= 2 members and the most massive bound clump outside Roche (main.rs:838-877,
+// Clump::accretes), with integer atomics so the result does not depend on scheduling: the largest mass first
+// (positive doubles order as their bit patterns), then the lowest clump index among clumps of that mass.
+__global__ void k_best_init(ClumpSummary* out) {
   if (threadIdx.x == 0) {
-    uint32_t t2 = 0, tb = 0;
-    double m = 0.0;
-    int64_t b = -1;
-    for (int i = 0; i < MB_THREADS; ++i) {
-      t2 += s_n2[i];
-      tb += s_nb[i];
-      if (s_bi[i] >= 0 && (s_bm[i] > m || (s_bm[i] == m && b >= 0 && s_bi[i] < b))) {
-        m = s_bm[i];
-        b = s_bi[i];
-      }
-    }
-    out->n2 = t2;
-    out->n_bound = tb;
-    out->best = (int32_t)b;
+    out->n2 = 0;
+    out->n_bound = 0;
+    out->best = 0x7fffffff;
     out->pad = 0;
+    out->best_mass = 0ull;
   }
+}
+
+__device__ __forceinline__ bool accretes(const ihs_clump& r) {
+  return r.n_members >= 2 && r.bound && r.outside_roche && r.mass > 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_best_mass(int64_t C, const ihs_clump* __restrict__ rec,
+                                                   ClumpSummary* __restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * MB_THREADS + threadIdx.x;
+  uint32_t n2 = 0, nb = 0;
+  unsigned long long mb = 0ull;
+  if (c < C) {
+    const ihs_clump& r = rec[c];
+    n2 = r.n_members >= 2 ? 1u : 0u;
+    nb = (r.n_members >= 2 && r.bound) ? 1u : 0u;
+    if (accretes(r)) mb = (unsigned long long)__double_as_longlong(r.mass);
+  }
+  n2 = wave_sum(n2);
+  nb = wave_sum(nb);
+  mb = wave_max(mb);
+  if ((threadIdx.x & 63) == 0) {
+    if (n2) atomicAdd(&out->n2, n2);
+    if (nb) atomicAdd(&out->n_bound, nb);
+    if (mb) atomicMax(&out->best_mass, mb);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_best_index(int64_t C, const ihs_clump* __restrict__ rec,
+                                                    ClumpSummary* __restrict__ out) {
+  const int64_t c = (int64_t)blockIdx.x * MB_THREADS + threadIdx.x;
+  if (c >= C) return;
+  const ihs_clump& r = rec[c];
+  if (out->best_mass != 0ull && accretes(r) && (unsigned long long)__double_as_longlong(r.mass) == out->best_mass)
+    atomicMin(&out->best, (int32_t)c);
 }
 
 }  // namespace
@@ -869,6 +960,7 @@ __global__ __launch_bounds__(256) void k_clump_best(int64_t C, const ihs_clump* 
 
 void measure_release(MeasureBufs& B) {
   for (void* p : B.allocs) (void)hipFree(p);
+  if (B.pe_part) (void)hipFree(B.pe_part);
   B = MeasureBufs{};
 }
 
@@ -915,9 +1007,6 @@ hipError_t measure_reserve(MeasureBufs& B, int64_t cap) {
   get(&B.cstart, c + 1);
   get(&B.ioff, c + 1);
   get(&B.crec, c);
-  dd* pe = nullptr;
-  get(&pe, (size_t)c + c / PE_ROWS + 2);
-  B.pe_part = pe;
   if (err != hipSuccess) {
     measure_release(B);
     return err;
@@ -980,6 +1069,8 @@ int sort_bits_u32(uint64_t maxval) { return std::max(8, (bits_for(maxval + 1) + 
 hipError_t clump_set(hipStream_t s, MeasureBufs& B, int64_t k, const ClumpQuery& q, std::vector<ihs_clump>* all,
                      std::vector<int32_t>* labels, ClumpSummary* sum, ihs_clump* best) {
   hipError_t e;
+  // Work items of one clump (T (T + 1) / 2 tiles) are counted in 32 bits.
+  if (k < 1 || k > (int64_t)1 << 24) return hipErrorInvalidValue;
   // Grid: cell side >= link (1 + 2^-20), so that rounding in (x - lo) / cs can never separate a linked pair by
   // two cells, and at most 2^20 cells per axis.
   k_bbox64_init<<<1, 64, 0, s>>>(B.bbox);
@@ -1021,11 +1112,18 @@ hipError_t clump_set(hipStream_t s, MeasureBufs& B, int64_t k, const ClumpQuery&
     cur = launch_radix_pairs(s, k, tb - 32, B.rk, B.rv, cur, B.hist, B.dtot);
   }
   k_gather_u64<<<g256(k), MB_THREADS, 0, s>>>(B.rv[cur], B.key64, k, B.skey);
-  // Links and roots.
-  if (q.impact_run)
-    k_fof_link<true><<<g256(k), MB_THREADS, 0, s>>>(k, B.skey, B.rv[cur], g, B, ll2, B.parent);
-  else
-    k_fof_link<false><<<g256(k), MB_THREADS, 0, s>>>(k, B.skey, B.rv[cur], g, B, ll2, B.parent);
+  k_sorted_pos<<<g256(k), MB_THREADS, 0, s>>>(k, B.rv[cur], B);
+  // Links and roots: one wave per body.
+  const unsigned wgrid = (unsigned)((k + 3) / 4);
+  if (q.impact_run) {
+    k_fof_minlink<true><<<wgrid, MB_THREADS, 0, s>>>(k, B.skey, B.rv[cur], g, B, ll2, B.parent);
+    k_fof_froot<<<g256(k), MB_THREADS, 0, s>>>(k, B.parent, B.label);
+    k_fof_link<true><<<wgrid, MB_THREADS, 0, s>>>(k, B.skey, B.rv[cur], g, B, ll2, B.label, B.parent);
+  } else {
+    k_fof_minlink<false><<<wgrid, MB_THREADS, 0, s>>>(k, B.skey, B.rv[cur], g, B, ll2, B.parent);
+    k_fof_froot<<<g256(k), MB_THREADS, 0, s>>>(k, B.parent, B.label);
+    k_fof_link<false><<<wgrid, MB_THREADS, 0, s>>>(k, B.skey, B.rv[cur], g, B, ll2, B.label, B.parent);
+  }
   k_fof_roots<<<g256(k), MB_THREADS, 0, s>>>(k, B.parent, B.rk[0], B.rv[0]);
   cur = launch_radix_pairs(s, k, sort_bits_u32((uint64_t)(k - 1)), B.rk, B.rv, 0, B.hist, B.dtot);
   // Clump segments.
@@ -1042,6 +1140,14 @@ hipError_t clump_set(hipStream_t s, MeasureBufs& B, int64_t k, const ClumpQuery&
   uint32_t T = 0;
   if ((e = hipMemcpyAsync(&T, &B.frame->n_items, sizeof T, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  if ((int64_t)T > B.pe_cap) {
+    if (B.pe_part) (void)hipFree(B.pe_part);
+    B.pe_part = nullptr;
+    B.pe_cap = 0;
+    const int64_t want = std::max<int64_t>((int64_t)T, 2 * B.cap);
+    if ((e = hipMalloc(&B.pe_part, (size_t)want * sizeof(dd))) != hipSuccess) return e;
+    B.pe_cap = want;
+  }
   if (q.impact_run) k_clump_sums<true><<<(unsigned)((C + 3) / 4), MB_THREADS, 0, s>>>(C, B);
   else k_clump_sums<false><<<(unsigned)((C + 3) / 4), MB_THREADS, 0, s>>>(C, B);
   if (T > 0) {
@@ -1051,7 +1157,9 @@ hipError_t clump_set(hipStream_t s, MeasureBufs& B, int64_t k, const ClumpQuery&
   const double rho_c = q.central_mass / (kFourThirdsPi * (q.central_radius * q.central_radius * q.central_radius));
   k_clump_final<<<g256(C), MB_THREADS, 0, s>>>(C, B, rho_c, q.central_radius, q.central[0], q.central[1],
                                                 q.central[2]);
-  k_clump_best<<<1, MB_THREADS, 0, s>>>(C, B.crec, B.summary);
+  k_best_init<<<1, 64, 0, s>>>(B.summary);
+  k_best_mass<<<g256(C), MB_THREADS, 0, s>>>(C, B.crec, B.summary);
+  k_best_index<<<g256(C), MB_THREADS, 0, s>>>(C, B.crec, B.summary);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   ClumpSummary cs_h{};
   if ((e = hipMemcpyAsync(&cs_h, B.summary, sizeof cs_h, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
@@ -1067,6 +1175,7 @@ hipError_t clump_set(hipStream_t s, MeasureBufs& B, int64_t k, const ClumpQuery&
       return e;
   }
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  if (cs_h.best == 0x7fffffff) cs_h.best = -1;
   if (best) {
     *best = ihs_clump{};
     if (cs_h.best >= 0) {

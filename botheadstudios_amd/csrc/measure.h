@@ -46,10 +46,11 @@ struct ClumpQuery {
 
 // Phase 2 summary over clumps of at least two members.
 struct ClumpSummary {
-  uint32_t n2;       // clumps with >= 2 members
-  uint32_t n_bound;  // ... of which self-bound
-  int32_t best;      // most massive bound clump outside Roche (lowest first member on ties), -1 if none
+  uint32_t n2;              // clumps with >= 2 members
+  uint32_t n_bound;         // ... of which self-bound
+  int32_t best;             // most massive bound clump outside Roche (lowest first member on ties), -1 if none
   uint32_t pad;
+  unsigned long long best_mass;  // bit pattern of that clump's mass (positive doubles order as integers)
 };
 
 // Device pointers (passed to kernels by value).
@@ -91,6 +92,7 @@ struct MeasureDev {
 struct MeasureBufs : MeasureDev {
   int64_t cap = 0;
   std::vector<void*> allocs;
+  int64_t pe_cap = 0;  // pair-potential items pe_part holds (grown on demand)
 };
 
 hipError_t measure_reserve(MeasureBufs& B, int64_t cap);

@@ -195,3 +195,70 @@ def sweep_scenarios(count: int = 64, seed: int = 0x1A7E0005, n_lo: int = 4096, n
     return [dict(n=int(n[i]), m_target=0.9, m_impactor=float(0.05 + 0.15 * u[1][i]),
                  angle_deg=float(60.0 * u[2][i]), v_over_vesc=float(1.0 + 0.5 * u[3][i]),
                  seed=int(seed * 1000 + i)) for i in range(count)]
+
+# ---------------------------------------------------------------------------------------------
+# Post-impact scenes for the disk / remnant measurement.
+
+
+def cold_blob(center, bulk, radius, n, m_i, rho):
+    """The accretion tests' blob (accretion.rs:199-213): n bodies on a Fibonacci spiral inside `radius`, all moving
+    at `bulk`."""
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    i = np.arange(n, dtype=np.float64)
+    rr = radius * np.cbrt((i + 0.5) / n)
+    y = 1.0 - 2.0 * (i + 0.5) / n
+    rad = np.sqrt(np.maximum(1.0 - y * y, 0.0))
+    th = golden * i
+    p = np.asarray(center, np.float64) + np.stack([np.cos(th) * rad * rr, y * rr, np.sin(th) * rad * rr], axis=1)
+    v = np.tile(np.asarray(bulk, np.float64), (n, 1))
+    return p, v, np.full(n, m_i), np.full(n, rho)
+
+
+def disk_scene(n_ball: int = 30000, n_disk: int = 1500, n_esc: int = 200, n_blob: int = 300, n_hot: int = 200,
+               seed: int = 1, m: float = 1.0e20, R: float = 6.4e6, h: float = 3.2e5):
+    """A synthetic post-impact snapshot with known orbit classes (the measurement's test and benchmark workload), as
+    48-byte records (target = index < n_target, prov 0):
+       - a remnant ball of n_ball bodies (uniform in radius R, near rest): remnant;
+       - n_disk bodies on circular orbits at 3R..6R in a thin annulus: disk;
+       - a cold blob of n_blob bodies at 4.5R on a circular orbit: disk, one bound clump outside Roche;
+       - a hot blob of n_hot bodies at 5.5R (internal speeds +-1.5 km/s normal to the orbit): disk, one unbound
+         clump;
+       - n_esc bodies moving radially outward at 3 escape speeds from 2R..4R: escaped.
+    The ball and the first half of the annulus are target material. Returns (records, n_target, classes)."""
+    rng = np.random.default_rng(seed)
+    M = n_ball * m
+    mu = G * M
+
+    def unit(k):
+        v = rng.normal(size=(k, 3))
+        return v / np.linalg.norm(v, axis=1, keepdims=True)
+
+    ball = unit(n_ball) * (R * np.cbrt(rng.random(n_ball)))[:, None]
+    vball = rng.normal(scale=5.0, size=(n_ball, 3))
+    r = R * (3.0 + 3.0 * rng.random(n_disk))
+    ph = 2 * math.pi * rng.random(n_disk)
+    disk = np.stack([r * np.cos(ph), r * np.sin(ph), 0.02 * R * rng.normal(size=n_disk)], axis=1)
+    vc = np.sqrt(mu / r)
+    vdisk = np.stack([-vc * np.sin(ph), vc * np.cos(ph), np.zeros(n_disk)], axis=1)
+    bp, bv, _, _ = cold_blob((4.5 * R, 0.0, 0.0), (0.0, math.sqrt(mu / (4.5 * R)), 0.0), 2.0e5, n_blob, m, 3000.0)
+    hp, hv, _, _ = cold_blob((0.0, -5.5 * R, 0.0), (math.sqrt(mu / (5.5 * R)), 0.0, 0.0), 1.0e6, n_hot, m, 3000.0)
+    hv[:, 2] = np.where(np.arange(n_hot) % 2 == 0, 1500.0, -1500.0)  # out of the plane: still a disk orbit
+    re = R * (2.0 + 2.0 * rng.random(n_esc))
+    ue = unit(n_esc)
+    esc = ue * re[:, None]
+    vesc = ue * (3.0 * np.sqrt(2.0 * mu / re))[:, None]
+    pos = np.concatenate([ball, disk[: n_disk // 2], disk[n_disk // 2:], bp, hp, esc])
+    vel = np.concatenate([vball, vdisk[: n_disk // 2], vdisk[n_disk // 2:], bv, hv, vesc])
+    cls = np.concatenate([np.zeros(n_ball, np.int32), np.ones(n_disk + n_blob + n_hot, np.int32),
+                          np.full(n_esc, 2, np.int32)])
+    n = pos.shape[0]
+    n_target = n_ball + n_disk // 2
+    out = np.zeros(n, SPH_PARTICLE_DTYPE)
+    out["pos"] = pos.astype(np.float32)
+    out["vel"] = vel.astype(np.float32)
+    out["h"] = np.float32(h)
+    out["u"] = np.float32(1.0e5)
+    out["mass"] = np.float32(m)
+    out["rho"] = np.float32(3000.0)
+    out["prov"] = (np.arange(n) >= n_target).astype(np.uint32)
+    return out, n_target, cls
