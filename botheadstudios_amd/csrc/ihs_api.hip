@@ -1328,4 +1328,94 @@ int ihs_find_clumps(ihs_handle* h, int64_t n, const double* pos3, const double* 
   return IHS_OK;
 }
 
+// ---- IC assembly on the device (gpu_sph.rs:279-339) ----
+
+int ihs_body_bulk(ihs_handle* h, int32_t segment, uint32_t prov, double* out5) {
+  if (!h || !out5) return IHS_ERR_ARG;
+  hipSetDevice(h->device);
+  const SphParticle48* P = nullptr;
+  int64_t n = 0;
+  if (int rc = measure_snapshot(h, segment, &P, &n)) return rc;
+  HIP_TRY(h, body_bulk(h->stream, h->mb, P, n, prov, out5));
+  return IHS_OK;
+}
+
+int ihs_assemble(ihs_handle* h, int32_t segment, const double* placements, int32_t nbody, float* softening,
+                 float* dt) {
+  if (!h || !placements || !softening || !dt) return IHS_ERR_ARG;
+  if (nbody < 1 || nbody > IHS_MAX_BODIES) return fail(h, IHS_ERR_ARG, "need 1..%d placements, got %d", IHS_MAX_BODIES,
+                                                          nbody);
+  for (int k = 0; k < 9 * nbody; ++k)
+    if (!std::isfinite(placements[k])) return fail(h, IHS_ERR_ARG, "placement %d is not finite", k / 9);
+  hipSetDevice(h->device);
+  const SphParticle48* P = nullptr;
+  int64_t n = 0;
+  if (int rc = measure_snapshot(h, segment, &P, &n)) return rc;
+  const int64_t t0 = h->seg_start_host[segment], t1 = h->seg_start_host[segment + 1];
+  MasterBufs& S = h->st[h->cur];
+  uint32_t* dflags = nullptr;
+  double* dpl = nullptr;
+  HIP_TRY(h, hipMalloc(&dflags, 2 * sizeof(uint32_t)));
+  hipError_t e = hipMalloc(&dpl, (size_t)12 * nbody * sizeof(double));
+  auto done = [&](hipError_t err) {
+    hipFree(dflags);
+    if (dpl) hipFree(dpl);
+    return err;
+  };
+  // Every particle must have a placement: the reference drops the others, which would change the count.
+  uint32_t init[2] = {0u, 0xffffffffu}, got[2] = {0u, 0u};
+  if (e == hipSuccess) e = hipMemcpyAsync(dflags, init, sizeof init, hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) launch_prov_max(h->stream, t0, t1, S, dflags);
+  if (e == hipSuccess) e = hipMemcpyAsync(got, dflags, sizeof got, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "assemble: %s", hipGetErrorString(done(e)));
+  if (got[0] >= (uint32_t)nbody) {
+    done(hipSuccess);
+    return fail(h, IHS_ERR_ARG, "particles of provenance %u have no placement (%d given); the in-place assembly keeps "
+                "every particle", got[0], nbody);
+  }
+  // Each body's robust centre (gpu_sph.rs:326).
+  std::vector<double> pl((size_t)12 * nbody);
+  double max_speed = 0.0;
+  for (int k = 0; k < nbody; ++k) {
+    double bulk[5];
+    e = body_bulk(h->stream, h->mb, P, n, (uint32_t)k, bulk);
+    if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "body_bulk: %s", hipGetErrorString(done(e)));
+    const double* q = placements + 9 * k;
+    for (int a = 0; a < 3; ++a) {
+      pl[12 * k + a] = bulk[a];
+      pl[12 * k + 3 + a] = q[a];
+      pl[12 * k + 6 + a] = q[3 + a];
+      pl[12 * k + 9 + a] = q[6 + a];
+    }
+    max_speed = std::max(max_speed, std::sqrt(q[3] * q[3] + q[4] * q[4] + q[5] * q[5]));  // gpu_sph.rs:337
+  }
+  e = hipMemcpyAsync(dpl, pl.data(), pl.size() * sizeof(double), hipMemcpyHostToDevice, h->stream);
+  if (e == hipSuccess) launch_assemble(h->stream, h->split, t0, t1, S, dpl, dflags + 1);
+  if (e == hipSuccess) e = hipMemcpyAsync(got, dflags, sizeof got, hipMemcpyDeviceToHost, h->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  done(e);
+  if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "assemble: %s", hipGetErrorString(e));
+  // softening = min h / 4; the shock-safe dt against the fastest approach (gpu_sph.rs:333-338).
+  uint32_t k = got[1];
+  const float min_h = (k >> 31) ? __builtin_bit_cast(float, k ^ 0x80000000u) : __builtin_bit_cast(float, ~k);
+  *softening = 0.25f * min_h;
+  *dt = (float)(0.01 * (double)min_h / (20000.0 + max_speed));
+  // The scenario's softening, clock and cached evaluation state follow the new configuration.
+  std::vector<float> soft2(h->nseg);
+  HIP_TRY(h, hipMemcpy(soft2.data(), h->sc.seg_soft2, h->nseg * sizeof(float), hipMemcpyDeviceToHost));
+  soft2[segment] = *softening * *softening;
+  const double s2d = (double)*softening * (double)*softening;
+  HIP_TRY(h, hipMemcpy(h->sc.seg_soft2 + segment, &soft2[segment], sizeof(float), hipMemcpyHostToDevice));
+  HIP_TRY(h, hipMemcpy(h->sc.seg_soft2d + segment, &s2d, sizeof(double), hipMemcpyHostToDevice));
+  h->zero_soft = false;
+  for (float v : soft2) h->zero_soft |= !(v > 0.f);
+  const SegClock clk{0.0, 0.0, 0ull, 0xffffffffu, 0u};
+  HIP_TRY(h, hipMemcpy(h->sc.seg_clk + segment, &clk, sizeof clk, hipMemcpyHostToDevice));
+  h->have_fe = false;
+  h->h_settled = false;
+  h->pos_fresh = false;
+  return IHS_OK;
+}
+
 }  // extern "C"

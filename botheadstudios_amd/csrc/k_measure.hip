@@ -262,17 +262,17 @@ __global__ __launch_bounds__(256) void k_mass_tiles(const SphParticle48* __restr
   if (threadIdx.x == 0) tsum[blockIdx.x] = t;
 }
 
-// One block: the first sorted particle whose running mass reaches 0.85 m_total (main.rs:738-747,
-// gpu_sph.rs:411-420). r_remnant = its radius, m_remnant = the running mass there; without a crossing the
-// largest radius and m_total, as the reference initialises them.
+// One block: the first sorted particle whose running mass reaches frac m_total (0.85: main.rs:738-747,
+// gpu_sph.rs:411-420; 0.98: body_bulk, gpu_sph.rs:293-304). r_remnant = its radius, m_remnant = the running mass
+// there; without a crossing the largest radius and m_total, as the reference initialises them.
 __global__ __launch_bounds__(256) void k_remnant(const SphParticle48* __restrict__ P, const uint32_t* __restrict__ perm,
-                                                 int64_t n, const dd* __restrict__ tsum, int ntile,
+                                                 int64_t n, const dd* __restrict__ tsum, int ntile, double frac,
                                                  MeasureFrame* __restrict__ fr) {
   __shared__ int s_tile;
   __shared__ double s_base_hi, s_base_lo;
   __shared__ double s_hi[256], s_lo[256];
   __shared__ int64_t s_best;
-  const double thr = 0.85 * fr->m_total;
+  const double thr = frac * fr->m_total;
   // Inclusive prefix over threads of the tile sums each thread owns, Hillis-Steele in LDS.
   auto block_inclusive = [&](dd v) -> dd {
     s_hi[threadIdx.x] = v.hi;
@@ -953,6 +953,28 @@ __global__ __launch_bounds__(256) void k_best_index(int64_t C, const ihs_clump* 
     atomicMin(&out->best, (int32_t)c);
 }
 
+// body_bulk filters: the records of one provenance, then those within `rmax` of a centre.
+__global__ __launch_bounds__(256) void k_flag_prov(const SphParticle48* __restrict__ P, int64_t n, uint32_t prov,
+                                                   uint32_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * MB_THREADS + threadIdx.x;
+  if (i < n) flag[i] = P[i].prov == prov ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_flag_within(const SphParticle48* __restrict__ P, int64_t n, double cx,
+                                                     double cy, double cz, double rmax, uint32_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * MB_THREADS + threadIdx.x;
+  if (i >= n) return;
+  const double dx = (double)P[i].pos[0] - cx, dy = (double)P[i].pos[1] - cy, dz = (double)P[i].pos[2] - cz;
+  flag[i] = sqrt(dx * dx + dy * dy + dz * dz) <= rmax ? 1u : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_compact_rec(const SphParticle48* __restrict__ P, int64_t n,
+                                                     const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                                     SphParticle48* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * MB_THREADS + threadIdx.x;
+  if (i < n && flag[i]) out[pos[i]] = P[i];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------------------------
@@ -1007,6 +1029,8 @@ hipError_t measure_reserve(MeasureBufs& B, int64_t cap) {
   get(&B.cstart, c + 1);
   get(&B.ioff, c + 1);
   get(&B.crec, c);
+  get(&B.rec[0], c);
+  get(&B.rec[1], c);
   if (err != hipSuccess) {
     measure_release(B);
     return err;
@@ -1030,7 +1054,7 @@ hipError_t measure_frame(hipStream_t s, MeasureBufs& B, const SphParticle48* P, 
   const uint32_t* perm = B.rv[cur];
   const int ntile = (int)((n + MB_TILE - 1) / MB_TILE);
   k_mass_tiles<<<ntile, MB_THREADS, 0, s>>>(P, perm, n, (dd*)B.tsum);
-  k_remnant<<<1, MB_THREADS, 0, s>>>(P, perm, n, (const dd*)B.tsum, ntile, B.frame);
+  k_remnant<<<1, MB_THREADS, 0, s>>>(P, perm, n, (const dd*)B.tsum, ntile, 0.85, B.frame);
   // Classification and the disk set.
   const Chunks ch = chunks_for(n);
   switch (mode) {
@@ -1184,6 +1208,67 @@ hipError_t clump_set(hipStream_t s, MeasureBufs& B, int64_t k, const ClumpQuery&
   }
   if (labels) labels->assign(lab.begin(), lab.end());
   if (sum) *sum = cs_h;
+  return hipSuccess;
+}
+
+namespace {
+// Centre and mass of P[0, n) and the radius about that centre enclosing `frac` of the mass, into the frame.
+hipError_t centre_and_radius(hipStream_t s, MeasureBufs& B, const SphParticle48* P, int64_t n, double frac,
+                             MeasureFrame* out) {
+  dd* part = (dd*)B.part;
+  const int nsum = (int)std::min<int64_t>(MB_MAXBLK, std::max<int64_t>(1, (n + 2047) / 2048));
+  k_frame_sums<<<nsum, MB_THREADS, 0, s>>>(P, n, part);
+  k_frame_final<<<1, MB_THREADS, 0, s>>>(part, nsum, B.frame);
+  k_radius_keys<<<g256(n), MB_THREADS, 0, s>>>(P, n, B.frame, B.rk[0], B.khi, B.rv[0]);
+  int cur = launch_radix_pairs(s, n, 32, B.rk, B.rv, 0, B.hist, B.dtot);
+  k_gather_u32<<<g256(n), MB_THREADS, 0, s>>>(B.rv[cur], B.khi, n, B.rk[cur]);
+  cur = launch_radix_pairs(s, n, 32, B.rk, B.rv, cur, B.hist, B.dtot);
+  const int ntile = (int)((n + MB_TILE - 1) / MB_TILE);
+  k_mass_tiles<<<ntile, MB_THREADS, 0, s>>>(P, B.rv[cur], n, (dd*)B.tsum);
+  k_remnant<<<1, MB_THREADS, 0, s>>>(P, B.rv[cur], n, (const dd*)B.tsum, ntile, frac, B.frame);
+  hipError_t e = hipMemcpyAsync(out, B.frame, sizeof *out, hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+
+// The records of P[0, n) whose flag is set, in order, into dst; returns their count.
+hipError_t compact_records(hipStream_t s, MeasureBufs& B, const SphParticle48* P, int64_t n, SphParticle48* dst,
+                           uint32_t* count) {
+  launch_exscan(s, B.flag, n, B.scan, B.bsum, &B.frame->n_sel);
+  k_compact_rec<<<g256(n), MB_THREADS, 0, s>>>(P, n, B.flag, B.scan, dst);
+  hipError_t e = hipMemcpyAsync(count, &B.frame->n_sel, sizeof *count, hipMemcpyDeviceToHost, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
+}  // namespace
+
+// gpu_sph.rs:279-316: centre of mass, then the radius enclosing 98% of the mass about it; twice, re-centre on the
+// bodies within 1.5 radii of the centre (always filtering the whole body) and re-measure. The mass is the whole
+// body's.
+hipError_t body_bulk(hipStream_t s, MeasureBufs& B, const SphParticle48* P, int64_t n, uint32_t prov, double out[5]) {
+  for (int q = 0; q < 5; ++q) out[q] = 0.0;
+  k_flag_prov<<<g256(n), MB_THREADS, 0, s>>>(P, n, prov, B.flag);
+  uint32_t m0 = 0;
+  hipError_t e = compact_records(s, B, P, n, B.rec[0], &m0);
+  if (e != hipSuccess || m0 == 0) return e;
+  MeasureFrame fr;
+  if ((e = centre_and_radius(s, B, B.rec[0], m0, 0.98, &fr)) != hipSuccess) return e;
+  const double mass = fr.m_total;
+  double c[3] = {fr.com[0], fr.com[1], fr.com[2]}, r = fr.r_remnant;
+  for (int pass = 0; pass < 2; ++pass) {
+    k_flag_within<<<g256(m0), MB_THREADS, 0, s>>>(B.rec[0], m0, c[0], c[1], c[2], 1.5 * r, B.flag);
+    uint32_t kk = 0;
+    if ((e = compact_records(s, B, B.rec[0], m0, B.rec[1], &kk)) != hipSuccess) return e;
+    if (kk == 0) break;
+    if ((e = centre_and_radius(s, B, B.rec[1], kk, 0.98, &fr)) != hipSuccess) return e;
+    for (int q = 0; q < 3; ++q) c[q] = fr.com[q];
+    r = fr.r_remnant;
+  }
+  out[0] = c[0];
+  out[1] = c[1];
+  out[2] = c[2];
+  out[3] = mass;
+  out[4] = r;
   return hipSuccess;
 }
 

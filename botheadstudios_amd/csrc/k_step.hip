@@ -255,6 +255,39 @@ __global__ __launch_bounds__(256) void k_pos_f64(int64_t n, MasterBufs S, double
   out[3 * (int64_t)i + 2] = (double)((const PT*)S.pz)[t];
 }
 
+// assemble_from_relaxed_n (gpu_sph.rs:318-339) in place over storage positions [t0, t1): body k = prov is
+// recentred on its robust centre, moved to its offset and set moving at its bulk velocity plus spin x (r - centre).
+// pl holds 12 doubles per body: centre, offset, velocity, spin. Positions and velocities are rounded to FP32, as
+// the reference's SphParticle output is; minh receives the order-preserving key of the smallest h.
+template <class PT>
+__global__ __launch_bounds__(256) void k_assemble(int64_t t0, int64_t t1, MasterBufs S, const double* __restrict__ pl,
+                                                  uint32_t* __restrict__ minh) {
+  const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= t1) return;
+  const double* q = pl + 12 * S.prov[t];
+  PT* px = (PT*)S.px;
+  PT* py = (PT*)S.py;
+  PT* pz = (PT*)S.pz;
+  const double l0 = (double)(float)px[t] - q[0], l1 = (double)(float)py[t] - q[1], l2 = (double)(float)pz[t] - q[2];
+  px[t] = (PT)(float)(l0 + q[3]);
+  py[t] = (PT)(float)(l1 + q[4]);
+  pz[t] = (PT)(float)(l2 + q[5]);
+  // glam DVec3::cross: (s.y l.z - l.y s.z, s.z l.x - l.z s.x, s.x l.y - l.x s.y)
+  S.vx[t] = (float)(q[6] + (q[10] * l2 - l1 * q[11]));
+  S.vy[t] = (float)(q[7] + (q[11] * l0 - l2 * q[9]));
+  S.vz[t] = (float)(q[8] + (q[9] * l1 - l0 * q[10]));
+  atomicMin(minh, f2key(S.h[t]));
+}
+
+// Largest provenance over storage positions [t0, t1).
+__global__ __launch_bounds__(256) void k_prov_max(int64_t t0, int64_t t1, const uint32_t* __restrict__ prov,
+                                                  uint32_t* __restrict__ out) {
+  const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  uint32_t v = t < t1 ? prov[t] : 0u;
+  v = wave_max(v);
+  if ((threadIdx.x & 63) == 0 && v) atomicMax(out, v);
+}
+
 }  // namespace
 
 void launch_fe_init(hipStream_t s, uint32_t* flags, const SegView& sv) { k_fe_init<<<1, 256, 0, s>>>(flags, sv); }
@@ -288,6 +321,17 @@ void launch_relax(hipStream_t s, int split, int64_t n, const MasterBufs& st, con
   const unsigned g = std::max(1u, g256(n));
   if (split) k_relax<double><<<g, 256, 0, s>>>(n, st, asph, accg, ctl, sv);
   else k_relax<float><<<g, 256, 0, s>>>(n, st, asph, accg, ctl, sv);
+}
+
+void launch_prov_max(hipStream_t s, int64_t t0, int64_t t1, const MasterBufs& st, uint32_t* out) {
+  k_prov_max<<<std::max(1u, g256(t1 - t0)), 256, 0, s>>>(t0, t1, st.prov, out);
+}
+
+void launch_assemble(hipStream_t s, int split, int64_t t0, int64_t t1, const MasterBufs& st, const double* pl,
+                     uint32_t* minh) {
+  const unsigned g = std::max(1u, g256(t1 - t0));
+  if (split) k_assemble<double><<<g, 256, 0, s>>>(t0, t1, st, pl, minh);
+  else k_assemble<float><<<g, 256, 0, s>>>(t0, t1, st, pl, minh);
 }
 
 void launch_energy(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* accg, double* part,

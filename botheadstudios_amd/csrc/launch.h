@@ -248,6 +248,12 @@ void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st
                        const float4* accg, const StepCtl* ctl, const SegView& sv);
 void launch_kick(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
                  const SegView& sv);
+// Largest provenance over storage positions [t0, t1) into *out (atomicMax; the caller clears it).
+void launch_prov_max(hipStream_t s, int64_t t0, int64_t t1, const MasterBufs& st, uint32_t* out);
+// In-place assembly of storage positions [t0, t1) (gpu_sph.rs:318-339): pl = 12 doubles per provenance (centre,
+// offset, velocity, spin); *minh receives the order-preserving key of the smallest h (the caller sets it to ~0).
+void launch_assemble(hipStream_t s, int split, int64_t t0, int64_t t1, const MasterBufs& st, const double* pl,
+                     uint32_t* minh);
 void launch_relax(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,
                   const StepCtl* ctl, const SegView& sv);
 // FP64 KE, IE, PE, p, L (9 values) into out; part needs 9 * 1024 doubles.

@@ -30,7 +30,7 @@ struct MeasureFrame {
   uint32_t n_disk;
   uint32_t n_clumps;  // phase 2: clump count
   uint32_t n_items;   // phase 2: pair-potential work items
-  uint32_t pad;
+  uint32_t n_sel;     // body_bulk: bodies selected by the last filter
 };
 
 // Phase 2 parameters.
@@ -87,6 +87,7 @@ struct MeasureDev {
   uint32_t* ioff = nullptr;    // C + 1
   ihs_clump* crec = nullptr;
   void* pe_part = nullptr;     // per work item
+  SphParticle48* rec[2] = {nullptr, nullptr};  // body_bulk: the body's records, the clipped subset
 };
 
 struct MeasureBufs : MeasureDev {
@@ -107,6 +108,9 @@ hipError_t measure_frame(hipStream_t s, MeasureBufs& B, const SphParticle48* P, 
 // member; labels (may be null) each body's clump index; sum the summary.
 hipError_t clump_set(hipStream_t s, MeasureBufs& B, int64_t k, const ClumpQuery& q, std::vector<ihs_clump>* all,
                      std::vector<int32_t>* labels, ClumpSummary* sum, ihs_clump* best);
+
+// body_bulk (gpu_sph.rs:279-316) over the records of provenance `prov` in P[0, n): out = centre xyz, mass, radius.
+hipError_t body_bulk(hipStream_t s, MeasureBufs& B, const SphParticle48* P, int64_t n, uint32_t prov, double out[5]);
 
 // Uploads n host bodies into the body-set arrays (find_clumps input).
 hipError_t clump_upload(hipStream_t s, MeasureBufs& B, int64_t n, const double* pos3, const double* vel3,

@@ -6,6 +6,8 @@ rho0, a, b, A, B, E0, E_iv, E_cv, alpha, beta.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 # ie/data/materials.json:69-82 (granite), 155-168 (basalt), 237-251 (peridotite), 520-534 (iron),
@@ -36,3 +38,32 @@ def sph_eos48(names) -> np.ndarray:
     out = np.zeros((t.shape[0], 12), np.float32)
     out[:, :10] = t
     return out
+
+
+def tillotson_pressure(eos10, rho: float, u: float) -> float:
+    """Host FP64 Tillotson pressure (ie/crates/engine/src/eos.rs:52-85), for set-up arithmetic such as the
+    relaxation time step. eos10 = rho0, a, b, A, B, E0, E_iv, E_cv, alpha, beta."""
+    rho0, a, b, cap_a, cap_b, e0, e_iv, e_cv, alpha, beta = (float(x) for x in eos10[:10])
+    rho = max(rho, 1.0e-9)
+    eta = rho / rho0
+    mu = eta - 1.0
+    omega = u / (e0 * eta * eta) + 1.0
+    p_c = (a + b / omega) * rho * u + cap_a * mu + cap_b * mu * mu
+    if eta >= 1.0 or u <= e_iv:
+        return p_c
+    z = rho0 / rho - 1.0
+    p_e = a * rho * u + (b * rho * u / omega + cap_a * mu * math.exp(-beta * z)) * math.exp(-alpha * z * z)
+    if u >= e_cv:
+        return p_e
+    return ((u - e_iv) * p_e + (e_cv - u) * p_c) / (e_cv - e_iv)
+
+
+def tillotson_c2(eos10, rho: float, u: float) -> float:
+    """Adiabatic sound speed squared by central differences (eos.rs:88-96)."""
+    rho = max(rho, 1.0e-9)
+    dr = rho * 1.0e-4
+    du = abs(u) * 1.0e-4 + 1.0
+    dp_drho = (tillotson_pressure(eos10, rho + dr, u) - tillotson_pressure(eos10, rho - dr, u)) / (2.0 * dr)
+    dp_du = (tillotson_pressure(eos10, rho, u + du) - tillotson_pressure(eos10, rho, u - du)) / (2.0 * du)
+    p = tillotson_pressure(eos10, rho, u)
+    return max(dp_drho + p / (rho * rho) * dp_du, 0.0)

@@ -275,6 +275,27 @@ int ihs_disk_stats(ihs_handle* h, int32_t segment, ihs_disk_summary* out);
 int ihs_disk_clumps(ihs_handle* h, int32_t segment, int32_t require_bound, double* frame, ihs_clump* clumps,
                     int64_t cap, int64_t* n_clumps);
 
+/* ---- IC assembly on the device: the steps around the GPU relaxation (gpu_sph.rs:198-339) ----
+ * A caller uploads the bodies far apart (each body's particles tagged with its source index as prov), relaxes
+ * them with ihs_relax (artificial viscosity off, ihs_set_av(h, 0, 0), as the reference's relax discipline asks),
+ * then places them on the collision in place, with no readback and no second upload. */
+
+/* body_bulk (gpu_sph.rs:279-316): the robust centre of mass, total mass and the radius enclosing 98% of the mass
+ * of the particles with provenance prov, after two passes that re-centre on the particles within 1.5 radii.
+ * out5 = centre x, y, z, mass, radius (all zero for a provenance with no particles). Synchronous. */
+int ihs_body_bulk(ihs_handle* h, int32_t segment, uint32_t prov, double* out5);
+
+/* assemble_from_relaxed_n (gpu_sph.rs:318-339) in place: every particle of body k = prov is re-centred on its
+ * body_bulk centre, moved to offset_k and set moving at vel_k + spin_k x (r - centre). placements holds nbody x
+ * 9 doubles: offset xyz, velocity xyz, spin (angular velocity) xyz. Positions and velocities are rounded to FP32,
+ * as the reference's output records are. Returns the reference's softening (min h / 4, now the scenario's
+ * softening) and its shock-safe dt, 0.01 min h / (20 km/s + the fastest placement speed), for ihs_set_dt. The
+ * scenario's clock restarts at 0. Particles whose prov has no placement are an error (the reference drops them).
+ * Synchronous. */
+#define IHS_MAX_BODIES 64
+int ihs_assemble(ihs_handle* h, int32_t segment, const double* placements, int32_t nbody, float* softening,
+                 float* dt);
+
 /* accretion::find_clumps (accretion.rs:73-155) on n host bodies (pos3, vel3: n*3 doubles; mass, rho: n):
  * friends-of-friends within linking_length, then each clump's mass, centre, density, radius, internal kinetic
  * and softened self-potential energy, boundness and the Roche gate against the central body. Clumps of zero

@@ -11,6 +11,8 @@
 //                      moonlet_bodies, largest_moonlet_orbit and disk_moonlets share) with orbit.rs:91-116
 //   or_find_clumps     ie/crates/engine/src/accretion.rs:73-163 over neighbors.rs:41-85 (NeighborGrid pairs,
 //                      brute force below 512 bodies)
+//   or_body_bulk       ie/crates/engine/src/gpu_sph.rs:279-316 (clipped centre, 98%-mass radius)
+//   or_assemble_n      ie/crates/engine/src/gpu_sph.rs:318-339 (placement of relaxed bodies)
 //
 // Two places where the reference is not deterministic are fixed here, the same way the HIP library fixes them:
 // clumps are visited in order of their lowest member (the reference iterates a HashMap), so the Moon candidate
@@ -459,6 +461,102 @@ int64_t or_find_clumps(int64_t n, const double* pos3, const double* vel3, const 
       for (size_t i : mem) label[i] = kept;
     ++kept;
   }
+  return kept;
+}
+
+}  // extern "C"
+
+namespace {
+// gpu_sph.rs:283-304: com and bulk_radius over a selection.
+void bulk_com(const or_particle* P, const std::vector<int64_t>& sel, double c[3]) {
+  double mm = 0.0;
+  for (int64_t i : sel) mm += (double)P[i].mass;
+  c[0] = c[1] = c[2] = 0.0;
+  if (mm <= 0.0) return;
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int64_t i : sel)
+    for (int q = 0; q < 3; ++q) s[q] += (double)P[i].pos[q] * (double)P[i].mass;
+  for (int q = 0; q < 3; ++q) c[q] = s[q] / mm;
+}
+double radius_about(const or_particle& p, const double c[3]) {
+  const double d[3] = {(double)p.pos[0] - c[0], (double)p.pos[1] - c[1], (double)p.pos[2] - c[2]};
+  return std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+}
+double bulk_radius(const or_particle* P, const std::vector<int64_t>& sel, const double c[3]) {
+  double mm = 0.0;
+  for (int64_t i : sel) mm += (double)P[i].mass;
+  std::vector<std::pair<double, double>> rr;
+  rr.reserve(sel.size());
+  for (int64_t i : sel) rr.push_back({radius_about(P[i], c), (double)P[i].mass});
+  std::stable_sort(rr.begin(), rr.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+  double cum = 0.0, r = 0.0;
+  for (const auto& x : rr) {
+    cum += x.second;
+    r = x.first;
+    if (cum >= 0.98 * mm) break;
+  }
+  return r;
+}
+}  // namespace
+
+extern "C" {
+
+void or_body_bulk(const or_particle* P, int64_t n, uint32_t prov, double* out5) {
+  std::vector<int64_t> ps;
+  for (int64_t i = 0; i < n; ++i)
+    if (P[i].prov == prov) ps.push_back(i);
+  double m = 0.0;
+  for (int64_t i : ps) m += (double)P[i].mass;
+  double c[3];
+  bulk_com(P, ps, c);
+  double r = bulk_radius(P, ps, c);
+  for (int pass = 0; pass < 2; ++pass) {
+    std::vector<int64_t> keep;
+    for (int64_t i : ps)
+      if (radius_about(P[i], c) <= 1.5 * r) keep.push_back(i);
+    if (keep.empty()) break;
+    bulk_com(P, keep, c);
+    r = bulk_radius(P, keep, c);
+  }
+  out5[0] = c[0];
+  out5[1] = c[1];
+  out5[2] = c[2];
+  out5[3] = m;
+  out5[4] = r;
+}
+
+int64_t or_assemble_n(const or_particle* P, int64_t n, const double* placements, int32_t nbody, or_particle* out,
+                      float* softening, float* dt) {
+  std::vector<double> coms(3 * (size_t)nbody);
+  for (int k = 0; k < nbody; ++k) {
+    double b[5];
+    or_body_bulk(P, n, (uint32_t)k, b);
+    for (int q = 0; q < 3; ++q) coms[3 * k + q] = b[q];
+  }
+  int64_t kept = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (P[i].prov >= (uint32_t)nbody) continue;
+    const double* pl = placements + 9 * P[i].prov;
+    const double* c = &coms[3 * P[i].prov];
+    const double l[3] = {(double)P[i].pos[0] - c[0], (double)P[i].pos[1] - c[1], (double)P[i].pos[2] - c[2]};
+    const double sp[3] = {pl[6], pl[7], pl[8]};
+    const double cr[3] = {sp[1] * l[2] - l[1] * sp[2], sp[2] * l[0] - l[2] * sp[0], sp[0] * l[1] - l[0] * sp[1]};
+    or_particle q = P[i];
+    for (int a = 0; a < 3; ++a) {
+      q.pos[a] = (float)(l[a] + pl[a]);
+      q.vel[a] = (float)(pl[3 + a] + cr[a]);
+    }
+    out[kept++] = q;
+  }
+  float min_h = INFINITY;
+  for (int64_t i = 0; i < kept; ++i) min_h = std::fmin(min_h, out[i].h);
+  *softening = 0.25f * min_h;
+  double max_speed = 0.0;
+  for (int k = 0; k < nbody; ++k) {
+    const double* v = placements + 9 * k + 3;
+    max_speed = std::fmax(max_speed, std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]));
+  }
+  *dt = (float)(0.01 * (double)min_h / (20000.0 + max_speed));
   return kept;
 }
 

@@ -14,6 +14,11 @@
  *                      link graph; clump potential equals a numpy pair sum; a cold bound blob outside Roche is the
  *                      Moon candidate, and a hot or inside-Roche one is not.
  *   or_engine_frame  — the same remnant and partition checks; a bound-only disk is a subset of the any-orbit disk.
+ *   or_body_bulk     — a clean ball: the plain centre of mass and the ceil(0.98 n)-th radius; strays far outside
+ *                      (1% of the mass at 20 radii) leave the centre and radius of the ball, where a plain mean
+ *                      moves by a fifth of a radius.
+ *   or_assemble_n    — each placed body's bulk centre lands on its offset, velocities are the bulk velocity plus
+ *                      spin x r, dropped provenances, softening = min h / 4 and the dt formula.
  *   or_find_clumps   — accretion.rs's own tests (two cold blobs accrete with mass, momentum and centre of mass
  *                      conserved; the Roche gate; a hot group is unbound), grid pairs == brute-force pairs above
  *                      512 bodies, components == scipy.
@@ -73,6 +78,12 @@ void or_engine_frame(const or_particle* P, int64_t n, int32_t require_bound, dou
 int64_t or_find_clumps(int64_t n, const double* pos3, const double* vel3, const double* mass, const double* rho,
                        double linking_length, double g, double softening, const double* central3,
                        double central_mass, double central_radius, or_clump* out, int64_t cap, int64_t* label);
+
+/* body_bulk (gpu_sph.rs:279-316): out5 = centre xyz, mass, radius. */
+void or_body_bulk(const or_particle* P, int64_t n, uint32_t prov, double* out5);
+/* assemble_from_relaxed_n (gpu_sph.rs:318-339): placements = nbody x 9 (offset, vel, spin); returns the kept count. */
+int64_t or_assemble_n(const or_particle* P, int64_t n, const double* placements, int32_t nbody, or_particle* out,
+                      float* softening, float* dt);
 
 #ifdef __cplusplus
 }

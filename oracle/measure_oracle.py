@@ -55,6 +55,10 @@ def _lib():
         L.or_engine_frame.restype = None
         L.or_find_clumps.argtypes = [i64, vp, vp, vp, vp, f64, f64, f64, vp, f64, f64, vp, i64, vp]
         L.or_find_clumps.restype = i64
+        L.or_body_bulk.argtypes = [vp, i64, C.c_uint32, vp]
+        L.or_body_bulk.restype = None
+        L.or_assemble_n.argtypes = [vp, i64, vp, i32, vp, vp, vp]
+        L.or_assemble_n.restype = i64
         _declared = True
     return L
 
@@ -141,3 +145,23 @@ def disk_set(parts, require_bound: bool):
     mass = d["mass"].astype(np.float64)
     rho = np.fmax(d["rho"], np.float32(1.0)).astype(np.float64)  # f32::max(1.0): NaN gives 1
     return f, np.nonzero(sel)[0], pos, vel, mass, rho
+
+
+def body_bulk(parts, prov: int):
+    """(centre, mass, radius) of gpu_sph.rs:279-316."""
+    P = as_particles(parts)
+    out = np.zeros(5, np.float64)
+    _lib().or_body_bulk(P.ctypes.data, P.shape[0], int(prov), out.ctypes.data)
+    return out[:3].copy(), float(out[3]), float(out[4])
+
+
+def assemble_n(parts, placements):
+    """gpu_sph.rs:318-339. placements: (nbody, 9) offset, vel, spin. Returns (records, softening, dt)."""
+    P = as_particles(parts)
+    pl = np.ascontiguousarray(placements, np.float64).reshape(-1, 9)
+    out = np.zeros(max(1, P.shape[0]), PARTICLE_DTYPE)
+    soft = C.c_float(0.0)
+    dt = C.c_float(0.0)
+    k = _lib().or_assemble_n(P.ctypes.data, P.shape[0], pl.ctypes.data, pl.shape[0], out.ctypes.data, C.byref(soft),
+                             C.byref(dt))
+    return out[:k], soft.value, dt.value
