@@ -19,6 +19,7 @@ Results are stored per particle-count bucket in botheadstudios_amd/data/tuned.js
 read through `tuned_params(n)`.
 
     python -m botheadstudios_amd.autotune --cfg 4          # tune for the 1M-particle impact
+    python -m botheadstudios_amd.autotune --knee           # the standalone gravity's direct-vs-tree knee
 """
 from __future__ import annotations
 
@@ -110,18 +111,90 @@ def tuned_params(n: int, path: str = TUNED) -> dict:
             "quadrupole": bool(best.get("quadrupole", False))}
 
 
+KNEE_NS = (1500, 3000, 6000, 12000, 24000, 48000, 96000, 192000, 384000)
+
+
+def gravity_knee(ns=KNEE_NS, leaves=(1, 16), theta: float = 0.5, reps: int = 5, seed: int = 11, device: int = 0,
+                 log=print) -> dict:
+    """The direct-vs-tree knee of the standalone gravity (GravityField, ihs_gravity) measured on this device.
+
+    The reference reads its knee off the same comparison on its own box (gpu_gravity.rs:40-50: N = 24,000 on an
+    Apple M4 Max); both paths here include the host-to-device copy of the bodies and the copy back of the
+    accelerations, as a GravityField call does. Clouds are uniform balls of equal masses. For each leaf size the
+    tree's and the direct sum's ms per call are measured at every N; the leaf with the fastest tree at the largest N
+    is kept, and the knee is the smallest N from which that tree is faster at every larger measured N."""
+    import numpy as np
+
+    from .gpu_sph import GpuSph
+
+    rng = np.random.default_rng(seed)
+    table = []
+    for leaf in leaves:
+        with GpuSph(max(ns), device=device, leaf_size=leaf, h_mode="fixed", nbr_capacity=1, use_graph=False) as s:
+            for n in ns:
+                v = rng.normal(size=(n, 3))
+                pos = v / np.linalg.norm(v, axis=1, keepdims=True) * (6.4e6 * np.cbrt(rng.random(n)))[:, None]
+                mass = np.full(n, 6.0e24 / n)
+                soft = 0.25 * 6.4e6 / np.cbrt(n)
+                row = {"n": n, "leaf": leaf}
+                for mode in ("direct", "tree"):
+                    s.gravity(pos, mass, soft, theta, mode)
+                    t0 = time.perf_counter()
+                    for _ in range(reps):
+                        s.gravity(pos, mass, soft, theta, mode)
+                    row[f"{mode}_ms"] = 1e3 * (time.perf_counter() - t0) / reps
+                table.append(row)
+                log(f"leaf {leaf} N={n}: direct {row['direct_ms']:.3f} ms, tree {row['tree_ms']:.3f} ms")
+    nmax = max(ns)
+    leaf = min(leaves, key=lambda k: next(r["tree_ms"] for r in table if r["leaf"] == k and r["n"] == nmax))
+    rows = sorted((r for r in table if r["leaf"] == leaf), key=lambda r: r["n"])
+    knee = nmax * 2
+    for i in range(len(rows) - 1, -1, -1):
+        if rows[i]["tree_ms"] < rows[i]["direct_ms"]:
+            knee = rows[i]["n"]
+        else:
+            break
+    return {"knee": int(knee), "leaf": int(leaf), "theta": theta, "table": table}
+
+
+def save_knee(result: dict, path: str = TUNED):
+    data = {}
+    if os.path.exists(path):
+        with open(path) as f:
+            data = json.load(f)
+    data["gravity_knee"] = result
+    os.makedirs(os.path.dirname(path), exist_ok=True)
+    with open(path, "w") as f:
+        json.dump(data, f, indent=1)
+
+
+def tuned_knee(path: str = TUNED):
+    """(knee, leaf) of the standalone gravity measured on this device type, or None."""
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f).get("gravity_knee")
+    return (int(k["knee"]), int(k["leaf"])) if k else None
+
+
 def main(argv=None):
     import argparse
 
     from . import ics
 
     ap = argparse.ArgumentParser()
+    ap.add_argument("--knee", action="store_true", help="measure the standalone gravity's direct-vs-tree knee")
     ap.add_argument("--cfg", type=int, default=4)
     ap.add_argument("--n", type=int, default=None)
     ap.add_argument("--err", type=float, default=1e-3)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--out", default=TUNED)
     a = ap.parse_args(argv)
+    if a.knee:
+        res = gravity_knee()
+        save_knee(res, a.out)
+        print(json.dumps({"knee": res["knee"], "leaf": res["leaf"]}), flush=True)
+        return
     ic = ics.config_ic(a.cfg, n=a.n)
     t0 = time.time()
     res = tune(ic, err_target=a.err, steps=a.steps)

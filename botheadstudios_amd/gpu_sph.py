@@ -296,21 +296,34 @@ class GpuSph:
         return (acc, pot) if with_pot else acc
 
 
-class GravityField:
-    """gpu_gravity.rs:532-626: direct sum below the knee, LBVH tree at or above it (native: synchronous)."""
+def gravity_route(n: int, knee: int) -> str:
+    """gpu_gravity.rs:556-578: the exact direct sum below the knee, the LBVH tree at or above it."""
+    return "tree" if n >= knee else "direct"
 
-    TREE_KNEE = 24_000  # gpu_gravity.rs:50
+
+class GravityField:
+    """gpu_gravity.rs:532-626: direct sum below the knee, LBVH tree at or above it (native: synchronous).
+
+    The reference's knee (TREE_KNEE) was measured on its own box; `autotune.gravity_knee` measures this device's,
+    stored in data/tuned.json with the leaf size of the faster tree, and that pair is used when present."""
+
+    TREE_KNEE = 24_000  # gpu_gravity.rs:50 (Apple M4 Max)
     THETA = 0.5         # gpu_gravity.rs:55
 
-    def __init__(self, capacity: int, device: int = 0, leaf_size: int = 16):
+    def __init__(self, capacity: int, device: int = 0, leaf_size: int | None = None, knee: int | None = None):
+        from .autotune import tuned_knee
+
+        measured = tuned_knee()
+        self.knee = int(knee if knee is not None else (measured[0] if measured else self.TREE_KNEE))
+        if leaf_size is None:
+            leaf_size = measured[1] if measured else 16
         self._sph = GpuSph(capacity, device=device, leaf_size=leaf_size, h_mode="fixed", nbr_capacity=1,
                            use_graph=False)
 
     def accelerations(self, pos, mass, softening: float):
         if len(pos) == 0:
             return np.zeros((0, 3))
-        mode = "tree" if len(pos) >= self.TREE_KNEE else "direct"
-        return self._sph.gravity(pos, mass, softening, self.THETA, mode)
+        return self._sph.gravity(pos, mass, softening, self.THETA, gravity_route(len(pos), self.knee))
 
     def close(self):
         self._sph.close()

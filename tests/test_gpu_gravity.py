@@ -59,7 +59,7 @@ def test_tree_repeatable(cloud):
 
 def test_gravity_field_knee():
     """gpu_gravity.rs:854-880: below the knee direct, at/above tree; both agree with f64."""
-    f = GravityField(30000, leaf_size=16)
+    f = GravityField(30000, leaf_size=16, knee=24000)
     try:
         for n in (600, 24000):
             pos, mass = bh_cloud(n, seed=0x51ED2707 + n)
@@ -67,7 +67,7 @@ def test_gravity_field_knee():
             idx = np.arange(0, n, max(1, n // 512))
             ref = oracle.direct_f64(pos, mass, 5.0e3, targets=idx)
             rms, _ = rms_rel(acc[idx], ref)
-            assert rms < (1e-5 if n < f.TREE_KNEE else 1e-2)
+            assert rms < (1e-5 if n < f.knee else 1e-2)
     finally:
         f.close()
 

@@ -108,3 +108,19 @@ def test_impact_ic_configs_sizes():
 def test_product_fails_loudly_without_library(tmp_path):
     with pytest.raises(RuntimeError, match="not built"):
         _lib.load(str(tmp_path / "missing.so"))
+
+
+def test_gravity_route_and_measured_knee(tmp_path):
+    """gpu_gravity.rs:556-578 routing (direct below the knee, tree at or above it) and the knee read back from the
+    autotuner's file."""
+    from botheadstudios_amd import autotune
+    from botheadstudios_amd.gpu_sph import gravity_route
+
+    assert gravity_route(599, 600) == "direct" and gravity_route(600, 600) == "tree"
+    path = tmp_path / "tuned.json"
+    assert autotune.tuned_knee(str(path)) is None
+    autotune.save_knee({"knee": 96000, "leaf": 1, "theta": 0.5, "table": []}, str(path))
+    assert autotune.tuned_knee(str(path)) == (96000, 1)
+    # the walk-tuning buckets in the same file are unaffected by the knee entry
+    autotune.save({"n": 1024, "best": {"leaf": 4, "group": 64, "theta": 0.7, "quadrupole": True}}, str(path))
+    assert autotune.tuned_params(1000, str(path))["leaf"] == 4
