@@ -1418,4 +1418,146 @@ int ihs_assemble(ihs_handle* h, int32_t segment, const double* placements, int32
   return IHS_OK;
 }
 
+// ---- Checkpoint / resume: a versioned binary image of the device state ----
+
+namespace {
+constexpr char kCkptMagic[8] = {'I', 'H', 'S', 'C', 'K', 'P', 'T', '1'};
+constexpr uint32_t kCkptVersion = 1;
+
+struct CkptHeader {
+  char magic[8];
+  uint32_t version;
+  uint32_t header_bytes;
+  int64_t n;
+  int64_t total_bytes;
+  int32_t nseg;
+  int32_t split;
+  ihs_config cfg;
+  StepCtl ctl;
+  int64_t force_evals;
+  uint8_t h_settled, zero_soft, mlp_trained, pad[5];
+};
+
+// The device sections of a checkpoint, in file order: the scenario tables, the EOS table, the network weights and
+// the master state in its storage order.
+std::vector<std::pair<void*, size_t>> ckpt_sections(ihs_handle* h, const MasterBufs& S, int64_t n, int nseg) {
+  Scratch& sc = h->sc;
+  const size_t pw = h->split ? 8 : 4;
+  return {
+      {sc.seg_start, sizeof(uint32_t) * (nseg + 1)}, {sc.segof, sizeof(uint32_t) * n},
+      {sc.seg_soft2, sizeof(float) * nseg},          {sc.seg_soft2d, sizeof(double) * nseg},
+      {sc.seg_hcap, sizeof(float) * nseg},           {sc.seg_clk, sizeof(SegClock) * nseg},
+      {sc.seg_root, sizeof(uint32_t) * nseg},        {sc.eos, 16 * sizeof(ihs_eos)},
+      {h->mlp.w, sizeof(float) * MLP_NPARAM},
+      {S.px, pw * n}, {S.py, pw * n}, {S.pz, pw * n}, {S.u, pw * n},
+      {S.vx, 4 * n}, {S.vy, 4 * n}, {S.vz, 4 * n}, {S.h, 4 * n}, {S.m, 4 * n}, {S.mat, (size_t)n},
+      {S.id, 4 * n}, {S.prov, 4 * n}, {S.rho, 4 * n}, {S.feat, 16 * n}, {S.hiter, (size_t)n},
+  };
+}
+
+int64_t ckpt_total(ihs_handle* h) {
+  int64_t t = sizeof(CkptHeader);
+  for (auto& sct : ckpt_sections(h, h->st[h->cur], h->n, h->nseg)) t += (int64_t)sct.second;
+  return t;
+}
+}  // namespace
+
+int64_t ihs_checkpoint_bytes(ihs_handle* h) {
+  if (!h) return IHS_ERR_ARG;
+  if (h->n < 1) return fail(h, IHS_ERR_STATE, "nothing to checkpoint: no particles uploaded");
+  return ckpt_total(h);
+}
+
+int64_t ihs_checkpoint_save(ihs_handle* h, void* buf, int64_t cap) {
+  if (!h || !buf) return IHS_ERR_ARG;
+  if (h->n < 1) return fail(h, IHS_ERR_STATE, "nothing to checkpoint: no particles uploaded");
+  const int64_t total = ckpt_total(h);
+  if (cap < total) return fail(h, IHS_ERR_CAPACITY, "checkpoint needs %lld bytes, buffer holds %lld", (long long)total,
+                               (long long)cap);
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  CkptHeader hd;
+  std::memset(&hd, 0, sizeof hd);
+  std::memcpy(hd.magic, kCkptMagic, sizeof hd.magic);
+  hd.version = kCkptVersion;
+  hd.header_bytes = sizeof(CkptHeader);
+  hd.n = h->n;
+  hd.total_bytes = total;
+  hd.nseg = h->nseg;
+  hd.split = h->split;
+  hd.cfg = h->cfg;
+  HIP_TRY(h, hipMemcpy(&hd.ctl, h->sc.ctl, sizeof(StepCtl), hipMemcpyDeviceToHost));
+  hd.force_evals = h->force_evals;
+  hd.h_settled = h->h_settled;
+  hd.zero_soft = h->zero_soft;
+  hd.mlp_trained = h->mlp.trained;
+  char* out = (char*)buf;
+  std::memcpy(out, &hd, sizeof hd);
+  int64_t off = sizeof hd;
+  for (auto& sct : ckpt_sections(h, h->st[h->cur], h->n, h->nseg)) {
+    HIP_TRY(h, hipMemcpy(out + off, sct.first, sct.second, hipMemcpyDeviceToHost));
+    off += (int64_t)sct.second;
+  }
+  return off;
+}
+
+int ihs_checkpoint_load(ihs_handle* h, const void* buf, int64_t bytes) {
+  if (!h || !buf) return IHS_ERR_ARG;
+  CkptHeader hd;
+  if (bytes < (int64_t)sizeof hd) return fail(h, IHS_ERR_ARG, "checkpoint truncated (%lld bytes)", (long long)bytes);
+  std::memcpy(&hd, buf, sizeof hd);
+  if (std::memcmp(hd.magic, kCkptMagic, sizeof hd.magic) != 0) return fail(h, IHS_ERR_ARG, "not a checkpoint");
+  if (hd.version != kCkptVersion || hd.header_bytes != sizeof hd)
+    return fail(h, IHS_ERR_ARG, "checkpoint version %u (header %u bytes) is not version %u", hd.version,
+                hd.header_bytes, kCkptVersion);
+  if (hd.n < 1 || hd.n > h->cap) return fail(h, IHS_ERR_CAPACITY, "checkpoint of %lld particles, capacity %lld",
+                                             (long long)hd.n, (long long)h->cap);
+  if (hd.nseg < 1 || hd.nseg > IHS_MAX_SEGMENTS) return fail(h, IHS_ERR_ARG, "checkpoint has %d scenarios", hd.nseg);
+  ihs_config mine = h->cfg, theirs = hd.cfg;
+  mine.capacity = theirs.capacity = 0;
+  if (std::memcmp(&mine, &theirs, sizeof mine) != 0)
+    return fail(h, IHS_ERR_ARG, "checkpoint was written with another configuration (create the handle from its "
+                "header's ihs_config)");
+  hipSetDevice(h->device);
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  drop_graph(h);
+  h->n = hd.n;
+  h->nseg = hd.nseg;
+  h->cur = 0;
+  auto secs = ckpt_sections(h, h->st[0], hd.n, hd.nseg);
+  int64_t total = sizeof hd;
+  for (auto& sct : secs) total += (int64_t)sct.second;
+  if (total != hd.total_bytes || total != bytes) {
+    h->n = 0;
+    return fail(h, IHS_ERR_ARG, "checkpoint size %lld does not match its header (%lld)", (long long)bytes,
+                (long long)total);
+  }
+  const char* in = (const char*)buf;
+  int64_t off = sizeof hd;
+  for (auto& sct : secs) {
+    HIP_TRY(h, hipMemcpy(sct.first, in + off, sct.second, hipMemcpyHostToDevice));
+    off += (int64_t)sct.second;
+  }
+  HIP_TRY(h, hipMemcpy(h->sc.ctl, &hd.ctl, sizeof(StepCtl), hipMemcpyHostToDevice));
+  std::vector<uint32_t> start32(hd.nseg + 1);
+  HIP_TRY(h, hipMemcpy(start32.data(), h->sc.seg_start, sizeof(uint32_t) * (hd.nseg + 1), hipMemcpyDeviceToHost));
+  h->seg_start_host.assign(start32.begin(), start32.end());
+  h->zero_soft = hd.zero_soft != 0;
+  h->h_settled = hd.h_settled != 0;
+  h->pos_fresh = false;
+  h->have_fe = false;
+  h->force_evals = hd.force_evals;
+  MlpDev& m = h->mlp;
+  m.trained = hd.mlp_trained != 0;
+  if (m.trained) {
+    HIP_TRY(h, hipMemset(m.adam_m, 0, MLP_NPARAM * sizeof(float)));
+    HIP_TRY(h, hipMemset(m.adam_v, 0, MLP_NPARAM * sizeof(float)));
+    m.adam_t = 0;
+    mlp_sync_bf16(h->stream, m);
+  }
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipGetLastError());
+  return IHS_OK;
+}
+
 }  // extern "C"

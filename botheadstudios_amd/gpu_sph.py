@@ -72,6 +72,20 @@ class GpuSph:
         self._h = h
         self._pending = None
 
+    @classmethod
+    def from_config(cls, cfg: _lib.IhsConfig, device: int = 0) -> "GpuSph":
+        """A handle created from a raw ihs_config (for example the one stored in a checkpoint)."""
+        self = cls.__new__(cls)
+        self._L = _lib.load()
+        self.config = cfg
+        h = C.c_void_p()
+        rc = self._L.ihs_create(int(device), C.byref(cfg), C.byref(h))
+        if rc != 0:
+            raise IhsError(f"ihs_create failed ({rc}); check the device index and configuration")
+        self._h = h
+        self._pending = None
+        return self
+
     # ---- lifetime ----
     def close(self):
         if getattr(self, "_h", None):

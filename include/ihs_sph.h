@@ -296,6 +296,19 @@ int ihs_body_bulk(ihs_handle* h, int32_t segment, uint32_t prov, double* out5);
 int ihs_assemble(ihs_handle* h, int32_t segment, const double* placements, int32_t nbody, float* softening,
                  float* dt);
 
+/* ---- Checkpoint / resume (no counterpart in the reference, whose snapshots stay in memory, impact-run:394-434) ----
+ * A versioned binary image of the device state: an "IHSCKPT1" header holding the ihs_config, step control and
+ * flags, then the scenario tables, the EOS table, the network weights and the master particle state (FP64
+ * positions and u in split precision) in its storage order. A run resumed from a checkpoint continues bit for
+ * bit as the run that wrote it (with reuse_positions off). The header holds, in order, the magic, version,
+ * header size, n, total size, scenario count and precision, then at byte 40 the ihs_config to create the resuming
+ * handle with (any capacity >= n will do). */
+int64_t ihs_checkpoint_bytes(ihs_handle* h);
+/* Writes the checkpoint into buf (cap bytes); returns the bytes written. Synchronous. */
+int64_t ihs_checkpoint_save(ihs_handle* h, void* buf, int64_t cap);
+/* Restores a checkpoint into a handle created with the same configuration. Synchronous. */
+int ihs_checkpoint_load(ihs_handle* h, const void* buf, int64_t bytes);
+
 /* accretion::find_clumps (accretion.rs:73-155) on n host bodies (pos3, vel3: n*3 doubles; mass, rho: n):
  * friends-of-friends within linking_length, then each clump's mass, centre, density, radius, internal kinetic
  * and softened self-potential energy, boundness and the Roche gate against the central body. Clumps of zero

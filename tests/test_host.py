@@ -124,3 +124,25 @@ def test_gravity_route_and_measured_knee(tmp_path):
     # the walk-tuning buckets in the same file are unaffected by the knee entry
     autotune.save({"n": 1024, "best": {"leaf": 4, "group": 64, "theta": 0.7, "quadrupole": True}}, str(path))
     assert autotune.tuned_params(1000, str(path))["leaf"] == 4
+
+
+def test_checkpoint_header_layout():
+    """The header's ihs_config sits at byte 40, after magic, version, header size, n, total size, scenario count
+    and precision; a wrong magic is refused before any device work."""
+    import ctypes as C
+
+    import numpy as np
+    import pytest
+
+    from botheadstudios_amd import _lib, checkpoint
+
+    cfg = _lib.IhsConfig()
+    cfg.capacity, cfg.leaf_size, cfg.theta = 1234, 4, 0.85
+    head = b"IHSCKPT1" + np.array([1, 40 + C.sizeof(cfg)], np.uint32).tobytes() + \
+        np.array([1000, 99999], np.int64).tobytes() + np.array([3, 1], np.int32).tobytes() + bytes(cfg)
+    hd = checkpoint.header(head)
+    assert (hd["n"], hd["scenarios"], hd["split"], hd["total_bytes"]) == (1000, 3, True, 99999)
+    assert hd["config"].capacity == 1234 and hd["config"].leaf_size == 4
+    assert hd["config"].theta == pytest.approx(0.85)
+    with pytest.raises(ValueError):
+        checkpoint.header(b"NOTACKPT" + head[8:])
