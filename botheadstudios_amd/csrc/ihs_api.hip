@@ -338,8 +338,24 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   launch_groups(s, n, K, h->tb, sv, nullptr, NBR_DIAG_HS * diag_scale);
   if (h->tb.ggroups != h->tb.groups) launch_groups(s, n, K, h->tb, sv, nullptr, GROUP_DIAG_HS * diag_scale, true);
   prof_mark(h, IHS_STAGE_TREE);
-  launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags, sc.counters);
-  prof_mark(h, IHS_STAGE_NBR);
+  // One traversal for gravity and the neighbour lists when the walk and neighbour groups coincide: the walk opens
+  // the nodes the neighbour search needs and writes the lists (gravity does not depend on h, so it runs first).
+  const bool fused = h->cfg.fused_walk && h->tb.ggroups == h->tb.groups && h->tb.group_max <= GROUP_MAX;
+  if (fused) {
+    NbrIo io;
+    io.hs = sc.hs;
+    io.nbmin = h->tb.nbmin;
+    io.nbr4 = (uint4*)sc.nbr;
+    io.ncount = sc.ncount;
+    io.cap = h->cfg.nbr_capacity;
+    io.stride = n;
+    launch_gravity_walk(s, n, K, kahan, sc.xm, h->tb, h->tb.ggroups, h->tb.nggroups, sv, sc.accg, sc.flags,
+                        sc.counters, need_pot, h->zero_soft, &io);
+    prof_mark(h, IHS_STAGE_GRAVITY);
+  } else {
+    launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags, sc.counters);
+    prof_mark(h, IHS_STAGE_NBR);
+  }
   if (newton) {
     const float eta3 = eta * eta * eta;
     if (record) mlp_record_features(s, h->mlp, n, D, sc.eos, sv, eta);
@@ -370,9 +386,11 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   launch_forces(s, kahan, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, sc.nmask, n,
                 (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph);
   prof_mark(h, IHS_STAGE_FORCES);
-  launch_gravity_walk(s, n, K, kahan, sc.xm, h->tb, h->tb.ggroups, h->tb.nggroups, sv, sc.accg, sc.flags,
-                      sc.counters, need_pot, h->zero_soft);
-  prof_mark(h, IHS_STAGE_GRAVITY);
+  if (!fused) {
+    launch_gravity_walk(s, n, K, kahan, sc.xm, h->tb, h->tb.ggroups, h->tb.nggroups, sv, sc.accg, sc.flags,
+                        sc.counters, need_pot, h->zero_soft);
+    prof_mark(h, IHS_STAGE_GRAVITY);
+  }
   h->force_evals++;
   h->have_fe = true;
   h->pos_fresh = true;
@@ -477,6 +495,7 @@ void ihs_default_config(ihs_config* c) {
   c->group_max = 64;
   c->quadrupole = 1;
   c->reuse_positions = 0;
+  c->fused_walk = 1;
 }
 
 int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
@@ -488,7 +507,8 @@ int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
   if (c.capacity < 1 || !valid_leaf(c.leaf_size) || c.nbr_capacity < 1 || c.h_mode < 0 || c.h_mode > 2 ||
       c.precision < 0 || c.precision > 1 || !(c.theta >= 0.f) || !(c.h_margin >= 1.0f) ||
       !(c.h_margin_seeded >= 1.0f) || c.group_max < c.leaf_size || c.group_max > 2 * GROUP_MAX ||
-      c.quadrupole < 0 || c.quadrupole > 1 || c.reuse_positions < 0 || c.reuse_positions > 1)
+      c.quadrupole < 0 || c.quadrupole > 1 || c.reuse_positions < 0 || c.reuse_positions > 1 || c.fused_walk < 0 ||
+      c.fused_walk > 1)
     return IHS_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IHS_ERR_HIP;

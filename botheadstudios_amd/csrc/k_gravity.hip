@@ -29,6 +29,7 @@ struct WalkLds {
   uint32_t leaf[GW_RING];
   float4 part[64];
   uint32_t pidx[64];
+  uint32_t phs2[64];  // with neighbour lists: the staged particles' search radius squared (float bits)
 };
 
 // Padding entry of the source rings: massless and far from every particle, so that 0 * 1/r stays 0 even without
@@ -104,13 +105,19 @@ __device__ __forceinline__ void interact_quad(const float4 s, const float4 qa, c
 // SELF: compare source indices to skip the particle itself. This is needed only without softening: with
 // softening the self term contributes zero force and a known potential -m/eps, which is removed at the end.
 // POT: accumulate the potential. QUAD: accepted cells act with their quadrupoles too.
-template <bool KAHAN, bool SELF, int T, bool POT, bool QUAD>
+// NBR: also write the group's neighbour lists (one target per lane; the neighbour build's contract: every j with
+// r_ij^2 < max(hs_i^2, hs_j^2), j != i, in 16-byte chunks nbr4[chunk * stride + t]). A node passing the
+// neighbour search's box test is opened instead of accepted, so every candidate leaf reaches the leaf path, where
+// the candidate's distance is the one its gravity term computes.
+template <bool KAHAN, bool SELF, int T, bool POT, bool QUAD, bool NBR>
 __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uint32_t lane, const float4* __restrict__ xm,
                                            int64_t n, int K, int64_t L, const int2* __restrict__ child,
                                            const float4* __restrict__ ncom, const float4* __restrict__ nbmax,
                                            const float4* __restrict__ nqa, const float2* __restrict__ nqb,
                                            float soft2, float4* __restrict__ accg, uint32_t* __restrict__ flags,
-                                           uint64_t& n_cell, uint64_t& n_pp, uint64_t& n_nodes) {
+                                           uint64_t& n_cell, uint64_t& n_pp, uint64_t& n_nodes, const NbrIo& io,
+                                           uint64_t& ncand, uint64_t& npairs) {
+  static_assert(!NBR || T == 1, "neighbour lists are written by walks with one target per lane");
   const int64_t t0 = grp.x;
   int64_t t[T];
   bool valid[T];
@@ -137,6 +144,12 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
     A[q].k.zero();
     A[q].kpot = A[q].cpot = 0.f;
   }
+  // Neighbour-list state (NBR): this lane's search radius, the group's largest, the 4-entry staging register.
+  const float myhs = NBR && valid[0] ? io.hs[t[0]] : 0.f;
+  const uint32_t myhs2u = __float_as_uint(myhs * myhs);
+  const float ghs = NBR ? wave_max(myhs) : 0.f;
+  uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, nb = 0, nchunk = 0, cntn = 0;
+  const uint32_t cap4 = (uint32_t)io.cap >> 2;
 
   const uint32_t leaf0 = (uint32_t)(L - 1);
   const uint32_t lpc = 64u / (uint32_t)K;
@@ -203,23 +216,70 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
     const bool ok = have && src < n;
     wave_sync();
     S.part[lane] = ok ? xm[src] : kPadSource;
-    if (SELF) S.pidx[lane] = ok ? (uint32_t)src : IHS_NONE;
+    if (SELF || NBR) S.pidx[lane] = ok ? (uint32_t)src : IHS_NONE;
+    if (NBR) {
+      const float ph = ok ? io.hs[src] : 0.f;
+      S.phs2[lane] = __float_as_uint(ph * ph);
+    }
     wave_sync();
     float ax[T], ay[T], az[T], pp[T];
 #pragma unroll
     for (int q = 0; q < T; ++q) ax[q] = ay[q] = az[q] = pp[q] = 0.f;
     const uint32_t cnt = (min(64u, nl * (uint32_t)K) + 3u) & ~3u;  // padding slots are massless
-    for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
+    if (NBR) {
+      // The gravity term and the neighbour test of each staged particle; candidate k's hit is bit k of a per-lane
+      // mask (the loop is unrolled so the bit is a constant), and only the hits are visited afterwards.
+      uint32_t hlo = 0, hhi = 0;
 #pragma unroll
-      for (uint32_t e = 0; e < 4; ++e) {
-        const float4 c = S.part[k0 + e];
+      for (uint32_t k0 = 0; k0 < 64; k0 += 4) {
+        if (k0 < cnt) {
 #pragma unroll
-        for (int q = 0; q < T; ++q)
-          interact<POT>(c, me[q], soft2, SELF && S.pidx[k0 + e] == (uint32_t)t[q], ax[q], ay[q], az[q], pp[q]);
+          for (uint32_t e = 0; e < 4; ++e) {
+            const uint32_t k = k0 + e;
+            const float4 c = S.part[k];
+            interact<POT>(c, me[0], soft2, SELF && S.pidx[k] == (uint32_t)t[0], ax[0], ay[0], az[0], pp[0]);
+            const float dx = c.x - me[0].x, dy = c.y - me[0].y, dz = c.z - me[0].z;
+            const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            // r < max(hs_i, hs_j): non-negative floats order as their bit patterns.
+            const bool hit = __float_as_uint(r2) < max(myhs2u, S.phs2[k]);
+            if (k < 32) hlo |= hit ? (1u << k) : 0u;
+            else hhi |= hit ? (1u << (k - 32)) : 0u;
+          }
+        }
       }
-    }
+      acc_add<KAHAN>(A[0], ax[0], ay[0], az[0], pp[0]);
+      ncand += nvalid * cnt;
+      uint64_t hits = valid[0] ? ((uint64_t)hhi << 32) | hlo : 0ull;
+      const uint32_t tt = (uint32_t)t[0];
+      while (hits) {
+        const uint32_t k = (uint32_t)__builtin_ctzll(hits);
+        hits &= hits - 1;
+        const uint32_t j = S.pidx[k];
+        if (j == tt) continue;
+        b0 = b1;
+        b1 = b2;
+        b2 = b3;
+        b3 = j;
+        ++cntn;
+        if (++nb == 4) {
+          if (nchunk < cap4) io.nbr4[(int64_t)nchunk * io.stride + t[0]] = make_uint4(b0, b1, b2, b3);
+          ++nchunk;
+          nb = 0;
+        }
+      }
+    } else {
+      for (uint32_t k0 = 0; k0 < cnt; k0 += 4) {
 #pragma unroll
-    for (int q = 0; q < T; ++q) acc_add<KAHAN>(A[q], ax[q], ay[q], az[q], pp[q]);
+        for (uint32_t e = 0; e < 4; ++e) {
+          const float4 c = S.part[k0 + e];
+#pragma unroll
+          for (int q = 0; q < T; ++q)
+            interact<POT>(c, me[q], soft2, SELF && S.pidx[k0 + e] == (uint32_t)t[q], ax[q], ay[q], az[q], pp[q]);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < T; ++q) acc_add<KAHAN>(A[q], ax[q], ay[q], az[q], pp[q]);
+    }
     lhead += nl;
     n_pp += nvalid * (uint64_t)__popcll(__ballot(ok));
     wave_sync();
@@ -247,6 +307,16 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
       // Without softening a leaf touching the group box may hold a target itself: it takes the
       // particle-particle path, which skips the self pair.
       acc = d2 >= rc * rc && (!SELF || node < leaf0 || d2 > 0.f);
+      if (NBR && acc) {
+        // The neighbour build's box test (group box against the node box grown by the larger search radius): a
+        // node that passes is opened, and its leaves come to the leaf path.
+        const float4 c0 = io.nbmin[node], c1 = nbmax[node];
+        const float ex = fmaxf(fmaxf(c0.x - bx1, bx0 - c1.x), 0.f);
+        const float ey = fmaxf(fmaxf(c0.y - by1, by0 - c1.y), 0.f);
+        const float ez = fmaxf(fmaxf(c0.z - bz1, bz0 - c1.z), 0.f);
+        const float R = fmaxf(ghs, c0.w);
+        if (ex * ex + ey * ey + ez * ez < R * R) acc = false;
+      }
     }
     const bool isleaf = live && !acc && node >= leaf0;
     const bool open = live && !acc && node < leaf0;
@@ -308,6 +378,18 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
   }
   while (ltail != lhead) eval_leaves(min(lpc, ltail - lhead));
 
+  if (NBR) {
+    // A partial last chunk holds its nb entries in b[4-nb..3]: move them to the front, padding with the last.
+    if (valid[0] && nb > 0 && nchunk < cap4)
+      io.nbr4[(int64_t)nchunk * io.stride + t[0]] = nb == 1 ? make_uint4(b3, b3, b3, b3)
+                                                  : nb == 2 ? make_uint4(b2, b3, b3, b3) : make_uint4(b1, b2, b3, b3);
+    const uint32_t capn = cap4 * 4u;
+    if (valid[0]) io.ncount[t[0]] = min(cntn, capn);
+    const uint64_t over = __ballot(valid[0] && cntn > capn);
+    npairs += wave_sum((unsigned long long)(valid[0] ? min(cntn, capn) : 0u));
+    if (lane == 0 && over) atomicAdd(&flags[FLAG_NBR_OVERFLOW], (uint32_t)__popcll(over));
+  }
+
 #pragma unroll
   for (int q = 0; q < T; ++q) {
     if (!valid[q]) continue;
@@ -326,7 +408,7 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
 // Persistent walk: every wave pulls groups from its XCD's queue (a contiguous eighth of the groups, so that
 // XCD's L2 serves neighbouring groups), then helps the other queues; irregular groups therefore overlap
 // instead of leaving the machine idle behind the slowest one.
-template <bool KAHAN, bool SELF, int T, bool POT, bool QUAD>
+template <bool KAHAN, bool SELF, int T, bool POT, bool QUAD, bool NBR>
 __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __restrict__ xm, int64_t n, int K,
                                                                  int64_t L, const int2* __restrict__ child,
                                                                  const float4* __restrict__ ncom,
@@ -337,14 +419,14 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
                                                                  const uint32_t* __restrict__ ngroups,
                                                                  uint32_t* __restrict__ wq, SegView sv,
                                                                  float4* __restrict__ accg, uint32_t* __restrict__ flags,
-                                                                 unsigned long long* __restrict__ counters) {
+                                                                 unsigned long long* __restrict__ counters, NbrIo io) {
   __shared__ WalkLds<QUAD> lds[GW_WAVES];
   const int w = threadIdx.x >> 6;
   const uint32_t lane = threadIdx.x & 63;
   WalkLds<QUAD>& S = lds[w];
   const uint32_t ng = *ngroups;
   const uint32_t home = blockIdx.x & 7u;
-  uint64_t n_cell = 0, n_pp = 0, n_nodes = 0;
+  uint64_t n_cell = 0, n_pp = 0, n_nodes = 0, ncand = 0, npairs = 0;
   for (uint32_t r = 0; r < 8; ++r) {
     const uint32_t q = (home + r) & 7u;
     const uint32_t lo = (uint32_t)((uint64_t)ng * q / 8u), hi = (uint32_t)((uint64_t)ng * (q + 1u) / 8u);
@@ -355,14 +437,18 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
       if (g >= hi) break;
       const int4 grp = groups[g];
       const float soft2 = sv.soft2[sv.of(grp.x)];
-      walk_group<KAHAN, SELF, T, POT, QUAD>(S, grp, lane, xm, n, K, L, child, ncom, nbmax, nqa, nqb, soft2, accg,
-                                            flags, n_cell, n_pp, n_nodes);
+      walk_group<KAHAN, SELF, T, POT, QUAD, NBR>(S, grp, lane, xm, n, K, L, child, ncom, nbmax, nqa, nqb, soft2,
+                                                 accg, flags, n_cell, n_pp, n_nodes, io, ncand, npairs);
     }
   }
   if (lane == 0) {
     cnt_add(counters, CNT_GRAV_CELL, n_cell);
     cnt_add(counters, CNT_GRAV_PP, n_pp);
     cnt_add(counters, CNT_WALK_NODES, n_nodes);
+    if (NBR) {
+      cnt_add(counters, CNT_PAIRS, npairs);
+      cnt_add(counters, CNT_NBR_CAND, ncand);
+    }
   }
 }
 
@@ -507,29 +593,32 @@ void launch_acc_out(hipStream_t s, const uint32_t* perm, int64_t n, const float4
 
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
                          const int4* groups, const uint32_t* ngroups, const SegView& sv, float4* accg, uint32_t* flags,
-                         unsigned long long* counters, bool pot, bool self) {
+                         unsigned long long* counters, bool pot, bool self, const NbrIo* nbr) {
   const int64_t L = (n + K - 1) / K;
   const bool two = tb.group_max > GROUP_MAX;
   const bool quad = tb.quadrupole != 0;
-  static unsigned nblk[32] = {0};
-  const int variant = (quad ? 16 : 0) + (kahan ? 8 : 0) + (self ? 4 : 0) + (two ? 2 : 0) + (pot ? 1 : 0);
-#define IHS_WALK(KA, SE, T, PO, QU)                                                                           \
-  do {                                                                                                        \
-    if (!nblk[variant])                                                                                       \
-      nblk[variant] = persistent_blocks((const void*)k_gravity_walk<KA, SE, T, PO, QU>, GW_WAVES * 64);       \
-    k_gravity_walk<KA, SE, T, PO, QU><<<nblk[variant], GW_WAVES * 64, 0, s>>>(                                \
-        xm, n, K, L, tb.child, tb.ncom, tb.nbmax, tb.nqa, tb.nqb, groups, ngroups, tb.wq, sv, accg,             \
-        flags, counters);                                                                                     \
+  const bool lists = nbr != nullptr && !two;
+  const NbrIo io = lists ? *nbr : NbrIo{};
+  static unsigned nblk[64] = {0};
+  const int variant = (lists ? 32 : 0) + (quad ? 16 : 0) + (kahan ? 8 : 0) + (self ? 4 : 0) + (two ? 2 : 0) + (pot ? 1 : 0);
+#define IHS_WALK(KA, SE, T, PO, QU, NB)                                                                         \
+  do {                                                                                                          \
+    if (!nblk[variant])                                                                                         \
+      nblk[variant] = persistent_blocks((const void*)k_gravity_walk<KA, SE, T, PO, QU, NB>, GW_WAVES * 64);     \
+    k_gravity_walk<KA, SE, T, PO, QU, NB><<<nblk[variant], GW_WAVES * 64, 0, s>>>(                              \
+        xm, n, K, L, tb.child, tb.ncom, tb.nbmax, tb.nqa, tb.nqb, groups, ngroups, tb.wq, sv, accg,               \
+        flags, counters, io);                                                                                   \
   } while (0)
-#define IHS_WALK_P(KA, SE, T, QU)           \
-  do {                                      \
-    if (pot) IHS_WALK(KA, SE, T, true, QU); \
-    else IHS_WALK(KA, SE, T, false, QU);    \
+#define IHS_WALK_P(KA, SE, T, QU, NB)           \
+  do {                                          \
+    if (pot) IHS_WALK(KA, SE, T, true, QU, NB); \
+    else IHS_WALK(KA, SE, T, false, QU, NB);    \
   } while (0)
-#define IHS_WALK_T(KA, SE, QU)                   \
-  do {                                           \
-    if (two) IHS_WALK_P(KA, SE, 2, QU);          \
-    else IHS_WALK_P(KA, SE, 1, QU);              \
+#define IHS_WALK_T(KA, SE, QU)                     \
+  do {                                             \
+    if (two) IHS_WALK_P(KA, SE, 2, QU, false);     \
+    else if (lists) IHS_WALK_P(KA, SE, 1, QU, true); \
+    else IHS_WALK_P(KA, SE, 1, QU, false);         \
   } while (0)
 #define IHS_WALK_Q(KA, SE)                       \
   do {                                           \

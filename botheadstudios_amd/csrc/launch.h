@@ -149,9 +149,21 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
 // Persistent: waves pull groups from per-XCD queues (tb.wq[0..8)).
 // Groups come from groups/ngroups (tb.ggroups); pot = false skips the potential (accg.w = 0).
 // self: some segment has zero softening (the walk then skips self pairs by index).
+// Neighbour lists written by the gravity walk (a walk of the same groups; one target per lane): the walk opens
+// every node whose box passes the neighbour search's box test, and its near leaves, taken one particle at a time
+// with their indices, are tested r^2 < max(hs_i^2, hs_j^2) as the neighbour build does.
+struct NbrIo {
+  const float* hs = nullptr;
+  const float4* nbmin = nullptr;  // node box min and largest search radius
+  uint4* nbr4 = nullptr;
+  uint32_t* ncount = nullptr;
+  int cap = 0;
+  int64_t stride = 0;
+};
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
                          const int4* groups, const uint32_t* ngroups, const SegView& sv, float4* accg, uint32_t* flags,
-                         unsigned long long* counters, bool pot, bool self);
+                         unsigned long long* counters, bool pot, bool self,
+                         const NbrIo* nbr = nullptr);
 // FP64 direct-sum accelerations (G included) at the particles with the given ids, from the state's positions
 // in storage order (accuracy probe of the walk), each over its own scenario with its softening. inv needs n
 // entries, tpos nt.

@@ -51,7 +51,8 @@ class GpuSph:
                  h_mode: str = "newton", theta: float = 0.5, cfl: float = 0.1, n_ngb: float = 48.0,
                  h_tol: float = 1e-4, h_max_iter: int = 40, h_margin: float = 1.25, h_cap_factor: float = 4.0,
                  nbr_capacity: int = 192, use_graph: bool = True, h_margin_seeded: float = 1.03,
-                 group_max: int = 64, quadrupole: bool = True, reuse_positions: bool = False):
+                 group_max: int = 64, quadrupole: bool = True, reuse_positions: bool = False,
+                 fused_walk: bool = True):
         self._L = _lib.load()
         cfg = default_config()
         cfg.capacity = int(capacity)
@@ -64,6 +65,7 @@ class GpuSph:
         cfg.h_margin_seeded, cfg.group_max = float(h_margin_seeded), int(group_max)
         cfg.quadrupole = int(bool(quadrupole))
         cfg.reuse_positions = int(bool(reuse_positions))
+        cfg.fused_walk = int(bool(fused_walk))
         self.config = cfg
         h = C.c_void_p()
         rc = self._L.ihs_create(int(device), C.byref(cfg), C.byref(h))

@@ -75,7 +75,10 @@ typedef struct {
                               evaluation saw, keeps that evaluation's tree, neighbours, h, density and gravity and
                               recomputes only the EOS and the velocity-dependent SPH forces (0: recompute all, as
                               the reference does, sph_step.wgsl:213-237) */
-  int32_t reserved[3];
+  int32_t fused_walk;    /* 1: the gravity walk also writes the neighbour lists (one traversal of the tree per
+                            evaluation instead of two; the same neighbour sets, listed in walk order), when walk and
+                            neighbour groups coincide (group_max 64); 0: a separate neighbour build (default 1) */
+  int32_t reserved[2];
 } ihs_config;
 
 typedef struct ihs_handle ihs_handle;
