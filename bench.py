@@ -30,6 +30,7 @@ FLOPS_PER_INTERACTION = 20.0  # softened monopole / particle-particle interactio
 # 3 sub, 3 FMA r^2, rsqrt, 5 mul for M r^-k, 15 for q.d, 5 for d.q.d, 3 for the radial factor, 12 for the
 # accumulation) plus 4 for the potential, which only the second evaluation of a step computes: 52 on average.
 FLOPS_PER_QUAD_CELL = 52.0
+FUSED_WALK = True  # the gravity walk writes the neighbour lists (ihs_config.fused_walk, the library default)
 
 
 def parse():
@@ -92,7 +93,7 @@ def measured_traffic(n, args, variant):
     rec = json.loads(line[len("traffic: "):])
     same = (rec.get("n") == n and rec.get("leaf") == args.leaf and rec.get("group") == args.group and
             abs(rec.get("theta", -1) - args.theta) < 1e-6 and rec.get("quadrupole") == bool(args.quadrupole) and
-            rec.get("variant") == variant)
+            rec.get("variant") == variant and rec.get("fused_walk", False) == FUSED_WALK)
     return rec["bytes_per_launch"] if same else None
 
 
@@ -289,7 +290,8 @@ def main():
         "data": "synthetic (seeded jittered-lattice two-body impact)",
         "config": {"workload": f"two-body giant impact, N={ic.n}, 0.9+0.1 Earth masses, 45 deg, 1.1 v_esc; "
                                f"KDK step = 2 force evaluations; variant={head}",
-                   "n": ic.n, "leaf": args.leaf, "group": args.group, "theta": args.theta, "quadrupole": args.quadrupole, "n_ngb": 48, "h_tol": 1e-4, "variant": head},
+                   "n": ic.n, "leaf": args.leaf, "group": args.group, "theta": args.theta, "quadrupole": args.quadrupole, "n_ngb": 48, "h_tol": 1e-4, "variant": head,
+                   "fused_walk": FUSED_WALK},
         "roofline": roofline(res, ic.n, args, head),
         "variants": {k: {"value": v["value"], "ms_per_step": v["ms_per_step"],
                          "mean_h_evals": v["stats"]["mean_h_evals"], "mean_h_iters": v["stats"]["mean_h_iters"],

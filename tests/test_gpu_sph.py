@@ -19,12 +19,14 @@ def _oracle_state(parts, eos, gravity_mode=1):
                         eos, SOFT, gravity_mode=gravity_mode, cs_fd=1, brute=True)
 
 
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "separate"])
 @pytest.mark.parametrize("precision", ["fp32", "split"])
-def test_sph_verify_force_eval(precision):
+def test_sph_verify_force_eval(precision, fused):
     """sph-verify:304-343: density max rel, acc RMS < 3e-2, du/dt RMS < 5e-2 (measured 5.6e-7 / 1.9e-6 /
-    4.4e-6 on the reference's GPU); here bounded at 1e-5 / 1e-4 / 1e-4."""
+    4.4e-6 on the reference's GPU); here bounded at 1e-5 / 1e-4 / 1e-4. Both neighbour paths: the lists written
+    by the gravity walk (fused) and by the separate neighbour build."""
     parts, eos = sph_verify_config(300)
-    with GpuSph(512, leaf_size=16, precision=precision, h_mode="fixed", theta=1e-6) as sph:
+    with GpuSph(512, leaf_size=16, precision=precision, h_mode="fixed", theta=1e-6, fused_walk=fused) as sph:
         sph.upload(parts, eos, SOFT)
         sph.force_eval()
         rho = sph.field("rho")
