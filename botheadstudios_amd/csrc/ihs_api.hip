@@ -384,7 +384,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
                  D.feat, sc.nmask, sc.flags, sv);
   prof_mark(h, IHS_STAGE_DENSITY);
   launch_forces(s, kahan, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, sc.nmask, n,
-                (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph);
+                (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph, feat ? D.feat : nullptr);
   prof_mark(h, IHS_STAGE_FORCES);
   if (!fused) {
     launch_gravity_walk(s, n, K, kahan, sc.xm, h->tb, h->tb.ggroups, h->tb.nggroups, sv, sc.accg, sc.flags,

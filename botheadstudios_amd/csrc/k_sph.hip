@@ -521,8 +521,9 @@ __global__ __launch_bounds__(256) void k_eos(int64_t n, const float* __restrict_
 }
 
 // Density, EOS and the Courant signal, one thread per particle.
-// vh holds (v, h_prev) from the permutation pass on entry (the feature pass reads the neighbours' velocities
-// from it, one 16-byte load each); its h is set here to the solved h.
+// vh holds (v, h_prev) from the permutation pass on entry; its h is set here to the solved h.
+// FEAT: also the network's first input, the neighbour count within h (feat.x); the force pass of the same
+// evaluation adds div v and |grad rho| (feat.y, feat.z) from its pair terms, so no pass here gathers velocities.
 // mask (optional): the support mask of each list, bit k set when entry k is inside the kernel support
 // (r < h_ij), the pairs the force pass sums; entries between h_ij and the searched radius are then not fetched
 // again. The words are assembled in LDS (thread-private) and written once, after the loop.
@@ -545,7 +546,6 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
     const uint32_t cnt = ncount[t];
     const float hinv_i = 1.0f / hi;
     float rho = me.w * (8.0f / kPI32) * hinv_i * hinv_i * hinv_i;
-    float dv = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
     uint32_t nn = 0, ent = 0;
     const uint32_t nwords = (cnt + 31u) >> 5;
     uint32_t* my = &smask[threadIdx.x];
@@ -554,10 +554,8 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
     struct Src {
       float4 q;
       float hj;
-      float4 vj;
     };
-    for_neighbours(nbr4, stride, t, cnt,
-                   [&](uint32_t j) { return Src{xm[j], h[j], FEAT ? vh[j] : make_float4(0.f, 0.f, 0.f, 0.f)}; },
+    for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return Src{xm[j], h[j]}; },
                    [&](uint32_t, const Src src, bool ok) {
       const float4 q = src.q;
       const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
@@ -570,14 +568,6 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
         float w, dw;
         spline_wdw(r * hinv, hinv, w, dw);
         rho += q.w * w;
-        if (FEAT && r > 1.0e-9f) {
-          dw *= __builtin_amdgcn_rcpf(r);
-          const float ddx = dx * dw, ddy = dy * dw, ddz = dz * dw;
-          gx += q.w * ddx;
-          gy += q.w * ddy;
-          gz += q.w * ddz;
-          dv += q.w * ((vi.x - src.vj.x) * ddx + (vi.y - src.vj.y) * ddy + (vi.z - src.vj.z) * ddz);
-        }
         if (mask) atomicOr(&my[(ent >> 5) * 256], 1u << (ent & 31u));
       }
       ++ent;
@@ -591,7 +581,7 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
     rpc[t] = make_float4(rho, p / (rho * rho), c, p);
     vh[t].w = hi;
     rho_out[t] = rho;
-    if (FEAT) feat[t] = make_float4((float)nn, -dv / rho, sqrtf(gx * gx + gy * gy + gz * gz), 0.f);
+    if (FEAT) feat[t] = make_float4((float)nn, 0.f, 0.f, 0.f);
     const float sig = hi / fmaxf(c + sqrtf(vi.x * vi.x + vi.y * vi.y + vi.z * vi.z), 1.0f);
     key = f2key(sig);
     if (!(sig == sig) || !(rho == rho)) atomicAdd(&flags[FLAG_NONFINITE], 1u);
@@ -599,8 +589,8 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
   signal_reduce(key, t, n, valid, sv);
 }
 
-// DIVV: also form the SPH velocity divergence (the network's first input) from the same pair terms, for an
-// evaluation whose density pass was skipped.
+// DIVV: also form the network's pair-term inputs from the same pair terms (the support pairs, r < h_ij, in list
+// order): the SPH velocity divergence (feat.y) and |grad rho| (feat.z). The density pass wrote feat.x.
 template <bool KAHAN, bool DIVV>
 __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, const float4* __restrict__ vh,
                                                 const float4* __restrict__ rpc, const uint4* __restrict__ nbr4,
@@ -615,7 +605,7 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
   const float4 vi = vh[t];
   const float4 ri = rpc[t];
   const uint32_t cnt = ncount[t];
-  float ax = 0.f, ay = 0.f, az = 0.f, de = 0.f, dv = 0.f;
+  float ax = 0.f, ay = 0.f, az = 0.f, de = 0.f, dv = 0.f, gx = 0.f, gy = 0.f, gz = 0.f;
   Kahan3 ka;
   ka.zero();
   struct Src {
@@ -656,7 +646,12 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
     }
     const float vdw = dvx * fx + dvy * fy + dvz * fz;
     de += 0.5f * q.w * coeff * vdw;
-    if (DIVV) dv += q.w * vdw;
+    if (DIVV) {
+      gx += q.w * fx;
+      gy += q.w * fy;
+      gz += q.w * fz;
+      dv += q.w * vdw;
+    }
   };
   if (mask) for_support(nbr4, mask, stride, t, cnt, fetch, use);
   else for_neighbours(nbr4, stride, t, cnt, fetch, use);
@@ -666,7 +661,10 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
     az = ka.s[2];
   }
   asph[t] = make_float4(ax, ay, az, de);
-  if (DIVV) feat[t].y = -dv / ri.x;
+  if (DIVV) {
+    feat[t].y = -dv / ri.x;
+    feat[t].z = sqrtf(gx * gx + gy * gy + gz * gz);
+  }
 }
 
 inline unsigned g256(int64_t n) { return (unsigned)((n + 255) / 256); }

@@ -135,3 +135,50 @@ def test_recorded_rows_train_a_seed_that_saves_newton_work():
     assert np.median(rel) < 1e-4 and np.max(rel) < 2e-3, (np.median(rel), np.max(rel))
     resid, _ = oracle.hresidual(pl, ic.m, hl, ics.ETA_48)
     assert np.max(resid) <= 1.2e-4
+
+
+@pytest.mark.parametrize("fused", [True, False], ids=["fused", "separate"])
+def test_network_inputs_match_brute_force(fused):
+    """The per-particle inputs of the network, from an evaluation before a drift (feat = n_ngb, div v,
+    |grad rho|, 0), against an FP64 brute force on the device's own state with the oracle's spline derivative:
+    n_ngb = #{j != i : r_ij < h_i}; over the support pairs r_ij < h_ij = (h_i + h_j) / 2,
+    grad rho_i = sum_j m_j dW(r_ij, h_ij) x_ij / r_ij and div v_i = -(1 / rho_i) sum_j m_j v_ij . grad W_ij.
+    The density pass counts n_ngb; the force pass forms div v and |grad rho| from its pair terms. Sums in FP32 are
+    bounded per particle by 2e-5 of the sum of the terms' magnitudes; n_ngb may differ by one where an FP32 r^2
+    sits on h_i (at most 0.1% of particles)."""
+    ic = ics.config_ic(1, n=8192)
+    p = ic.particles48().copy()
+    rng = np.random.default_rng(5)
+    p["vel"] = (p["vel"].astype(np.float64) + rng.normal(scale=300.0, size=(len(p), 3))).astype(np.float32)
+    with GpuSph(ic.n, leaf_size=1, precision="fp32", h_mode="learned", fused_walk=fused) as s:
+        s.load_mlp(mlp.default_weights_path())
+        s.upload(p, ic.eos, ic.softening)
+        s.force_eval()
+        feat = s.field("feat").astype(np.float64)
+        h = s.field("h").astype(np.float64)
+        rho = s.field("rho").astype(np.float64)
+        x = s.field("pos64")
+        out = s.readback()
+    v = out["vel"].astype(np.float64)
+    m = out["mass"].astype(np.float64)
+    n = len(h)
+    pairs = oracle.grid_pairs(x, float(h.max()), radius=float(h.max()))
+    i, j = np.concatenate([pairs[:, 0], pairs[:, 1]]), np.concatenate([pairs[:, 1], pairs[:, 0]])
+    d = x[i] - x[j]
+    r = np.sqrt((d * d).sum(1))
+    nn = np.bincount(i[r < h[i]], minlength=n)
+    sup = (r < 0.5 * (h[i] + h[j])) & (r > 1e-9)
+    i, j, d, r = i[sup], j[sup], d[sup], r[sup]
+    hij = 0.5 * (h[i] + h[j])
+    dw = np.array([oracle.sph_dw(a, b) for a, b in zip(r, hij)])
+    gw = (m[j] * dw / r)[:, None] * d
+    grad = np.stack([np.bincount(i, gw[:, k], minlength=n) for k in range(3)], 1)
+    vdw = ((v[i] - v[j]) * gw).sum(1)
+    divv = -np.bincount(i, vdw, minlength=n) / rho
+    gscale = np.bincount(i, np.abs(gw).sum(1), minlength=n)
+    vscale = np.bincount(i, np.abs(vdw), minlength=n) / rho
+    assert np.abs(feat[:, 0] - nn).max() <= 1 and (feat[:, 0] != nn).mean() <= 1e-3
+    assert np.all(np.abs(feat[:, 1] - divv) <= 2e-5 * vscale + 1e-30)
+    assert np.all(np.abs(feat[:, 2] - np.sqrt((grad ** 2).sum(1))) <= 2e-5 * gscale + 1e-30)
+    assert np.all(feat[:, 3] == 0.0)
+    assert np.abs(divv).mean() > 1e-3 * vscale.mean() and np.sqrt((grad ** 2).sum(1)).mean() > 0.0
