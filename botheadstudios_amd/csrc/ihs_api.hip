@@ -170,6 +170,7 @@ int alloc_all(ihs_handle* h) {
   t.nb_cap = radix_tiles(C);
   t.group_max = h->cfg.group_max;
   HIP_TRY(h, dalloc(h, &t.bbox, 8));
+  HIP_TRY(h, dalloc(h, &t.bbox_part, 6 * BBOX_PARTS));
   for (int b = 0; b < 2; ++b) {
     HIP_TRY(h, dalloc(h, &t.keys[b], C));
     HIP_TRY(h, dalloc(h, &t.vals[b], C));

@@ -21,18 +21,13 @@ __device__ __forceinline__ uint32_t expand_bits(uint32_t v) {
   return v;
 }
 
-__global__ void k_bbox_init(uint32_t* bbox, int S) {
-  for (int i = threadIdx.x; i < 6 * S; i += blockDim.x) bbox[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
-}
-
 // bps blocks per segment; block b of segment s strides over the segment's storage range.
 template <class PT>
 __global__ __launch_bounds__(256) void k_bbox(const PT* __restrict__ px, const PT* __restrict__ py,
                                               const PT* __restrict__ pz, const uint32_t* __restrict__ start, int bps,
-                                              uint32_t* __restrict__ bboxes) {
+                                              uint32_t* __restrict__ part) {
   const int sg = blockIdx.x / bps, b = blockIdx.x % bps;
   const int64_t i0 = start[sg], i1 = start[sg + 1];
-  uint32_t* bbox = bboxes + 6 * sg;
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};
   for (int64_t i = i0 + (int64_t)b * blockDim.x + threadIdx.x; i < i1; i += (int64_t)bps * blockDim.x) {
     const uint32_t k[3] = {f2key((float)px[i]), f2key((float)py[i]), f2key((float)pz[i])};
@@ -59,9 +54,41 @@ __global__ __launch_bounds__(256) void k_bbox(const PT* __restrict__ px, const P
     const int q = threadIdx.x;
     uint32_t v = s[0][q];
     for (int ww = 1; ww < (int)(blockDim.x >> 6); ++ww) v = q < 3 ? min(v, s[ww][q]) : max(v, s[ww][q]);
-    if (q < 3) atomicMin(&bbox[q], v);
-    else atomicMax(&bbox[q], v);
+    part[6 * blockIdx.x + q] = v;
   }
+}
+
+// Folds the blocks' partial boxes into each segment's box (one workgroup; block b of k_bbox is segment b / bps).
+// Same-address atomics from every block of k_bbox would serialize in L2; here they are LDS atomics, and a wave whose
+// lanes share a segment folds its lanes first.
+__global__ __launch_bounds__(1024) void k_bbox_final(const uint32_t* __restrict__ part, int nbk, int bps, int S,
+                                                     uint32_t* __restrict__ bboxes) {
+  __shared__ uint32_t sb[6 * IHS_MAX_SEGMENTS];
+  for (int i = threadIdx.x; i < 6 * S; i += blockDim.x) sb[i] = (i % 6) < 3 ? 0xffffffffu : 0u;
+  __syncthreads();
+  for (int t0 = 0; t0 < nbk; t0 += blockDim.x) {
+    const int t = t0 + threadIdx.x;
+    const bool ok = t < nbk;
+    const int sg = ok ? t / bps : 0;
+    const int sg0 = __shfl(sg, 0, 64);
+    const bool uniform = __all(!ok || sg == sg0);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+      uint32_t v = ok ? part[6 * t + q] : (q < 3 ? 0xffffffffu : 0u);
+      if (uniform) {
+        v = q < 3 ? wave_min(v) : wave_max(v);
+        if ((threadIdx.x & 63) == 0) {
+          if (q < 3) atomicMin(&sb[6 * sg0 + q], v);
+          else atomicMax(&sb[6 * sg0 + q], v);
+        }
+      } else if (ok) {
+        if (q < 3) atomicMin(&sb[6 * sg + q], v);
+        else atomicMax(&sb[6 * sg + q], v);
+      }
+    }
+  }
+  __syncthreads();
+  for (int i = threadIdx.x; i < 6 * S; i += blockDim.x) bboxes[i] = sb[i];
 }
 
 template <class PT>
@@ -296,9 +323,12 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cc,
 // child to arrive (arrival counter with workgroup-scope acquire/release) as long as the node's leaf
 // range lies inside this block's leaves. A climb that reaches a node spanning several blocks records that
 // arrival instead. Phase 2 (one workgroup) replays the recorded arrivals up the remaining top nodes.
+
 constexpr int UPS_THREADS = 1024;
 constexpr int UPS_PER_THREAD = 4;
 constexpr int UPS_LEAVES = UPS_THREADS * UPS_PER_THREAD;  // leaves per block of the first phase
+constexpr int TOP_SLOTS = 8192;                          // top-phase arrival table in LDS (node ids)
+constexpr int TOP_PROBES = 32;                           // probes before a node counts in global memory
 
 // Mass-normalised traceless quadrupole q = Q / M (units m^2: Q itself overflows FP32 in SI units) merged about
 // the parent's centre of mass (parallel-axis shift): q = sum_c (m_c / M) (q_c + 3 s s^T - |s|^2 I),
@@ -316,16 +346,36 @@ __device__ __forceinline__ void quad_shift(float4& qa, float2& qb, const float4 
   qb.y += w * (qcb.y + 3.0f * sy * sz);
 }
 
+// One node's moments and boxes, as the node arrays hold them.
+struct NodeVals {
+  float4 c, b0, b1, qa;
+  float2 qb;
+};
+
+__device__ __forceinline__ NodeVals load_node(uint32_t x, const float4* ncom, const float4* nbmin, const float4* nbmax,
+                                              const float4* nqa, const float2* nqb) {
+  return NodeVals{ncom[x], nbmin[x], nbmax[x], nqa[x], nqb[x]};
+}
+
+__device__ __forceinline__ void store_node(uint32_t x, const NodeVals& v, float4* ncom, float4* nbmin, float4* nbmax,
+                                           float4* nqa, float2* nqb) {
+  ncom[x] = v.c;
+  nbmin[x] = v.b0;
+  nbmax[x] = v.b1;
+  nqa[x] = v.qa;
+  nqb[x] = v.qb;
+}
+
 // The moment arithmetic is not contracted into FMAs, so a node gets the same bits whichever kernel merges it
-// (batched and single scenarios cut the leaves into blocks at different places).
-__device__ __forceinline__ void node_merge(uint32_t p, const int2 ch, float4* ncom, float4* nbmin, float4* nbmax,
-                                           float4* nqa, float2* nqb, float inv_theta) {
+// (batched and single scenarios cut the leaves into blocks at different places). l and r are the left and right
+// children.
+__device__ __forceinline__ NodeVals merge_vals(const NodeVals& l, const NodeVals& r, float inv_theta) {
 #pragma clang fp contract(off)
-  const float4 cl = ncom[ch.x], cr = ncom[ch.y];
-  const float4 bl0 = nbmin[ch.x], br0 = nbmin[ch.y];
-  const float4 bl1 = nbmax[ch.x], br1 = nbmax[ch.y];
-  const float4 qla = nqa[ch.x], qra = nqa[ch.y];
-  const float2 qlb = nqb[ch.x], qrb = nqb[ch.y];
+  const float4 cl = l.c, cr = r.c;
+  const float4 bl0 = l.b0, br0 = r.b0;
+  const float4 bl1 = l.b1, br1 = r.b1;
+  const float4 qla = l.qa, qra = r.qa;
+  const float2 qlb = l.qb, qrb = r.qb;
   const float m = cl.w + cr.w;
   float4 c = make_float4(0.f, 0.f, 0.f, m);
   if (m > 0.f) {
@@ -346,17 +396,22 @@ __device__ __forceinline__ void node_merge(uint32_t p, const int2 ch, float4* nc
   const float sx = b1.x - b0.x, sy = b1.y - b0.y, sz = b1.z - b0.z;
   const float ex = 0.5f * (b0.x + b1.x) - c.x, ey = 0.5f * (b0.y + b1.y) - c.y, ez = 0.5f * (b0.z + b1.z) - c.z;
   b1.w = sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta + sqrtf(ex * ex + ey * ey + ez * ez);
-  ncom[p] = c;
-  nbmin[p] = b0;
-  nbmax[p] = b1;
-  nqa[p] = qa;
-  nqb[p] = qb;
+  return NodeVals{c, b0, b1, qa, qb};
+}
+
+// Parent p of `from`, whose values the caller holds: the sibling is loaded and the children merged in order.
+__device__ __forceinline__ NodeVals merge_with_sibling(const NodeVals& cur, uint32_t from, const int2 ch,
+                                                       const float4* ncom, const float4* nbmin, const float4* nbmax,
+                                                       const float4* nqa, const float2* nqb, float inv_theta) {
+  const bool left = (uint32_t)ch.x == from;
+  const NodeVals sib = load_node(left ? (uint32_t)ch.y : (uint32_t)ch.x, ncom, nbmin, nbmax, nqa, nqb);
+  return left ? merge_vals(cur, sib, inv_theta) : merge_vals(sib, cur, inv_theta);
 }
 
 // Leaf moments of leaf c (K particles) into the node arrays.
-__device__ __forceinline__ void leaf_moments(int64_t c, const float4* __restrict__ xm, const float* __restrict__ hs,
-                                             int64_t n, int K, int64_t L, float4* ncom, float4* nbmin, float4* nbmax,
-                                             float4* nqa, float2* nqb, float inv_theta) {
+__device__ __forceinline__ NodeVals leaf_moments(int64_t c, const float4* __restrict__ xm, const float* __restrict__ hs,
+                                                 int64_t n, int K, int64_t L, float4* ncom, float4* nbmin,
+                                                 float4* nbmax, float4* nqa, float2* nqb, float inv_theta) {
   const int64_t start = c * K, end = min(start + (int64_t)K, n);
   // The centre of mass is formed relative to the leaf's first particle: a one-particle leaf's is then exactly
   // that particle's position (m x / m is not, in FP32), so a walk never sees a particle's own leaf as a source
@@ -390,20 +445,21 @@ __device__ __forceinline__ void leaf_moments(int64_t c, const float4* __restrict
       quad_shift(qa, qb, b, make_float4(0.f, 0.f, 0.f, 0.f), make_float2(0.f, 0.f), cm, b.w * im);
     }
   }
-  const uint32_t li = (uint32_t)(L - 1 + c);
-  ncom[li] = cm;
-  nqa[li] = qa;
-  nqb[li] = qb;
-  nbmin[li] = make_float4(lo[0], lo[1], lo[2], hm);
   const float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
   const float ex = 0.5f * (lo[0] + hi[0]) - cm.x, ey = 0.5f * (lo[1] + hi[1]) - cm.y, ez = 0.5f * (lo[2] + hi[2]) - cm.z;
-  nbmax[li] = make_float4(hi[0], hi[1], hi[2],
-                          sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta + sqrtf(ex * ex + ey * ey + ez * ez));
+  const NodeVals v{cm, make_float4(lo[0], lo[1], lo[2], hm),
+                   make_float4(hi[0], hi[1], hi[2],
+                               sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta + sqrtf(ex * ex + ey * ey + ez * ez)),
+                   qa, qb};
+  store_node((uint32_t)(L - 1 + c), v, ncom, nbmin, nbmax, nqa, nqb);
+  return v;
 }
 
 // Phase 1. Arrival counters of the block's internal nodes live in LDS (node i of the block at i - blo: a node
-// whose leaf range lies inside the block has its index there); a node's parent, children and range are
-// fetched before its counter, so each level of a climb costs one LDS atomic and one round of global loads.
+// whose leaf range lies inside the block has its index there). Pass j takes leaves blo + j * UPS_THREADS + t, so
+// each wave's leaf loads and node stores are contiguous. A climb keeps the node it merged in registers and loads
+// only the sibling; the next level's parent, children and range (read-only here) are fetched before the arrival
+// counter.
 __global__ __launch_bounds__(UPS_THREADS) void k_upsweep_local(const float4* __restrict__ xm, const float* __restrict__ hs,
                                                                int64_t n, int K, int64_t L, const int2* __restrict__ child,
                                                                const int2* __restrict__ range,
@@ -420,21 +476,35 @@ __global__ __launch_bounds__(UPS_THREADS) void k_upsweep_local(const float4* __r
   for (int j = 0; j < UPS_PER_THREAD; ++j) {
     const int64_t c = blo + (int64_t)j * UPS_THREADS + threadIdx.x;
     if (c >= L) break;
-    leaf_moments(c, xm, hs, n, K, L, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
-    uint32_t p = parent[L - 1 + c];
-    while (p != IHS_NONE) {
-      const int2 rg = range[p];
-      const int2 ch = child[p];
-      const uint32_t pp = parent[p];
+    NodeVals cur = leaf_moments(c, xm, hs, n, K, L, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
+    uint32_t from = (uint32_t)(L - 1 + c);
+    uint32_t p = parent[from];
+    if (p == IHS_NONE) continue;
+    int2 rg = range[p], ch = child[p];
+    uint32_t pp = parent[p];
+    for (;;) {
       if (rg.x < blo || rg.y >= bhi) {  // top node: phase 2 replays this arrival
         pend[atomicAdd(npend, 1u)] = p;
         break;
       }
+      int2 rg2 = make_int2(0, 0), ch2 = make_int2(0, 0);
+      uint32_t pp2 = IHS_NONE;
+      if (pp != IHS_NONE) {
+        rg2 = range[pp];
+        ch2 = child[pp];
+        pp2 = parent[pp];
+      }
       const uint32_t old =
           __hip_atomic_fetch_add(&lready[p - blo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
       if (old == 0u) break;
-      node_merge(p, ch, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
+      cur = merge_with_sibling(cur, from, ch, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
+      store_node(p, cur, ncom, nbmin, nbmax, nqa, nqb);
+      if (pp == IHS_NONE) break;
+      from = p;
       p = pp;
+      rg = rg2;
+      ch = ch2;
+      pp = pp2;
     }
   }
 }
@@ -448,23 +518,61 @@ __global__ __launch_bounds__(256) void k_tree_rearm(int64_t L, uint32_t* __restr
   if (i < L - 1) ready[i] = 0u;
 }
 
-// Each thread replays recorded arrivals; every node above the blocks is merged once, by its second arrival.
+// Arrival at node q in the top phase's table: true for the second arrival. The first arrival at q inserts q with
+// one compare-and-swap per probe (linear probing from a multiplicative hash); the second finds it. Slots are never
+// freed, so a node whose probe run is full for its first arrival is full for its second too, and both count in
+// ready[q] instead.
+__device__ __forceinline__ bool top_arrive(uint32_t* keys, uint32_t q, uint32_t* ready) {
+  uint32_t slot = (q * 2654435761u) >> (32 - 13);
+  static_assert(TOP_SLOTS == 1 << 13, "the hash yields 13 bits");
+  for (int k = 0; k < TOP_PROBES; ++k) {
+    uint32_t expected = IHS_NONE;
+    if (__hip_atomic_compare_exchange_strong(&keys[slot], &expected, q, __ATOMIC_ACQ_REL, __ATOMIC_ACQUIRE,
+                                             __HIP_MEMORY_SCOPE_WORKGROUP))
+      return false;  // inserted: first arrival
+    if (expected == q) return true;
+    slot = (slot + 1u) & (TOP_SLOTS - 1);
+  }
+  return __hip_atomic_fetch_add(&ready[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP) != 0u;
+}
+
+// Each thread replays recorded arrivals; every node above the blocks is merged once, by its second arrival, found
+// in an LDS table of node ids. As in phase 1 a climb holds its merged node and prefetches the next level's topology.
 __global__ __launch_bounds__(1024) void k_upsweep_top(const uint32_t* __restrict__ pend,
                                                       const uint32_t* __restrict__ npend, const int2* __restrict__ child,
                                                       const uint32_t* __restrict__ parent, uint32_t* ready,
                                                       float4* ncom, float4* nbmin, float4* nbmax, float4* nqa,
                                                       float2* nqb, float inv_theta, const uint32_t* __restrict__ gate) {
   if (gate && *gate == 0u) return;
+  __shared__ uint32_t keys[TOP_SLOTS];
+  for (int i = threadIdx.x; i < TOP_SLOTS; i += blockDim.x) keys[i] = IHS_NONE;
+  __syncthreads();
   const uint32_t np = *npend;
   for (uint32_t e = threadIdx.x; e < np; e += blockDim.x) {
     uint32_t q = pend[e];
-    while (q != IHS_NONE) {
-      const int2 ch = child[q];
-      const uint32_t qp = parent[q];
-      const uint32_t old = __hip_atomic_fetch_add(&ready[q], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0u) break;
-      node_merge(q, ch, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
+    int2 ch = child[q];
+    uint32_t qp = parent[q];
+    NodeVals cur;
+    uint32_t from = IHS_NONE;
+    for (;;) {
+      int2 ch2 = make_int2(0, 0);
+      uint32_t qp2 = IHS_NONE;
+      if (qp != IHS_NONE) {
+        ch2 = child[qp];
+        qp2 = parent[qp];
+      }
+      if (!top_arrive(keys, q, ready)) break;
+      if (from == IHS_NONE)
+        cur = merge_vals(load_node((uint32_t)ch.x, ncom, nbmin, nbmax, nqa, nqb),
+                         load_node((uint32_t)ch.y, ncom, nbmin, nbmax, nqa, nqb), inv_theta);
+      else
+        cur = merge_with_sibling(cur, from, ch, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
+      store_node(q, cur, ncom, nbmin, nbmax, nqa, nqb);
+      if (qp == IHS_NONE) break;
+      from = q;
       q = qp;
+      ch = ch2;
+      qp = qp2;
     }
   }
 }
@@ -615,16 +723,21 @@ inline unsigned grid_for(int64_t n, int b) { return (unsigned)((n + b - 1) / b);
 
 void launch_bbox_morton(hipStream_t s, int split, const void* px, const void* py, const void* pz, int64_t n,
                         TreeBufs& tb, const SegView& sv) {
-  k_bbox_init<<<1, 256, 0, s>>>(sv.bbox, sv.S);
-  const int bps = (int)std::max<int64_t>(1, std::min<int64_t>(1024 / sv.S, (n / sv.S + 255) / 256));
+  // bps * S <= BBOX_PARTS blocks (S <= IHS_MAX_SEGMENTS == BBOX_PARTS).
+  const int bps = (int)std::max<int64_t>(1, std::min<int64_t>(BBOX_PARTS / sv.S, (n / sv.S + 255) / 256));
   const unsigned nbk = (unsigned)(bps * sv.S);
   uint32_t* codes0 = sv.S > 1 ? tb.codes0 : nullptr;
+  if (split)
+    k_bbox<double><<<nbk, 256, 0, s>>>((const double*)px, (const double*)py, (const double*)pz, sv.start, bps,
+                                       tb.bbox_part);
+  else
+    k_bbox<float><<<nbk, 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, sv.start, bps,
+                                      tb.bbox_part);
+  k_bbox_final<<<1, 1024, 0, s>>>(tb.bbox_part, (int)nbk, bps, sv.S, sv.bbox);
   if (split) {
-    k_bbox<double><<<nbk, 256, 0, s>>>((const double*)px, (const double*)py, (const double*)pz, sv.start, bps, sv.bbox);
     k_morton<double><<<grid_for(n, 256), 256, 0, s>>>((const double*)px, (const double*)py, (const double*)pz, n,
                                                       sv.bbox, sv.segof, tb.keys[0], tb.vals[0], codes0);
   } else {
-    k_bbox<float><<<nbk, 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, sv.start, bps, sv.bbox);
     k_morton<float><<<grid_for(n, 256), 256, 0, s>>>((const float*)px, (const float*)py, (const float*)pz, n,
                                                      sv.bbox, sv.segof, tb.keys[0], tb.vals[0], codes0);
   }

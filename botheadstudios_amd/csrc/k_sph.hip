@@ -484,7 +484,10 @@ __device__ __forceinline__ void signal_reduce(uint32_t key, int64_t t, int64_t n
     __syncthreads();
     if (threadIdx.x == 0) {
       key = min(min(skey[0], skey[1]), min(skey[2], skey[3]));
-      if (key != 0xffffffffu) atomicMin(&sv.clk[sg0].signal, key);
+      // Most blocks cannot lower the minimum: a plain read first keeps them off the one address (a stale read
+      // only costs an atomic that changes nothing).
+      if (key != 0xffffffffu && key < __hip_atomic_load(&sv.clk[sg0].signal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+        atomicMin(&sv.clk[sg0].signal, key);
     }
   } else if (valid && key != 0xffffffffu) {
     atomicMin(&sv.clk[sv.of(t)].signal, key);

@@ -263,6 +263,9 @@ template <class PT>
 __global__ __launch_bounds__(256) void k_assemble(int64_t t0, int64_t t1, MasterBufs S, const double* __restrict__ pl,
                                                   uint32_t* __restrict__ minh) {
   const int64_t t = t0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  // The smallest h of the wave first, so one lane per wave meets the single address.
+  const uint32_t hk = wave_min(t < t1 ? f2key(S.h[t]) : 0xffffffffu);
+  if ((threadIdx.x & 63) == 0 && hk != 0xffffffffu) atomicMin(minh, hk);
   if (t >= t1) return;
   const double* q = pl + 12 * S.prov[t];
   PT* px = (PT*)S.px;
@@ -276,7 +279,6 @@ __global__ __launch_bounds__(256) void k_assemble(int64_t t0, int64_t t1, Master
   S.vx[t] = (float)(q[6] + (q[10] * l2 - l1 * q[11]));
   S.vy[t] = (float)(q[7] + (q[11] * l0 - l2 * q[9]));
   S.vz[t] = (float)(q[8] + (q[9] * l1 - l0 * q[10]));
-  atomicMin(minh, f2key(S.h[t]));
 }
 
 // Largest provenance over storage positions [t0, t1).

@@ -4,6 +4,8 @@
 #include <stdint.h>
 #include <algorithm>
 
+#include "ihs_sph.h"
+
 // Error/diagnostic counters kept on device (indices into Workspace::flags).
 enum FlagIndex : int {
   FLAG_NBR_OVERFLOW = 0,   // particles whose neighbour list exceeded its capacity
@@ -68,6 +70,7 @@ struct TreeBufs {
   int nb_cap = 0;    // radix tile capacity
   int group_max = 64;  // gravity walk-group size limit (K <= group_max <= 2 * GROUP_MAX: up to 2 targets per lane)
   uint32_t* bbox = nullptr;  // 6 order-preserving keys
+  uint32_t* bbox_part = nullptr;  // 6 BBOX_PARTS: per-block partial boxes of the bbox pass
   uint32_t* keys[2] = {nullptr, nullptr};
   uint32_t* vals[2] = {nullptr, nullptr};
   uint32_t* hist = nullptr;  // 256 * nb
@@ -77,8 +80,8 @@ struct TreeBufs {
   int2* range = nullptr;     // L-1 (first, last leaf)
   uint32_t* parent = nullptr;  // 2L-1
   uint32_t* ready = nullptr;   // L-1
-  uint32_t* pend = nullptr;    // 2L: upsweep arrivals at nodes above the 1024-leaf blocks
-  uint32_t* npend = nullptr;   // 1
+  uint32_t* pend = nullptr;    // 2L: upsweep arrivals at nodes above the upsweep's leaf blocks
+  uint32_t* npend = nullptr;   // 1: arrivals recorded in pend
   float4* ncom = nullptr;   // 2L-1: com xyz, mass
   float4* nbmin = nullptr;  // 2L-1: bmin xyz, max search radius
   float4* nbmax = nullptr;  // 2L-1: bmax xyz, critical radius |bmax-bmin|/theta + |centre-com|
@@ -101,6 +104,8 @@ struct TreeBufs {
   uint32_t* codes0 = nullptr;  // n: Morton codes by storage position (the segment pass re-gathers them)
 };
 
+constexpr int BBOX_PARTS = 1024;          // blocks of the bbox pass (their partial boxes fold in one workgroup)
+static_assert(IHS_MAX_SEGMENTS <= BBOX_PARTS, "every segment needs a bbox block");
 constexpr int NMASK_WORDS = 8;           // support masks cover neighbour lists of up to 256 entries
 constexpr int GROUP_MAX = 64;            // one wavefront: the largest walk group
 constexpr float GROUP_DIAG_HS = 6.0f;     // a gravity group's box diagonal is at most this many search radii (at
