@@ -16,6 +16,32 @@
 
 namespace {
 
+constexpr int FWIN_HALO = 128;                  // force pass: particles staged on each side of the block
+constexpr int FWIN = 256 + 2 * FWIN_HALO;        // window size (blocks of 256)
+constexpr int DWIN_HALO = 256;                  // density and first h-solve pass
+constexpr int DWIN = 256 + 2 * DWIN_HALO;
+
+// A block's Morton neighbourhood of positions staged in LDS: neighbours inside [lo, lo + n) are read there, the
+// others from global memory (n == 0: no window). The compiler makes the two-way read one flat load on a selected
+// address; lanes inside the window are served by LDS and issue no L2 request. (Forcing a separate LDS read and
+// global load per branch was slower: the branches then serialize and fewer loads are in flight.)
+struct XWin {
+  const float4* w;
+  uint32_t lo, n;
+};
+
+__device__ __forceinline__ float4 win_xm(const XWin& W, const float4* __restrict__ xm, uint32_t j) {
+  const uint32_t o = j - W.lo;
+  return o < W.n ? W.w[o] : xm[j];
+}
+
+// Window bounds of block t0 (blocks of 256) with `halo` particles on each side, clipped to [0, n).
+__device__ __forceinline__ void win_bounds(int64_t t0, int64_t n, int halo, int64_t& lo, int64_t& hi) {
+  lo = t0 > halo ? t0 - halo : 0;
+  hi = min(t0 + 256 + (int64_t)halo, n);
+}
+
+
 constexpr int NB_WAVES = 4;
 constexpr int NB_STACK = 512;
 constexpr int NB_SMALL = 8;     // a subtree of at most this many leaves that passes the box test is taken whole
@@ -284,15 +310,16 @@ __device__ __forceinline__ void spline_wdw(float q, float hinv, float& w, float&
 }
 
 __device__ __forceinline__ float own_density(const float4 me, float hh, const uint4* __restrict__ nbr4, int64_t stride,
-                                             int64_t t, uint32_t cnt, const float4* __restrict__ xm, float& hdrho,
-                                             uint32_t& nngb) {
+                                             int64_t t, uint32_t cnt, const float4* __restrict__ xm, const XWin& W,
+                                             float& hdrho, uint32_t& nngb) {
   const float hinv = 1.0f / hh;
   const float sig = (8.0f / kPI32) * hinv * hinv * hinv;
   float rho = me.w * sig;
   float hd = -3.0f * rho;
   const float h2 = hh * hh;
   uint32_t c = 0;
-  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return xm[j]; }, [&](uint32_t, const float4 q, bool ok) {
+  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return win_xm(W, xm, j); },
+                 [&](uint32_t, const float4 q, bool ok) {
     const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
     if (ok && r2 < h2) {
@@ -329,12 +356,13 @@ struct HsolveState {
 
 __device__ __forceinline__ void hsolve_loop(HsolveState& st, const float4 me, float hsi, uint32_t max_evals,
                                             const uint4* __restrict__ nbr4, int64_t stride, int64_t t, uint32_t cnt,
-                                            const float4* __restrict__ xm, float eta3, float tol, int max_iter) {
+                                            const float4* __restrict__ xm, const XWin& W, float eta3, float tol,
+                                            int max_iter) {
   float hh = st.h;
   for (;;) {
     float hd;
     uint32_t nn;
-    const float rho = own_density(me, hh, nbr4, stride, t, cnt, xm, hd, nn);
+    const float rho = own_density(me, hh, nbr4, stride, t, cnt, xm, W, hd, nn);
     ++st.evals;
     const float hinv = 1.0f / hh;
     const float target = me.w * eta3 * hinv * hinv * hinv;
@@ -406,7 +434,14 @@ __global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, 
                                                  uint32_t* __restrict__ npend, uint32_t* __restrict__ flags,
                                                  unsigned long long* __restrict__ counters) {
   if (gate && *gate == 0u) return;
-  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  __shared__ float4 wx[DWIN];
+  const int64_t t0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+  int64_t wlo, whi;
+  win_bounds(t0, n, DWIN_HALO, wlo, whi);
+  for (int64_t k = wlo + threadIdx.x; k < whi; k += blockDim.x) wx[k - wlo] = xm[k];
+  __syncthreads();
+  const XWin W{wx, (uint32_t)wlo, (uint32_t)(whi - wlo)};
+  const int64_t t = t0 + threadIdx.x;
   const bool valid = t < n;
   HsolveState st{0.f, 0.f, INFINITY, 0u, 0u, HS_DONE};
   float hsi = 0.f, hcap = 0.f;
@@ -414,7 +449,7 @@ __global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, 
     hsi = hs[t];
     hcap = sv.hcap[sv.of(t)];
     st.h = fminf(h[t], hsi);
-    hsolve_loop(st, xm[t], hsi, 1u, nbr4, stride, t, ncount[t], xm, eta3, tol, max_iter);
+    hsolve_loop(st, xm[t], hsi, 1u, nbr4, stride, t, ncount[t], xm, W, eta3, tol, max_iter);
     h[t] = st.h;
   }
   const bool more = valid && st.state == HS_PENDING;
@@ -461,7 +496,8 @@ __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, 
       st.h = h[t];
       st.lo = br.x;
       st.hi = br.y;
-      hsolve_loop(st, xm[t], hsi, 0xffffffffu, nbr4, stride, t, ncount[t], xm, eta3, tol, max_iter);
+      hsolve_loop(st, xm[t], hsi, 0xffffffffu, nbr4, stride, t, ncount[t], xm, XWin{nullptr, 0u, 0u}, eta3, tol,
+                  max_iter);
       h[t] = st.h;
       iters[t] = (uint8_t)min(st.it, 255u);
     }
@@ -539,7 +575,18 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
                                                  float4* __restrict__ feat, uint32_t* __restrict__ mask,
                                                  uint32_t* __restrict__ flags, SegView sv) {
   __shared__ uint32_t smask[NMASK_WORDS * 256];
-  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  __shared__ float4 wx[DWIN];
+  __shared__ float wh[DWIN];
+  const int64_t t0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+  int64_t wlo, whi;
+  win_bounds(t0, n, DWIN_HALO, wlo, whi);
+  for (int64_t k = wlo + threadIdx.x; k < whi; k += blockDim.x) {
+    wx[k - wlo] = xm[k];
+    wh[k - wlo] = h[k];
+  }
+  __syncthreads();
+  const uint32_t wl = (uint32_t)wlo, wn = (uint32_t)(whi - wlo);
+  const int64_t t = t0 + threadIdx.x;
   const bool valid = t < n;
   uint32_t key = 0xffffffffu;
   if (valid) {
@@ -558,7 +605,12 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
       float4 q;
       float hj;
     };
-    for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return Src{xm[j], h[j]}; },
+    for_neighbours(nbr4, stride, t, cnt,
+                   [&](uint32_t j) {
+                     const uint32_t o = j - wl;
+                     if (o < wn) return Src{wx[o], wh[o]};
+                     return Src{xm[j], h[j]};
+                   },
                    [&](uint32_t, const Src src, bool ok) {
       const float4 q = src.q;
       const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
@@ -594,6 +646,10 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
 
 // DIVV: also form the network's pair-term inputs from the same pair terms (the support pairs, r < h_ij, in list
 // order): the SPH velocity divergence (feat.y) and |grad rho| (feat.z). The density pass wrote feat.x.
+// The block's Morton neighbourhood, particles [t0 - FWIN_HALO, t0 + 256 + FWIN_HALO), is staged in LDS with
+// coalesced loads; neighbours inside it are read from LDS and only the others are gathered from global memory
+// (each gathered lane costs its own L2 request, the bound of these passes). In Morton order about two thirds of
+// the pairs of the impact fall inside such a window.
 template <bool KAHAN, bool DIVV>
 __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, const float4* __restrict__ vh,
                                                 const float4* __restrict__ rpc, const uint4* __restrict__ nbr4,
@@ -601,7 +657,18 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
                                                 const uint32_t* __restrict__ mask, int64_t stride, int64_t n,
                                                 const float* __restrict__ av, float4* __restrict__ asph,
                                                 float4* __restrict__ feat) {
-  const int64_t t = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  __shared__ float4 wx[FWIN], wv[FWIN], wr[FWIN];
+  const int64_t t0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
+  int64_t wlo, whi;
+  win_bounds(t0, n, FWIN_HALO, wlo, whi);
+  for (int64_t k = wlo + threadIdx.x; k < whi; k += blockDim.x) {
+    wx[k - wlo] = xm[k];
+    wv[k - wlo] = vh[k];
+    wr[k - wlo] = rpc[k];
+  }
+  __syncthreads();
+  const uint32_t wl = (uint32_t)wlo, wn = (uint32_t)(whi - wlo);
+  const int64_t t = t0 + threadIdx.x;
   if (t >= n) return;
   const float alpha = av[0], beta = av[1];
   const float4 me = xm[t];
@@ -614,7 +681,11 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
   struct Src {
     float4 q, vj, rj;
   };
-  auto fetch = [&](uint32_t j) { return Src{xm[j], vh[j], rpc[j]}; };
+  auto fetch = [&](uint32_t j) {
+    const uint32_t o = j - wl;
+    if (o < wn) return Src{wx[o], wv[o], wr[o]};
+    return Src{xm[j], vh[j], rpc[j]};
+  };
   auto use = [&](uint32_t, const Src& src, bool ok) {
     const float4 q = src.q, vj = src.vj, rj = src.rj;
     const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
