@@ -75,6 +75,20 @@ __device__ __forceinline__ void interact(const float4 s, const float4 me, float 
   if (POT) pot -= mr;
 }
 
+// As interact, from the unsoftened r^2 the neighbour test has already formed (one add instead of three FMAs).
+template <bool POT = true>
+__device__ __forceinline__ void interact_r2(const float4 s, float dx, float dy, float dz, float r2u, float soft2,
+                                            bool self, float& ax, float& ay, float& az, float& pot) {
+  float rinv = __builtin_amdgcn_rsqf(r2u + soft2);
+  rinv = self ? 0.f : rinv;
+  const float mr = s.w * rinv;
+  const float mr3 = mr * rinv * rinv;
+  ax = fmaf(dx, mr3, ax);
+  ay = fmaf(dy, mr3, ay);
+  az = fmaf(dz, mr3, az);
+  if (POT) pot -= mr;
+}
+
 // Monopole + traceless quadrupole of a cell (com, M, mass-normalised q = Q/M) at d = com - x,
 // r^2 = |d|^2 + eps^2: a = M [d / r^3 - q d / r^5 + 5/2 (d.q.d) d / r^7], phi = -M [1/r + (d.q.d) / (2 r^5)]
 // (G applied later). M r^-k is formed by repeated multiplication so that no intermediate leaves FP32 range.
@@ -93,6 +107,9 @@ __device__ __forceinline__ void interact_quad(const float4 s, const float4 qa, c
   const float qdy = fmaf(qa.w, dx, fmaf(qa.y, dy, qb.y * dz));
   const float qdz = fmaf(qb.x, dx, fmaf(qb.y, dy, qa.z * dz));
   const float dqd = fmaf(dx, qdx, fmaf(dy, qdy, dz * qdz));
+  // Every product here is a power of 1/r times at most |d|^3 / r^3, so a far or padding source (mass 0 at
+  // 1e18 m) yields exact zeros. Folding the r^-7 term as M r^-5 [d (r^2 + 5/2 (d.q.d) / r^2) - q d] saves one
+  // multiply but forms |d| r^2, which overflows beyond ~7e12 m and turns 0 * inf into NaN.
   const float f = fmaf(2.5f * dqd, mr7, mr3);
   ax = fmaf(dx, f, fmaf(-qdx, mr5, ax));
   ay = fmaf(dy, f, fmaf(-qdy, mr5, ay));
@@ -237,9 +254,10 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
           for (uint32_t e = 0; e < 4; ++e) {
             const uint32_t k = k0 + e;
             const float4 c = S.part[k];
-            interact<POT>(c, me[0], soft2, SELF && S.pidx[k] == (uint32_t)t[0], ax[0], ay[0], az[0], pp[0]);
             const float dx = c.x - me[0].x, dy = c.y - me[0].y, dz = c.z - me[0].z;
             const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            interact_r2<POT>(c, dx, dy, dz, r2, soft2, SELF && S.pidx[k] == (uint32_t)t[0], ax[0], ay[0], az[0],
+                             pp[0]);
             // r < max(hs_i, hs_j): non-negative floats order as their bit patterns.
             const bool hit = __float_as_uint(r2) < max(myhs2u, S.phs2[k]);
             if (k < 32) hlo |= hit ? (1u << k) : 0u;
