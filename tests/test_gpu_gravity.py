@@ -152,3 +152,22 @@ def test_quadrupole_never_worse_on_the_cloud(cloud):
                 acc = sph.gravity(pos, mass, soft, theta=theta, mode="tree")
             r[quad], _ = rms_rel(acc, ref)
         assert r[True] < r[False], (theta, r)
+
+
+def test_tree_top_phase_beyond_its_lds_table():
+    """4,194,304 bodies, one per leaf (bh_cloud seed 7): 12,623 tree nodes cross the upsweep's 4,096-leaf blocks
+    (counted on the oracle's tree), more than the top phase's 8,192-entry LDS table holds, so part of the top tree
+    counts its arrivals in global memory. A lost or doubled merge there would leave wrong moments in the largest
+    nodes; the tree force at 256 targets agrees with the FP64 direct sum to the walk's accuracy at theta 0.5."""
+    n = 1 << 22
+    pos, mass = bh_cloud(n, seed=7)
+    f = GravityField(n, leaf_size=1, knee=1000)
+    try:
+        acc = f.accelerations(pos, mass, 5.0e3)
+    finally:
+        f.close()
+    assert np.all(np.isfinite(acc))
+    idx = np.linspace(0, n - 1, 256).astype(np.int64)
+    ref = oracle.direct_f64(pos, mass, 5.0e3, targets=idx)
+    rms, mx = rms_rel(acc[idx], ref)
+    assert rms < 1e-3, (rms, mx)
