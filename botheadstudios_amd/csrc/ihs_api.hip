@@ -233,9 +233,9 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &t.codes0, C));
   HIP_TRY(h, alloc_raw(h, &s.eos, 16 * sizeof(ihs_eos)));
   HIP_TRY(h, dalloc(h, &s.staging, C));
-  HIP_TRY(h, dalloc(h, &h->hw.pend, C));
+  HIP_TRY(h, dalloc(h, &h->hw.pend, C + PEND_SHARDS * 256));
   HIP_TRY(h, dalloc(h, &h->hw.bracket, C));
-  HIP_TRY(h, dalloc(h, &h->hw.npend, 1));
+  HIP_TRY(h, dalloc(h, &h->hw.npend, PEND_SHARDS));
   HIP_TRY(h, hipMemset(s.flags, 0, FLAG_COUNT * sizeof(uint32_t)));
   HIP_TRY(h, hipMemset(s.counters, 0, CNT_SLOTS * CNT_STRIDE * sizeof(unsigned long long)));
   HIP_TRY(h, hipMemset(s.nbr, 0, (size_t)C * h->cfg.nbr_capacity * sizeof(uint32_t)));

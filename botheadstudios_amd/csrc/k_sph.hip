@@ -418,21 +418,22 @@ __device__ __forceinline__ void hsolve_account(bool valid, const HsolveState& st
 
 __global__ void k_hsolve_begin(const uint32_t* __restrict__ gate, uint32_t* __restrict__ npend) {
   if (gate && *gate == 0u) return;
-  if (threadIdx.x == 0) *npend = 0u;
+  if (threadIdx.x < PEND_SHARDS) npend[threadIdx.x] = 0u;
 }
 
 // First pass: one density sum per particle at its seed h. Converged particles are done; the others take one
-// Newton update and join the pending list (wave-aggregated append, in particle order within a wave) with their
-// bracket, so that the second pass runs full waves of unconverged particles instead of every wave waiting for
-// its slowest lane.
+// Newton update and join a pending list (wave-aggregated append, in particle order within a wave; block b appends to
+// shard b mod PEND_SHARDS, whose entries start at shard * region) with their bracket, so that the second pass runs
+// full waves of unconverged particles instead of every wave waiting for its slowest lane. One shared counter took
+// an L2 atomic from every wave with a pending lane, one after another (140 us per seeded evaluation at 1M).
 __global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, const float* __restrict__ hs,
                                                  float* __restrict__ h, uint8_t* __restrict__ iters,
                                                  const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
                                                  int64_t stride, int64_t n, float eta3, float tol, int max_iter,
                                                  const uint32_t* __restrict__ gate, uint32_t* __restrict__ retry,
                                                  SegView sv, uint32_t* __restrict__ pend, float2* __restrict__ bracket,
-                                                 uint32_t* __restrict__ npend, uint32_t* __restrict__ flags,
-                                                 unsigned long long* __restrict__ counters) {
+                                                 uint32_t* __restrict__ npend, uint32_t region,
+                                                 uint32_t* __restrict__ flags, unsigned long long* __restrict__ counters) {
   if (gate && *gate == 0u) return;
   __shared__ float4 wx[DWIN];
   const int64_t t0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
@@ -455,9 +456,10 @@ __global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, 
   const bool more = valid && st.state == HS_PENDING;
   const uint64_t mm = __ballot(more);
   if (mm) {
+    const uint32_t shard = blockIdx.x & (PEND_SHARDS - 1);
     uint32_t base = 0;
-    if ((threadIdx.x & 63) == 0) base = atomicAdd(npend, (uint32_t)__popcll(mm));
-    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base);
+    if ((threadIdx.x & 63) == 0) base = atomicAdd(&npend[shard], (uint32_t)__popcll(mm));
+    base = (uint32_t)__builtin_amdgcn_readfirstlane((int)base) + shard * region;
     if (more) {
       pend[base + lanes_below(mm)] = (uint32_t)t;
       bracket[t] = make_float2(st.lo, st.hi);
@@ -467,7 +469,8 @@ __global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, 
   hsolve_account(valid, st, hsi, retry, hcap, flags, counters, true);
 }
 
-// Second pass over the pending list (persistent grid-stride loop): resumes each solve from its first update.
+// Second pass over the pending lists (persistent grid-stride loop over their concatenation): resumes each solve
+// from its first update.
 __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, const float* __restrict__ hs,
                                                  float* __restrict__ h, uint8_t* __restrict__ iters,
                                                  const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
@@ -475,10 +478,24 @@ __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, 
                                                  const uint32_t* __restrict__ gate, uint32_t* __restrict__ retry,
                                                  SegView sv, const uint32_t* __restrict__ pend,
                                                  const float2* __restrict__ bracket, const uint32_t* __restrict__ npend,
-                                                 uint32_t* __restrict__ flags,
+                                                 uint32_t region, uint32_t* __restrict__ flags,
                                                  unsigned long long* __restrict__ counters) {
   if (gate && *gate == 0u) return;
-  const uint32_t np = *npend;
+  // Offsets of the shards in the concatenation: soff[r] = sum of the lengths of shards below r.
+  __shared__ uint32_t soff[PEND_SHARDS + 1];
+  if (threadIdx.x < 64) {
+    static_assert(PEND_SHARDS == 64, "one wave scans the shard lengths");
+    uint32_t c = npend[threadIdx.x];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t y = __shfl_up(c, o, 64);
+      if ((int)threadIdx.x >= o) c += y;
+    }
+    soff[threadIdx.x + 1] = c;
+    if (threadIdx.x == 0) soff[0] = 0u;
+  }
+  __syncthreads();
+  const uint32_t np = soff[PEND_SHARDS];
   const uint32_t stride_g = gridDim.x * blockDim.x;
   // Every lane of a wave runs the same number of rounds so the wave-wide accounting stays converged.
   const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
@@ -489,7 +506,13 @@ __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, 
     float hsi = 0.f, hcap = 0.f;
     int64_t t = 0;
     if (valid) {
-      t = pend[k];
+      uint32_t lo = 0, hi = PEND_SHARDS;  // the shard r with soff[r] <= k < soff[r + 1]
+      while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (soff[mid] <= k) lo = mid;
+        else hi = mid;
+      }
+      t = pend[lo * region + (k - soff[lo])];
       hsi = hs[t];
       hcap = sv.hcap[sv.of(t)];
       const float2 br = bracket[t];
@@ -762,11 +785,13 @@ void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, 
                    const uint32_t* gate, uint32_t* retry) {
   static unsigned nblk2 = 0;
   if (!nblk2) nblk2 = persistent_blocks((const void*)k_hsolve2, 256);
+  // Shard r holds the pending particles of blocks r, r + PEND_SHARDS, ...: at most region entries.
+  const uint32_t region = (uint32_t)((g256(n) + PEND_SHARDS - 1) / PEND_SHARDS) * 256u;
   k_hsolve_begin<<<1, 64, 0, s>>>(gate, hw.npend);
   k_hsolve1<<<g256(n), 256, 0, s>>>(xm, hs, h, iters, (const uint4*)nbr, ncount, stride, n, eta3, tol, max_iter, gate,
-                                    retry, sv, hw.pend, hw.bracket, hw.npend, flags, counters);
+                                    retry, sv, hw.pend, hw.bracket, hw.npend, region, flags, counters);
   k_hsolve2<<<nblk2, 256, 0, s>>>(xm, hs, h, iters, (const uint4*)nbr, ncount, stride, eta3, tol, max_iter, gate, retry,
-                                  sv, hw.pend, hw.bracket, hw.npend, flags, counters);
+                                  sv, hw.pend, hw.bracket, hw.npend, region, flags, counters);
 }
 
 void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const void* u,

@@ -185,10 +185,13 @@ void launch_gravity_direct(hipStream_t s, int64_t n, float soft2, const float4* 
 void launch_nbr_build(hipStream_t s, int64_t n, int K, const float4* xm, const float* hs, const TreeBufs& tb,
                       uint32_t* nbr, uint32_t* ncount, int cap, int64_t stride, uint32_t* flags,
                       unsigned long long* counters, const uint32_t* gate = nullptr);
+// The first pass appends unconverged particles to PEND_SHARDS lists (block b to list b mod PEND_SHARDS, each in its
+// own region of pend), so that no single counter takes an atomic from every wave.
+constexpr int PEND_SHARDS = 64;
 struct HsolveScratch {
-  uint32_t* pend = nullptr;    // n: particles left unconverged by the first pass
-  float2* bracket = nullptr;   // n: their (lo, hi) after the first update
-  uint32_t* npend = nullptr;   // 1
+  uint32_t* pend = nullptr;    // n + PEND_SHARDS * 256: particles left unconverged by the first pass
+  float2* bracket = nullptr;   // n: their (lo, hi) after the first update, by particle
+  uint32_t* npend = nullptr;   // PEND_SHARDS list lengths
 };
 // retry != null: solves stopped by a search radius below hcap increment *retry instead of FLAG_H_CAPPED.
 void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, float* h, uint8_t* iters,
