@@ -354,6 +354,91 @@ struct HsolveState {
   uint32_t it, evals, state;
 };
 
+// Own-h density of particle t summed by a group of HS2_G lanes: lane `sub` takes chunks sub, sub + HS2_G, ... of
+// the list, and a butterfly over the group gives every lane of it the same totals (a + b == b + a in floating point),
+// so the group's lanes take the same Newton steps.
+constexpr int HS2_G = 8;
+__device__ __forceinline__ float own_density_group(const float4 me, float hh, const uint4* __restrict__ nbr4,
+                                                   int64_t stride, int64_t t, uint32_t cnt,
+                                                   const float4* __restrict__ xm, uint32_t sub, float& hdrho) {
+  const float hinv = 1.0f / hh;
+  const float sig = (8.0f / kPI32) * hinv * hinv * hinv;
+  const float h2 = hh * hh;
+  float rho = 0.f, hd = 0.f;
+  for (uint32_t c = sub; 4u * c < cnt; c += HS2_G) {
+    const uint4 q = nbr4[(int64_t)c * stride + t];
+    const uint32_t jj[4] = {q.x, q.y, q.z, q.w};
+    float4 src[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) src[e] = xm[jj[e]];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const float4 qv = src[e];
+      const float dx = qv.x - me.x, dy = qv.y - me.y, dz = qv.z - me.z;
+      const float r2 = dx * dx + dy * dy + dz * dz;
+      if (4u * c + (uint32_t)e < cnt && r2 < h2) {
+        const float qq = __builtin_amdgcn_sqrtf(r2) * hinv;
+        float wq, dq;
+        if (qq < 0.5f) {
+          wq = 1.0f - 6.0f * qq * qq + 6.0f * qq * qq * qq;
+          dq = -12.0f * qq + 18.0f * qq * qq;
+        } else {
+          const float tt = 1.0f - qq;
+          wq = 2.0f * tt * tt * tt;
+          dq = -6.0f * tt * tt;
+        }
+        const float wv = sig * wq;
+        rho += qv.w * wv;
+        hd -= qv.w * (3.0f * wv + qq * sig * dq);
+      }
+    }
+  }
+#pragma unroll
+  for (int o = 1; o < HS2_G; o <<= 1) {
+    rho += __shfl_xor(rho, o, 64);
+    hd += __shfl_xor(hd, o, 64);
+  }
+  const float self = me.w * sig;
+  hdrho = hd - 3.0f * self;
+  return rho + self;
+}
+
+// hsolve_loop for a group of HS2_G lanes sharing one particle (every lane of the group holds the same state).
+__device__ __forceinline__ void hsolve_loop_group(HsolveState& st, const float4 me, float hsi,
+                                                  const uint4* __restrict__ nbr4, int64_t stride, int64_t t,
+                                                  uint32_t cnt, const float4* __restrict__ xm, uint32_t sub,
+                                                  float eta3, float tol, int max_iter) {
+  float hh = st.h;
+  for (;;) {
+    float hd;
+    const float rho = own_density_group(me, hh, nbr4, stride, t, cnt, xm, sub, hd);
+    ++st.evals;
+    const float hinv = 1.0f / hh;
+    const float target = me.w * eta3 * hinv * hinv * hinv;
+    const float f = rho - target;
+    if (fabsf(f) <= tol * rho) {
+      st.state = HS_DONE;
+      break;
+    }
+    if ((int)st.it >= max_iter) {
+      st.state = HS_UNCONVERGED;
+      break;
+    }
+    if (f < 0.f) st.lo = hh;
+    else st.hi = hh;
+    if (f < 0.f && hh >= hsi) {
+      st.state = HS_CAPPED;
+      break;
+    }
+    const float hdf = hd + 3.0f * target;  // h * df/dh
+    float hn = hdf > 0.f ? hh - f * hh / hdf : -1.0f;
+    if (!(hn > st.lo && hn < st.hi)) hn = isinf(st.hi) ? 2.0f * hh : 0.5f * (st.lo + st.hi);
+    hh = fminf(hn, hsi);
+    ++st.it;
+  }
+  st.h = hh;
+}
+
 __device__ __forceinline__ void hsolve_loop(HsolveState& st, const float4 me, float hsi, uint32_t max_evals,
                                             const uint4* __restrict__ nbr4, int64_t stride, int64_t t, uint32_t cnt,
                                             const float4* __restrict__ xm, const XWin& W, float eta3, float tol,
@@ -496,11 +581,14 @@ __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, 
   }
   __syncthreads();
   const uint32_t np = soff[PEND_SHARDS];
-  const uint32_t stride_g = gridDim.x * blockDim.x;
-  // Every lane of a wave runs the same number of rounds so the wave-wide accounting stays converged.
-  const uint32_t base0 = blockIdx.x * blockDim.x + (threadIdx.x & ~63u);
-  for (uint32_t b = base0; b < np; b += stride_g) {
-    const uint32_t k = b + (threadIdx.x & 63u);
+  // HS2_G lanes per pending particle: 64 / HS2_G particles per wave and round. Every lane of a wave runs the same
+  // number of rounds so the wave-wide accounting stays converged.
+  constexpr uint32_t PER_WAVE = 64u / HS2_G;
+  const uint32_t lane = threadIdx.x & 63u, sub = lane % HS2_G;
+  const uint32_t wave = (blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) / 64u;
+  const uint32_t nwave = gridDim.x * blockDim.x / 64u;
+  for (uint32_t b = wave * PER_WAVE; b < np; b += nwave * PER_WAVE) {
+    const uint32_t k = b + lane / HS2_G;
     const bool valid = k < np;
     HsolveState st{0.f, 0.f, INFINITY, 1u, 0u, HS_DONE};
     float hsi = 0.f, hcap = 0.f;
@@ -519,13 +607,16 @@ __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, 
       st.h = h[t];
       st.lo = br.x;
       st.hi = br.y;
-      hsolve_loop(st, xm[t], hsi, 0xffffffffu, nbr4, stride, t, ncount[t], xm, XWin{nullptr, 0u, 0u}, eta3, tol,
-                  max_iter);
-      h[t] = st.h;
-      iters[t] = (uint8_t)min(st.it, 255u);
+      hsolve_loop_group(st, xm[t], hsi, nbr4, stride, t, ncount[t], xm, sub, eta3, tol, max_iter);
+      if (sub == 0) {
+        h[t] = st.h;
+        iters[t] = (uint8_t)min(st.it, 255u);
+      }
     }
-    st.it = valid ? st.it - 1u : 0u;  // the first update was counted by the first pass
-    hsolve_account(valid, st, hsi, retry, hcap, flags, counters, false);
+    const bool lead = valid && sub == 0;
+    st.it = lead ? st.it - 1u : 0u;  // the first update was counted by the first pass
+    if (!lead) st.evals = 0u;
+    hsolve_account(lead, st, hsi, retry, hcap, flags, counters, false);
   }
 }
 
