@@ -357,7 +357,7 @@ struct HsolveState {
 // Own-h density of particle t summed by a group of HS2_G lanes: lane `sub` takes chunks sub, sub + HS2_G, ... of
 // the list, and a butterfly over the group gives every lane of it the same totals (a + b == b + a in floating point),
 // so the group's lanes take the same Newton steps.
-constexpr int HS2_G = 8;
+constexpr int HS2_G = 4;
 __device__ __forceinline__ float own_density_group(const float4 me, float hh, const uint4* __restrict__ nbr4,
                                                    int64_t stride, int64_t t, uint32_t cnt,
                                                    const float4* __restrict__ xm, uint32_t sub, float& hdrho) {
