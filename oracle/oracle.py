@@ -103,6 +103,7 @@ def _declare(L):
     L.or_courant_dt.restype = d
     L.or_kdk.argtypes = [C.POINTER(OrState), d, d, i32]
     L.or_kdk.restype = d
+    L.or_relax.argtypes = [C.POINTER(OrState), d, d, i32]
     L.or_energy.argtypes = [C.POINTER(OrState), _dp]
     L.or_hsolve.argtypes = [_dp, _dp, i64, d, d, i32, d, _dp, C.c_void_p, C.c_void_p, C.c_int]
     L.or_hsolve.restype = i64
@@ -305,6 +306,10 @@ class Hydro:
 
     def kdk(self, steps, dt=0.0, cfl=0.1):
         return lib().or_kdk(C.byref(self.st), float(dt), float(cfl), int(steps))
+
+    def relax(self, steps, dt, damp):
+        """hydrostatic.rs:283-291 relax_step, `steps` times (AV off, u fixed)."""
+        lib().or_relax(C.byref(self.st), float(dt), float(damp), int(steps))
 
     def energy(self):
         out = np.zeros(9)

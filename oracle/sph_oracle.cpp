@@ -733,6 +733,28 @@ double or_kdk(or_state* s, double dt_fixed, double cfl, int32_t steps) {
   return t;
 }
 
+// ie/crates/engine/src/hydrostatic.rs:283-291 (relax_step) over `steps` steps: density, then the pressure-only
+// accelerations of hydrostatic.rs:258-281 (gravity + the symmetric SPH-EOS pressure term, no artificial
+// viscosity), then v = (v + a dt) damp, x += v dt. u is left unchanged (isothermal relaxation).
+void or_relax(or_state* s, double dt, double damp, int32_t steps) {
+  const int64_t n = s->n;
+  std::vector<double> rho(n), acc(3 * n), dudt(n);
+  const double alpha = s->av_alpha, beta = s->av_beta;
+  s->av_alpha = 0.0;
+  s->av_beta = 0.0;
+  for (int32_t st = 0; st < steps; ++st) {
+    or_density(s, rho.data());
+    or_forces(s, rho.data(), acc.data(), dudt.data());
+    for (int64_t i = 0; i < n; ++i)
+      for (int q = 0; q < 3; ++q) {
+        s->vel[3 * i + q] = (s->vel[3 * i + q] + acc[3 * i + q] * dt) * damp;
+        s->xyz[3 * i + q] += s->vel[3 * i + q] * dt;
+      }
+  }
+  s->av_alpha = alpha;
+  s->av_beta = beta;
+}
+
 // ie/crates/engine/src/gpu_sph.rs:628-649 (KE, IE, direct softened PE) plus momentum and angular
 // momentum about the origin (impact-run:528-536 takes L_z about the COM; callers that want that
 // pass COM-frame states).

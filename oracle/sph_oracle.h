@@ -27,8 +27,16 @@
  *                                       sphere potential -3GM^2/5R.
  *   or_grid_pairs                     — pair set == brute force (neighbors.rs:103-145).
  *   or_density / or_forces / or_kdk   — brute vs grid paths agree; momentum conservation of the
- *                                       symmetric pair forces; KDK reversibility at AV=0;
- *                                       sph-verify N=300 known-answer config runs.
+ *                                       symmetric pair forces (uniform and mixed h); KDK reversibility
+ *                                       at AV=0; sph-verify N=300 known-answer config runs; lattice-sum
+ *                                       identities for a linear pressure field and for uniform pressure
+ *                                       over 1/rho^2 linear; the two-particle P/rho^2 coefficient;
+ *                                       Monaghan's viscosity on one approaching pair (mean c, 0.01 h^2,
+ *                                       beta mu^2, heating); h_ij = (h_i+h_j)/2 in density and forces
+ *                                       (tests/test_oracle_sph_pins.py).
+ *   or_relax / or_kdk (physics)       — the reference's hydrostatic balance of a relaxed basalt sphere
+ *                                       (hydrostatic.rs:442-488) and head-on collision energy
+ *                                       conservation with shock heating (hydrostatic.rs:1082-1148).
  *   or_hsolve                         — NEW (no reference: the reference keeps h fixed,
  *                                       hydrostatic.rs:58-61). Parity unpinned by the reference;
  *                                       pinned by residual <= tol, bracketing/monotonicity,
@@ -114,6 +122,9 @@ double or_courant_dt(const or_state* s, const double* rho, double cfl);
 /* KDK steps (hydrostatic.rs:345-359 / sph-verify CpuState::step). dt > 0: fixed; dt <= 0: Courant
  * dt with cfl (impact-run loop: density -> dt -> step). Returns the total time advanced. */
 double or_kdk(or_state* s, double dt, double cfl, int32_t steps);
+/* Damped relaxation steps (hydrostatic.rs:283-291): density, gravity + pressure-only SPH accelerations (no
+ * artificial viscosity), v = (v + a dt) damp, x += v dt. */
+void or_relax(or_state* s, double dt, double damp, int32_t steps);
 /* FP64 energy (KE, IE, PE with direct O(N^2) softened PE), momentum and angular momentum about
  * the origin: out[0..8] = KE, IE, PE, px, py, pz, Lx, Ly, Lz. */
 void or_energy(const or_state* s, double* out);

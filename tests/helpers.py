@@ -65,6 +65,27 @@ def sph_verify_config(n: int = 300):
     return p, eos_table(["basalt", "iron"])
 
 
+def fib_dir(i: int, n: int, offset: float) -> np.ndarray:
+    """hydrostatic.rs:30-38: the i-th of n Fibonacci-sphere directions."""
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    y = 1.0 - 2.0 * (i + 0.5) / n
+    r = math.sqrt(max(1.0 - y * y, 0.0))
+    th = golden * i + offset
+    return np.array([math.cos(th) * r, y, math.sin(th) * r])
+
+
+def new_sphere(eos10, total_mass: float, temp_k: float, specific_heat: float, n: int) -> dict:
+    """hydrostatic.rs:64-88 HydroBody::new_sphere: one material, equal masses at equal-volume radii on a Fibonacci
+    spiral, h = 2 (m/rho0)^(1/3) (hydrostatic.rs:58-61), softening half the spacing, u = c T."""
+    rho0 = float(eos10[0])
+    r0 = (total_mass / (4.0 / 3.0 * math.pi * rho0)) ** (1.0 / 3.0)
+    m_i = total_mass / n
+    pos = np.array([fib_dir(i, n, 0.0) * (r0 * ((i + 0.5) / n) ** (1.0 / 3.0)) for i in range(n)])
+    return dict(pos=pos, vel=np.zeros((n, 3)), mass=np.full(n, m_i), u=np.full(n, specific_heat * temp_k),
+                h=np.full(n, 2.0 * (m_i / rho0) ** (1.0 / 3.0)), softening=0.5 * (m_i / rho0) ** (1.0 / 3.0),
+                mat=np.zeros(n, np.int32), r0=r0)
+
+
 def rms_rel(approx, exact):
     """gpu-bh-verify:164-181: sqrt(sum |err|^2 / sum |ref|^2) and the worst per-particle error."""
     approx = np.asarray(approx, np.float64)
