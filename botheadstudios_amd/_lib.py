@@ -73,7 +73,7 @@ EXPORTS = [
     "ihs_mlp_samples", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict", "ihs_direct_f64",
     "ihs_upload_batch", "ihs_segment_clock", "ihs_measure_impact", "ihs_disk_stats", "ihs_disk_clumps",
     "ihs_find_clumps", "ihs_body_bulk", "ihs_assemble", "ihs_checkpoint_bytes", "ihs_checkpoint_save",
-    "ihs_checkpoint_load",
+    "ihs_checkpoint_load", "ihs_begin_readback", "ihs_take_readback", "ihs_segment_energy",
 ]
 
 MLP_NPARAM = 1313
@@ -140,12 +140,17 @@ def load(path: str | None = None):
     L.ihs_checkpoint_save.argtypes = [vp, vp, i64]
     L.ihs_checkpoint_save.restype = i64
     L.ihs_checkpoint_load.argtypes = [vp, vp, i64]
+    L.ihs_begin_readback.argtypes = [vp]
+    L.ihs_take_readback.argtypes = [vp, vp, i64]
+    L.ihs_take_readback.restype = i64
+    L.ihs_segment_energy.argtypes = [vp, i32, vp]
     for name in ("ihs_create", "ihs_destroy", "ihs_upload", "ihs_set_dt", "ihs_set_av", "ihs_set_omega",
                  "ihs_step_kdk", "ihs_relax", "ihs_force_eval", "ihs_sync", "ihs_stats_get", "ihs_gravity",
                  "ihs_debug_field", "ihs_profile", "ihs_profile_read", "ihs_mlp_set_weights", "ihs_mlp_get_weights",
                  "ihs_mlp_record", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict", "ihs_direct_f64",
                  "ihs_upload_batch", "ihs_segment_clock", "ihs_measure_impact", "ihs_disk_stats", "ihs_disk_clumps",
-                 "ihs_find_clumps", "ihs_body_bulk", "ihs_assemble", "ihs_checkpoint_load"):
+                 "ihs_find_clumps", "ihs_body_bulk", "ihs_assemble", "ihs_checkpoint_load", "ihs_begin_readback",
+                 "ihs_segment_energy"):
         getattr(L, name).restype = C.c_int
     if path is None:
         _lib = L

@@ -102,6 +102,14 @@ struct ihs_handle {
   int64_t prof_calls[IHS_STAGE_COUNT] = {0};
   std::vector<void*> allocs;
   MeasureBufs mb;  // disk / remnant measurement and clump buffers, allocated on first use
+  // Two-phase readback (gpu_store.rs:126-152): ihs_begin_readback packs the state as the stream reaches it into
+  // snap_dev and copies it to the pinned snap_host, recording snap_ev; ihs_take_readback hands over that snapshot
+  // once the event has completed, whatever was queued after it. Allocated on the first begin.
+  SphParticle48* snap_dev = nullptr;
+  ihs_particle* snap_host = nullptr;
+  hipEvent_t snap_ev = nullptr;
+  int64_t snap_n = 0;
+  bool snap_pending = false;
   std::string err;
 };
 
@@ -539,6 +547,8 @@ int ihs_destroy(ihs_handle* h) {
   drop_graph(h);
   mlp_free(h->mlp);
   measure_release(h->mb);
+  if (h->snap_host) hipHostFree(h->snap_host);
+  if (h->snap_ev) hipEventDestroy(h->snap_ev);
   for (hipEvent_t e : h->prof_ev) hipEventDestroy(e);
   for (void* p : h->allocs) hipFree(p);
   if (h->stream) hipStreamDestroy(h->stream);
@@ -779,6 +789,39 @@ int64_t ihs_readback(ihs_handle* h, ihs_particle* out, int64_t cap, int32_t bloc
   return n;
 }
 
+int ihs_begin_readback(ihs_handle* h) {
+  if (!h) return IHS_ERR_ARG;
+  if (h->n < 1 || h->snap_pending) return IHS_OK;  // nothing to copy, or a snapshot in flight (gpu_store.rs:127-129)
+  hipSetDevice(h->device);
+  if (!h->snap_dev) {
+    HIP_TRY(h, dalloc(h, &h->snap_dev, h->cap));
+    HIP_TRY(h, hipHostMalloc((void**)&h->snap_host, (size_t)h->cap * sizeof(ihs_particle), hipHostMallocDefault));
+    HIP_TRY(h, hipEventCreateWithFlags(&h->snap_ev, hipEventDisableTiming));
+  }
+  // h->cur already names the buffer the state will be in once the queued steps have run.
+  launch_pack(h->stream, h->split, h->n, h->st[h->cur], h->snap_dev);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(h->snap_host, h->snap_dev, (size_t)h->n * sizeof(ihs_particle), hipMemcpyDeviceToHost,
+                            h->stream));
+  HIP_TRY(h, hipEventRecord(h->snap_ev, h->stream));
+  h->snap_n = h->n;
+  h->snap_pending = true;
+  return IHS_OK;
+}
+
+int64_t ihs_take_readback(ihs_handle* h, ihs_particle* out, int64_t cap) {
+  if (!h || !out || cap < 0) return IHS_ERR_ARG;
+  if (!h->snap_pending) return IHS_EAGAIN;  // idle: nothing begun (gpu_store.rs:147-152 returns None)
+  hipSetDevice(h->device);
+  const hipError_t q = hipEventQuery(h->snap_ev);
+  if (q == hipErrorNotReady) return IHS_EAGAIN;
+  if (q != hipSuccess) return fail(h, IHS_ERR_HIP, "readback: %s", hipGetErrorString(q));
+  const int64_t n = std::min(cap, h->snap_n);
+  std::memcpy(out, h->snap_host, (size_t)n * sizeof(ihs_particle));
+  h->snap_pending = false;
+  return n;
+}
+
 int64_t ihs_count(const ihs_handle* h) { return h ? h->n : IHS_ERR_ARG; }
 
 int ihs_sync(ihs_handle* h) {
@@ -839,6 +882,31 @@ int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset) {
     HIP_TRY(h, hipMemcpy(h->sc.flags, fl, sizeof fl, hipMemcpyHostToDevice));
     h->force_evals = 0;
   }
+  return IHS_OK;
+}
+
+int ihs_segment_energy(ihs_handle* h, int32_t seg, double* out9) {
+  if (!h || !out9) return IHS_ERR_ARG;
+  if (seg < 0 || seg >= h->nseg) return fail(h, IHS_ERR_ARG, "scenario %d out of range (%d uploaded)", seg, h->nseg);
+  if (!h->have_fe) return fail(h, IHS_ERR_STATE, "no force evaluation of the current state (the potential is unset)");
+  hipSetDevice(h->device);
+  const int64_t t0 = h->seg_start_host[seg], t1 = h->seg_start_host[seg + 1];
+  // The scenario's storage range [t0, t1): a view of the master state from t0.
+  const MasterBufs& S = h->st[h->cur];
+  const size_t pw = h->split ? 8 : 4;
+  MasterBufs V = S;
+  V.px = (char*)S.px + pw * t0;
+  V.py = (char*)S.py + pw * t0;
+  V.pz = (char*)S.pz + pw * t0;
+  V.u = (char*)S.u + pw * t0;
+  V.vx = S.vx + t0;
+  V.vy = S.vy + t0;
+  V.vz = S.vz + t0;
+  V.m = S.m + t0;
+  launch_energy(h->stream, h->split, t1 - t0, V, h->sc.accg + t0, h->sc.epart, h->sc.eout);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
+  HIP_TRY(h, hipMemcpy(out9, h->sc.eout, 9 * sizeof(double), hipMemcpyDeviceToHost));
   return IHS_OK;
 }
 

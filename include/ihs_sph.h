@@ -160,14 +160,29 @@ int ihs_step_kdk(ihs_handle* h, int32_t substeps);
 int ihs_relax(ihs_handle* h, int32_t steps);
 /* One force evaluation at the current state, no integration (for inspection). */
 int ihs_force_eval(ihs_handle* h);
-/* begin_readback/take_readback (gpu_store.rs:126-152). blocking=0 returns IHS_EAGAIN while the
- * stream is busy; otherwise the particle count written (<= cap), in upload order. */
+/* The current state in upload order (the particle count written, <= cap). blocking = 1 waits for the queued work;
+ * blocking = 0 returns IHS_EAGAIN while the stream is busy. For a frame loop that keeps queuing steps, use the
+ * two-phase pair below. */
 int64_t ihs_readback(ihs_handle* h, ihs_particle* out, int64_t cap, int32_t blocking);
+/* GpuSph::begin_readback (gpu_store.rs:126-145): enqueues, behind the work already queued, a copy of the state as
+ * it will then be into a library-owned pinned host buffer, and returns without waiting. A no-op when nothing is
+ * uploaded or a snapshot is already in flight, as in the reference. */
+int ihs_begin_readback(ihs_handle* h);
+/* GpuSph::take_readback (gpu_store.rs:147-152): the snapshot of the last ihs_begin_readback once its copy has
+ * completed (the particle count written, <= cap, in upload order), however many steps were queued after it;
+ * IHS_EAGAIN while the copy is pending or when no snapshot was begun (the reference's None). Never waits. */
+int64_t ihs_take_readback(ihs_handle* h, ihs_particle* out, int64_t cap);
 /* GpuSph::count (gpu_sph.rs:870). */
 int64_t ihs_count(const ihs_handle* h);
 int ihs_sync(ihs_handle* h);
-/* FP64 energy / momentum reductions and solver counters (blocks). reset=1 clears the counters. */
+/* FP64 energy / momentum reductions and solver counters (blocks). reset=1 clears the counters. The energies,
+ * momentum and angular momentum are totals over every scenario of a batch (ihs_segment_energy gives one
+ * scenario's); the clock fields are scenario 0's. */
 int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset);
+/* One scenario's FP64 sums (blocks), as gpu_sph.rs:624-649 total_energy and impact-run:528-536 over its
+ * particles: out9 = KE, IE, PE (1/2 sum m phi, the last evaluation's tree potential), p xyz, L xyz about the
+ * origin. Needs a force evaluation of the current state. */
+int ihs_segment_energy(ihs_handle* h, int32_t seg, double* out9);
 /* GravityField::accelerations / GpuGravity::accelerations[_tree] (gpu_gravity.rs:181-230,556-578):
  * softened self-gravity on n bodies (pos3 and acc3 are n*3 doubles). mode = IHS_GRAVITY_*;
  * pot (may be NULL) receives the potential. Synchronous. Uses the handle's device and leaf size. */
