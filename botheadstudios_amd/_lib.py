@@ -74,6 +74,7 @@ EXPORTS = [
     "ihs_upload_batch", "ihs_segment_clock", "ihs_measure_impact", "ihs_disk_stats", "ihs_disk_clumps",
     "ihs_find_clumps", "ihs_body_bulk", "ihs_assemble", "ihs_checkpoint_bytes", "ihs_checkpoint_save",
     "ihs_checkpoint_load", "ihs_begin_readback", "ihs_take_readback", "ihs_segment_energy",
+    "ihs_direct_f64_noalloc",
 ]
 
 MLP_NPARAM = 1313
@@ -144,13 +145,14 @@ def load(path: str | None = None):
     L.ihs_take_readback.argtypes = [vp, vp, i64]
     L.ihs_take_readback.restype = i64
     L.ihs_segment_energy.argtypes = [vp, i32, vp]
+    L.ihs_direct_f64_noalloc.argtypes = [vp, vp, i64, vp]
     for name in ("ihs_create", "ihs_destroy", "ihs_upload", "ihs_set_dt", "ihs_set_av", "ihs_set_omega",
                  "ihs_step_kdk", "ihs_relax", "ihs_force_eval", "ihs_sync", "ihs_stats_get", "ihs_gravity",
                  "ihs_debug_field", "ihs_profile", "ihs_profile_read", "ihs_mlp_set_weights", "ihs_mlp_get_weights",
                  "ihs_mlp_record", "ihs_mlp_train", "ihs_mlp_gradient", "ihs_mlp_predict", "ihs_direct_f64",
                  "ihs_upload_batch", "ihs_segment_clock", "ihs_measure_impact", "ihs_disk_stats", "ihs_disk_clumps",
                  "ihs_find_clumps", "ihs_body_bulk", "ihs_assemble", "ihs_checkpoint_load", "ihs_begin_readback",
-                 "ihs_segment_energy"):
+                 "ihs_segment_energy", "ihs_direct_f64_noalloc"):
         getattr(L, name).restype = C.c_int
     if path is None:
         _lib = L

@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -19,6 +20,35 @@
 namespace {
 
 constexpr float kPi = 3.14159265358979f;
+
+// Scratch of the allocation-free probe ihs_direct_f64_noalloc: per call, the target ids, their storage positions
+// and the FP64 accelerations, for at most kProbeMaxTargets targets (2 MiB per GPU, held until the process ends).
+// In host code `scratch` is a host-side stand-in, never a device address: each device's copy is looked up once with
+// hipGetSymbolAddress (device_scratch) and only that address is passed to kernels and copies.
+constexpr int64_t kProbeMaxTargets = 65536;
+constexpr size_t kProbeIdsOff = 0;
+constexpr size_t kProbeTposOff = 4 * kProbeMaxTargets;
+constexpr size_t kProbeAccOff = 8 * kProbeMaxTargets;
+constexpr size_t kProbeBytes = kProbeAccOff + 24 * kProbeMaxTargets;
+static __device__ __align__(256) unsigned char scratch[kProbeBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {nullptr};
+
+// The device address of `scratch` on the current device `device` (looked up once per device).
+hipError_t device_scratch(int device, void** out) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = p;
+  }
+  *out = g_scratch[device];
+  return hipSuccess;
+}
 
 struct Scratch {
   // sorted-order compute arrays
@@ -105,6 +135,7 @@ struct ihs_handle {
   // Two-phase readback (gpu_store.rs:126-152): ihs_begin_readback packs the state as the stream reaches it into
   // snap_dev and copies it to the pinned snap_host, recording snap_ev; ihs_take_readback hands over that snapshot
   // once the event has completed, whatever was queued after it. Allocated on the first begin.
+  void* probe_scratch = nullptr;  // this device's copy of `scratch` (set at ihs_create)
   SphParticle48* snap_dev = nullptr;
   ihs_particle* snap_host = nullptr;
   hipEvent_t snap_ev = nullptr;
@@ -532,7 +563,7 @@ int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
     delete h;
     return IHS_ERR_HIP;
   }
-  if (alloc_all(h) != IHS_OK) {
+  if (alloc_all(h) != IHS_OK || device_scratch(device, &h->probe_scratch) != hipSuccess) {
     ihs_destroy(h);
     return IHS_ERR_HIP;
   }
@@ -1076,6 +1107,32 @@ int ihs_direct_f64(ihs_handle* h, const uint32_t* ids, int64_t nt, double* acc3)
   if (d_tpos) hipFree(d_tpos);
   if (d_acc) hipFree(d_acc);
   if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "direct sum: %s", hipGetErrorString(e));
+  return IHS_OK;
+}
+
+int ihs_direct_f64_noalloc(ihs_handle* h, const uint32_t* ids, int64_t nt, double* acc3) {
+  if (!h || !ids || !acc3 || nt < 1) return IHS_ERR_ARG;
+  if (nt > kProbeMaxTargets)
+    return fail(h, IHS_ERR_CAPACITY, "the allocation-free probe takes at most %lld targets per call, got %lld",
+                (long long)kProbeMaxTargets, (long long)nt);
+  if (h->n < 1) return fail(h, IHS_ERR_STATE, "no particles uploaded");
+  for (int64_t k = 0; k < nt; ++k)
+    if (ids[k] >= (uint64_t)h->n) return fail(h, IHS_ERR_ARG, "target id %u >= n", ids[k]);
+  if (!h->probe_scratch) return fail(h, IHS_ERR_STATE, "probe scratch unavailable on device %d", h->device);
+  hipSetDevice(h->device);
+  // Every word the kernels read is written first in this call (ids copied, positions mapped, accelerations
+  // stored), so nothing is cleared; the storage-index table is the handle's own workspace (hw.pend).
+  char* base = (char*)h->probe_scratch;
+  uint32_t* d_ids = (uint32_t*)(base + kProbeIdsOff);
+  uint32_t* d_tpos = (uint32_t*)(base + kProbeTposOff);
+  double* d_acc = (double*)(base + kProbeAccOff);
+  HIP_TRY(h, hipMemcpyAsync(d_ids, ids, nt * sizeof(uint32_t), hipMemcpyHostToDevice, h->stream));
+  const MasterBufs& S = h->st[h->cur];
+  launch_direct_f64_targets(h->stream, h->split, S.px, S.py, S.pz, S.m, S.id, h->n, d_ids, nt, h->sc.seg_soft2d,
+                            h->nseg > 1 ? h->sc.segof : nullptr, h->sc.seg_start, h->hw.pend, d_tpos, d_acc);
+  HIP_TRY(h, hipGetLastError());
+  HIP_TRY(h, hipMemcpyAsync(acc3, d_acc, nt * 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIP_TRY(h, hipStreamSynchronize(h->stream));
   return IHS_OK;
 }
 

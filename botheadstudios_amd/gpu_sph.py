@@ -239,6 +239,13 @@ class GpuSph:
         self._check(self._L.ihs_direct_f64(self._h, ids.ctypes.data, ids.shape[0], acc.ctypes.data))
         return acc
 
+    def direct_f64_noalloc(self, ids) -> np.ndarray:
+        """direct_f64 through the allocation-free entry point (at most 65,536 targets per call)."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        acc = np.zeros((ids.shape[0], 3), np.float64)
+        self._check(self._L.ihs_direct_f64_noalloc(self._h, ids.ctypes.data, ids.shape[0], acc.ctypes.data))
+        return acc
+
     def gravity_error(self, n_targets: int = 8192, seed: int = 7) -> dict:
         """Error of the last evaluation's tree-walk gravity against the FP64 direct sum on sampled targets.
 

@@ -199,6 +199,13 @@ int ihs_debug_field(ihs_handle* h, int32_t field, void* out);
  * precision). Compare with IHS_FIELD_AGRAV of the last force evaluation (gpu-bh-verify's cpu_direct check,
  * ie/tools/gpu-bh-verify/src/main.rs:140-158, on the device). acc3 receives nt*3 doubles. Synchronous. */
 int ihs_direct_f64(ihs_handle* h, const uint32_t* ids, int64_t nt, double* acc3);
+/* ihs_direct_f64 without any device allocation: no counterpart in the reference. For callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops that check the walk's force error as they run,
+ * harnesses that forbid hipMalloc). The per-call staging lives in a static 2 MiB array on each GPU (held until the
+ * process ends), so a call takes at most 65,536 targets (IHS_ERR_CAPACITY beyond). Every handle on one GPU shares
+ * that array: calls that overlap on one GPU (from several threads) race. Synchronous; the results equal
+ * ihs_direct_f64's bit for bit. */
+int ihs_direct_f64_noalloc(ihs_handle* h, const uint32_t* ids, int64_t nt, double* acc3);
 /* Per-stage timing: enable=1 makes ihs_step_kdk launch directly (no graph) with HIP events around every
  * stage; ihs_profile_read returns accumulated milliseconds and launch counts per IHS_STAGE_*. */
 int ihs_profile(ihs_handle* h, int32_t enable);
