@@ -108,6 +108,7 @@ struct ihs_handle {
   Scratch sc;
   MlpDev mlp;
   int nseg = 1;
+  int n_mat = 0;                        // EOS records uploaded (materials index below this)
   std::vector<int64_t> seg_start_host;  // nseg + 1
   bool zero_soft = false;               // some scenario has no softening: the walk skips self pairs by index
   int64_t force_evals = 0;
@@ -638,6 +639,7 @@ int ihs_upload_batch(ihs_handle* h, const ihs_particle* p, const int64_t* counts
   h->zero_soft = false;
   for (int sg = 0; sg < nseg; ++sg) h->zero_soft |= !(soft2[sg] > 0.f);
   h->seg_start_host = start;
+  h->n_mat = n_mat;
   h->cur = 0;
   h->have_fe = false;
   h->h_settled = false;
@@ -1581,7 +1583,16 @@ struct CkptHeader {
   ihs_config cfg;
   StepCtl ctl;
   int64_t force_evals;
-  uint8_t h_settled, zero_soft, mlp_trained, pad[5];
+  uint8_t h_settled, zero_soft, mlp_trained;
+  uint8_t n_mat;  // EOS records in use (0 in files that predate the field: then the table's 16 bound)
+  uint8_t pad[4];
+};
+
+// Indices of the sections ckpt_sections lists, for the checks on the host before a load.
+enum CkptSection {
+  CK_SEG_START = 0, CK_SEGOF, CK_SOFT2, CK_SOFT2D, CK_HCAP, CK_CLK, CK_ROOT, CK_EOS, CK_MLP,
+  CK_PX, CK_PY, CK_PZ, CK_U, CK_VX, CK_VY, CK_VZ, CK_H, CK_M, CK_MAT, CK_ID, CK_PROV, CK_RHO, CK_FEAT, CK_HITER,
+  CK_COUNT
 };
 
 // The device sections of a checkpoint, in file order: the scenario tables, the EOS table, the network weights and
@@ -1637,6 +1648,7 @@ int64_t ihs_checkpoint_save(ihs_handle* h, void* buf, int64_t cap) {
   hd.h_settled = h->h_settled;
   hd.zero_soft = h->zero_soft;
   hd.mlp_trained = h->mlp.trained;
+  hd.n_mat = (uint8_t)h->n_mat;
   char* out = (char*)buf;
   std::memcpy(out, &hd, sizeof hd);
   int64_t off = sizeof hd;
@@ -1679,15 +1691,80 @@ int ihs_checkpoint_load(ihs_handle* h, const void* buf, int64_t bytes) {
                 (long long)total);
   }
   const char* in = (const char*)buf;
-  int64_t off = sizeof hd;
-  for (auto& sct : secs) {
-    HIP_TRY(h, hipMemcpy(sct.first, in + off, sct.second, hipMemcpyHostToDevice));
-    off += (int64_t)sct.second;
+  if (secs.size() != CK_COUNT) {
+    h->n = 0;
+    return fail(h, IHS_ERR_STATE, "checkpoint section table out of step with its reader");
+  }
+  std::vector<const char*> at(CK_COUNT);
+  {
+    int64_t off = sizeof hd;
+    for (int k = 0; k < CK_COUNT; ++k) {
+      at[k] = in + off;
+      off += (int64_t)secs[k].second;
+    }
+  }
+  // Everything a kernel indexes with is checked here, before any of it reaches the device: the scenario table,
+  // the particle ids (the scatter to upload order writes through them), the material indices (the EOS table
+  // has 16 records) and h; the per-particle scenario index and the tree roots are rebuilt from the table.
+  const int64_t n = hd.n;
+  const int nseg = hd.nseg;
+  const int K = h->cfg.leaf_size;
+  std::vector<uint32_t> start32(nseg + 1);
+  std::memcpy(start32.data(), at[CK_SEG_START], sizeof(uint32_t) * (nseg + 1));
+  auto refuse = [&](const char* what) {
+    h->n = 0;
+    return fail(h, IHS_ERR_ARG, "corrupt checkpoint: %s", what);
+  };
+  if (start32[0] != 0u || (int64_t)start32[nseg] != n) return refuse("scenario table does not span the particles");
+  for (int sg = 0; sg < nseg; ++sg) {
+    if (start32[sg + 1] <= start32[sg]) return refuse("scenario table not increasing");
+    if (nseg > 1 && (start32[sg + 1] - start32[sg]) % (uint32_t)K != 0) return refuse("scenario not leaf-aligned");
+  }
+  {
+    std::vector<uint8_t> seen(n, 0);
+    const uint32_t* id = (const uint32_t*)at[CK_ID];
+    for (int64_t t = 0; t < n; ++t) {
+      uint32_t v;
+      std::memcpy(&v, id + t, sizeof v);
+      if (v >= (uint64_t)n || seen[v]) return refuse("particle ids are not a permutation of 0..n-1");
+      seen[v] = 1;
+    }
+  }
+  const int mat_bound = hd.n_mat >= 1 && hd.n_mat <= 16 ? hd.n_mat : 16;
+  for (int64_t t = 0; t < n; ++t)
+    if ((uint8_t)at[CK_MAT][t] >= mat_bound) return refuse("material index outside the EOS table");
+  for (int64_t t = 0; t < n; ++t) {
+    float v;
+    std::memcpy(&v, at[CK_H] + 4 * t, sizeof v);
+    if (!(v > 0.f) || !std::isfinite(v)) return refuse("smoothing length not positive and finite");
+  }
+  for (int sg = 0; sg < nseg; ++sg) {
+    float f;
+    double d;
+    std::memcpy(&f, at[CK_SOFT2] + 4 * sg, sizeof f);
+    std::memcpy(&d, at[CK_SOFT2D] + 8 * sg, sizeof d);
+    if (!(f >= 0.f) || !std::isfinite(f) || !(d >= 0.0) || !std::isfinite(d)) return refuse("softening");
+  }
+  if (hd.mlp_trained)
+    for (int k = 0; k < MLP_NPARAM; ++k) {
+      float w;
+      std::memcpy(&w, at[CK_MLP] + 4 * k, sizeof w);
+      if (!std::isfinite(w)) return refuse("network weight not finite");
+    }
+  std::vector<uint32_t> segof(n), root(nseg);
+  const int64_t L = (n + K - 1) / K;
+  for (int sg = 0; sg < nseg; ++sg) {
+    for (int64_t t = start32[sg]; t < start32[sg + 1]; ++t) segof[t] = (uint32_t)sg;
+    const int64_t leaf0 = start32[sg] / K, nleaf = (start32[sg + 1] - start32[sg] + K - 1) / K;
+    root[sg] = nleaf >= 2 ? (uint32_t)leaf0 : (uint32_t)(L - 1 + leaf0);
+  }
+  for (int k = 0; k < CK_COUNT; ++k) {
+    const void* src = k == CK_SEGOF ? (const void*)segof.data() : k == CK_ROOT ? (const void*)root.data() : at[k];
+    HIP_TRY(h, hipMemcpy(secs[k].first, src, secs[k].second, hipMemcpyHostToDevice));
   }
   HIP_TRY(h, hipMemcpy(h->sc.ctl, &hd.ctl, sizeof(StepCtl), hipMemcpyHostToDevice));
-  std::vector<uint32_t> start32(hd.nseg + 1);
-  HIP_TRY(h, hipMemcpy(start32.data(), h->sc.seg_start, sizeof(uint32_t) * (hd.nseg + 1), hipMemcpyDeviceToHost));
   h->seg_start_host.assign(start32.begin(), start32.end());
+  h->n_mat = mat_bound;
   h->zero_soft = hd.zero_soft != 0;
   h->h_settled = hd.h_settled != 0;
   h->pos_fresh = false;

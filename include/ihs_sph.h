@@ -331,7 +331,12 @@ int ihs_assemble(ihs_handle* h, int32_t segment, const double* placements, int32
 int64_t ihs_checkpoint_bytes(ihs_handle* h);
 /* Writes the checkpoint into buf (cap bytes); returns the bytes written. Synchronous. */
 int64_t ihs_checkpoint_save(ihs_handle* h, void* buf, int64_t cap);
-/* Restores a checkpoint into a handle created with the same configuration. Synchronous. */
+/* Restores a checkpoint into a handle created with the same configuration. Synchronous. Before anything reaches
+ * the device, the sections kernels index with are checked on the host (particle ids a permutation of 0..n-1,
+ * material indices inside the EOS table the checkpoint was written with, a scenario table that spans the
+ * particles in leaf-aligned increasing runs, h > 0, finite softening and weights), and the per-particle scenario
+ * index and tree roots are rebuilt from the scenario table; a corrupt file is refused (IHS_ERR_ARG) and the handle
+ * then holds no particles. */
 int ihs_checkpoint_load(ihs_handle* h, const void* buf, int64_t bytes);
 
 /* accretion::find_clumps (accretion.rs:73-155) on n host bodies (pos3, vel3: n*3 doubles; mass, rho: n):

@@ -71,3 +71,51 @@ def test_checkpoint_refusals(tmp_path):
         checkpoint.from_bytes(data[:-100])
     with pytest.raises(ValueError, match="not a checkpoint"):
         checkpoint.from_bytes(b"x" * 200)
+
+
+def _sections(data):
+    """Byte offsets of a single-scenario FP32 checkpoint's sections (the order ihs_checkpoint_save writes)."""
+    hd = checkpoint.header(data)
+    n, nseg = hd["n"], hd["scenarios"]
+    assert not hd["split"]
+    sizes = [("seg_start", 4 * (nseg + 1)), ("segof", 4 * n), ("soft2", 4 * nseg), ("soft2d", 8 * nseg),
+             ("hcap", 4 * nseg), ("clk", 32 * nseg), ("root", 4 * nseg), ("eos", 16 * 48), ("mlp", 4 * 1313),
+             ("px", 4 * n), ("py", 4 * n), ("pz", 4 * n), ("u", 4 * n), ("vx", 4 * n), ("vy", 4 * n), ("vz", 4 * n),
+             ("h", 4 * n), ("m", 4 * n), ("mat", n), ("id", 4 * n), ("prov", 4 * n), ("rho", 4 * n), ("feat", 16 * n),
+             ("hiter", n)]
+    off, out = hd["header_bytes"], {}
+    for name, size in sizes:
+        out[name] = off
+        off += size
+    assert off == len(data)
+    return out, n
+
+
+@pytest.mark.parametrize("corruption", ["id_duplicate", "id_range", "mat", "seg_start", "h"])
+def test_checkpoint_corrupt_sections_are_refused_on_the_host(corruption):
+    """Sections the kernels index with are checked before anything reaches the device: the ids must be a permutation,
+    materials inside the uploaded EOS table, the scenario table must span the particles, h positive. The refused
+    handle takes a fresh upload afterwards."""
+    ic = ics.config_ic(1, n=4096)
+    with GpuSph(ic.n, leaf_size=16) as a:
+        a.upload(ic.particles48(), ic.eos, ic.softening)
+        a.encode_kdk(1)
+        data = bytearray(checkpoint.to_bytes(a))
+    at, n = _sections(bytes(data))
+    if corruption == "id_duplicate":
+        data[at["id"] + 4 * 5:at["id"] + 4 * 6] = data[at["id"] + 4 * 6:at["id"] + 4 * 7]
+    elif corruption == "id_range":
+        data[at["id"]:at["id"] + 4] = np.uint32(n + 7).tobytes()
+    elif corruption == "mat":
+        data[at["mat"] + 3] = len(ic.eos)  # one past the uploaded table
+    elif corruption == "seg_start":
+        data[at["seg_start"] + 4:at["seg_start"] + 8] = np.uint32(n - 1).tobytes()
+    else:
+        data[at["h"] + 4 * 9:at["h"] + 4 * 10] = np.float32(-1.0).tobytes()
+    with GpuSph(ic.n, leaf_size=16) as b:
+        buf = np.frombuffer(bytes(data), np.uint8)
+        with pytest.raises(IhsError, match="corrupt checkpoint"):
+            b._check(b._L.ihs_checkpoint_load(b._h, buf.ctypes.data, len(data)))
+        b.upload(ic.particles48(), ic.eos, ic.softening)
+        b.encode_kdk(1)
+        assert b.stats()["nonfinite"] == 0
