@@ -13,6 +13,36 @@ constexpr float kPI32 = 3.14159265359f;
 // ---------------------------------------------------------------------------------------------
 // Order-preserving float <-> u32 keys (the reference's bbox_key / bbox_decode,
 // ie/shaders/bh_gravity.wgsl:49-57): unsigned order of the key equals the float order.
+// Split precision keeps each position as hi + lo: hi is the FP32 rounding of the FP64 coordinate (the working copy
+// xm the passes read) and lo the remainder, held per axis as a 10-bit signed multiple of ulp(hi) / 512 in one 32-bit
+// word per particle (x in bits 0-9, y in 10-19, z in 20-29). |lo| <= ulp(hi) / 2 bounds the multiple by 256, and the
+// stored remainder is within ulp(hi) / 1024 of the true one (1 mm at 1e7 m).
+__device__ __forceinline__ uint32_t lo_code(double x, float hi) {
+  const int e = __builtin_amdgcn_frexp_expf(hi);  // hi = f 2^e, 0.5 <= |f| < 1: ulp(hi) = 2^(e - 24)
+  const int q = (int)rint(ldexp(x - (double)hi, 33 - e));
+  return (uint32_t)q & 0x3ffu;
+}
+
+__device__ __forceinline__ float lo_axis(uint32_t code, int shift, float hi) {
+  const int q = (int)(code << (22 - shift)) >> 22;  // the signed 10-bit field at `shift`
+  return __builtin_amdgcn_ldexpf((float)q, __builtin_amdgcn_frexp_expf(hi) - 33);
+}
+
+__device__ __forceinline__ float3 lo3(uint32_t code, const float4 hi) {
+  return make_float3(lo_axis(code, 0, hi.x), lo_axis(code, 10, hi.y), lo_axis(code, 20, hi.z));
+}
+
+// The displacement a - b of two particles: FP32 positions (REL = false), or hi/lo positions with remainders la
+// (decoded) and code cb. For neighbours the hi difference is exact (the coordinates agree in sign and lie within a
+// factor of two of each other, or are both small) and the lo difference is tiny, so only the final add rounds: the
+// displacement carries FP32 precision of its own size, not of the absolute position.
+template <bool REL>
+__device__ __forceinline__ float3 pair_d(const float4 a, const float3 la, const float4 b, uint32_t cb) {
+  if (!REL) return make_float3(a.x - b.x, a.y - b.y, a.z - b.z);
+  const float3 lb = lo3(cb, b);
+  return make_float3((a.x - b.x) + (la.x - lb.x), (a.y - b.y) + (la.y - lb.y), (a.z - b.z) + (la.z - lb.z));
+}
+
 __device__ __forceinline__ uint32_t f2key(float f) {
   uint32_t k = __float_as_uint(f);
   return (k >> 31) ? ~k : (k ^ 0x80000000u);

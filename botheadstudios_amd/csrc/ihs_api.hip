@@ -53,6 +53,7 @@ hipError_t device_scratch(int device, void** out) {
 struct Scratch {
   // sorted-order compute arrays
   float4* xm = nullptr;
+  uint32_t* xlo = nullptr;  // split precision: each position's remainder below xm (lo_code)
   float* hs = nullptr;
   float4* rpc = nullptr;
   float4* vh = nullptr;
@@ -244,6 +245,7 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &t.wq, 16));
   Scratch& s = h->sc;
   HIP_TRY(h, dalloc(h, &s.xm, C));
+  if (h->split) HIP_TRY(h, dalloc(h, &s.xlo, C));
   HIP_TRY(h, dalloc(h, &s.hs, C));
   HIP_TRY(h, dalloc(h, &s.rpc, C));
   HIP_TRY(h, dalloc(h, &s.vh, C));
@@ -364,7 +366,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   prof_mark(h, IHS_STAGE_SORT);
   float margin = 1.0f;
   if (newton) margin = (!after_drift && h->h_settled) ? kMarginSettled : h->cfg.h_margin;
-  launch_gather(s, h->split, n, h->tb.perm, S, D, sc.xm, sc.vh, sc.hs, margin, sv);
+  launch_gather(s, h->split, n, h->tb.perm, S, D, sc.xm, sc.vh, sc.hs, margin, sv, sc.xlo);
   h->cur ^= 1;
   prof_mark(h, IHS_STAGE_GATHER);
   if (seeded) {
@@ -400,8 +402,8 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   if (newton) {
     const float eta3 = eta * eta * eta;
     if (record) mlp_record_features(s, h->mlp, n, D, sc.eos, sv, eta);
-    launch_hsolve(s, n, sc.xm, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol, h->cfg.h_max_iter,
-                  h->hw, sc.flags, sc.counters, sv, nullptr, seeded ? sc.flags + FLAG_H_RETRY : nullptr);
+    launch_hsolve(s, n, sc.xm, sc.xlo, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol,
+                  h->cfg.h_max_iter, h->hw, sc.flags, sc.counters, sv, nullptr, seeded ? sc.flags + FLAG_H_RETRY : nullptr);
     if (record) mlp_record_targets(s, h->mlp, n, D);
     prof_mark(h, IHS_STAGE_HSOLVE);
     if (seeded) {
@@ -412,8 +414,8 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
       launch_groups(s, n, K, h->tb, sv, gate, NBR_DIAG_HS);
       launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags,
                        sc.counters, gate);
-      launch_hsolve(s, n, sc.xm, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol, h->cfg.h_max_iter,
-                    h->hw, sc.flags, sc.counters, sv, gate, nullptr);
+      launch_hsolve(s, n, sc.xm, sc.xlo, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol,
+                    h->cfg.h_max_iter, h->hw, sc.flags, sc.counters, sv, gate, nullptr);
       prof_mark(h, IHS_STAGE_HSOLVE);
     }
     h->h_settled = true;
@@ -421,10 +423,10 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   // The network's inputs come from the evaluation before a drift. With reuse_positions that evaluation is
   // velocity-only (it refreshes div v), so the position-dependent inputs are taken here, at the same positions.
   const int feat = (h->cfg.h_mode == IHS_H_LEARNED || h->mlp.record) && (!after_drift || h->cfg.reuse_positions);
-  launch_density(s, h->split, feat, n, sc.xm, D.h, D.u, D.mat, sc.eos, sc.nbr, sc.ncount, n, sc.rpc, sc.vh, D.rho,
+  launch_density(s, h->split, feat, n, sc.xm, sc.xlo, D.h, D.u, D.mat, sc.eos, sc.nbr, sc.ncount, n, sc.rpc, sc.vh, D.rho,
                  D.feat, sc.nmask, sc.flags, sv);
   prof_mark(h, IHS_STAGE_DENSITY);
-  launch_forces(s, kahan, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, sc.nmask, n,
+  launch_forces(s, kahan, n, sc.xm, sc.xlo, sc.vh, sc.rpc, sc.nbr, sc.ncount, sc.nmask, n,
                 (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph, feat ? D.feat : nullptr);
   prof_mark(h, IHS_STAGE_FORCES);
   if (!fused) {
@@ -451,7 +453,7 @@ void force_eval_velocity(ihs_handle* h) {
   launch_fe_init(s, sc.flags, sv);
   launch_eos(s, h->split, n, S.rho, S.h, S.vx, S.vy, S.vz, S.u, S.mat, sc.eos, sc.rpc, sc.vh, sc.flags, sv);
   prof_mark(h, IHS_STAGE_DENSITY);
-  launch_forces(s, h->split != 0, n, sc.xm, sc.vh, sc.rpc, sc.nbr, sc.ncount, sc.nmask, n,
+  launch_forces(s, h->split != 0, n, sc.xm, sc.xlo, sc.vh, sc.rpc, sc.nbr, sc.ncount, sc.nmask, n,
                 (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph, learned ? S.feat : nullptr);
   prof_mark(h, IHS_STAGE_FORCES);
   h->force_evals++;

@@ -27,12 +27,31 @@ constexpr int DWIN = 256 + 2 * DWIN_HALO;
 // global load per branch was slower: the branches then serialize and fewer loads are in flight.)
 struct XWin {
   const float4* w;
+  const uint32_t* wl;  // REL: the window's remainder codes (split precision)
   uint32_t lo, n;
 };
 
-__device__ __forceinline__ float4 win_xm(const XWin& W, const float4* __restrict__ xm, uint32_t j) {
+// A neighbour's working position and, in split precision (REL), its remainder code (pair_d in common.h).
+struct PosLo {
+  float4 q;
+  uint32_t c;
+};
+
+template <bool REL>
+__device__ __forceinline__ PosLo win_xm(const XWin& W, const float4* __restrict__ xm, const uint32_t* __restrict__ xlo,
+                                        uint32_t j) {
   const uint32_t o = j - W.lo;
-  return o < W.n ? W.w[o] : xm[j];
+  const bool inwin = o < W.n;
+  PosLo p;
+  p.q = inwin ? W.w[o] : xm[j];
+  p.c = REL ? (inwin ? W.wl[o] : xlo[j]) : 0u;
+  return p;
+}
+
+// This particle's remainder (zero without split precision).
+template <bool REL>
+__device__ __forceinline__ float3 own_lo(const uint32_t* __restrict__ xlo, int64_t t, const float4 me) {
+  return REL ? lo3(xlo[t], me) : make_float3(0.f, 0.f, 0.f);
 }
 
 // Window bounds of block t0 (blocks of 256) with `halo` particles on each side, clipped to [0, n).
@@ -309,19 +328,22 @@ __device__ __forceinline__ void spline_wdw(float q, float hinv, float& w, float&
   }
 }
 
-__device__ __forceinline__ float own_density(const float4 me, float hh, const uint4* __restrict__ nbr4, int64_t stride,
-                                             int64_t t, uint32_t cnt, const float4* __restrict__ xm, const XWin& W,
-                                             float& hdrho, uint32_t& nngb) {
+template <bool REL>
+__device__ __forceinline__ float own_density(const float4 me, const float3 melo, float hh,
+                                             const uint4* __restrict__ nbr4, int64_t stride, int64_t t, uint32_t cnt,
+                                             const float4* __restrict__ xm, const uint32_t* __restrict__ xlo,
+                                             const XWin& W, float& hdrho, uint32_t& nngb) {
   const float hinv = 1.0f / hh;
   const float sig = (8.0f / kPI32) * hinv * hinv * hinv;
   float rho = me.w * sig;
   float hd = -3.0f * rho;
   const float h2 = hh * hh;
   uint32_t c = 0;
-  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return win_xm(W, xm, j); },
-                 [&](uint32_t, const float4 q, bool ok) {
-    const float dx = q.x - me.x, dy = q.y - me.y, dz = q.z - me.z;
-    const float r2 = dx * dx + dy * dy + dz * dz;
+  for_neighbours(nbr4, stride, t, cnt, [&](uint32_t j) { return win_xm<REL>(W, xm, xlo, j); },
+                 [&](uint32_t, const PosLo p, bool ok) {
+    const float4 q = p.q;
+    const float3 d = pair_d<REL>(me, melo, q, p.c);
+    const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
     if (ok && r2 < h2) {
       const float qq = __builtin_amdgcn_sqrtf(r2) * hinv;
       float wq, dq;
@@ -358,9 +380,11 @@ struct HsolveState {
 // the list, and a butterfly over the group gives every lane of it the same totals (a + b == b + a in floating point),
 // so the group's lanes take the same Newton steps.
 constexpr int HS2_G = 4;
-__device__ __forceinline__ float own_density_group(const float4 me, float hh, const uint4* __restrict__ nbr4,
-                                                   int64_t stride, int64_t t, uint32_t cnt,
-                                                   const float4* __restrict__ xm, uint32_t sub, float& hdrho) {
+template <bool REL>
+__device__ __forceinline__ float own_density_group(const float4 me, const float3 melo, float hh,
+                                                   const uint4* __restrict__ nbr4, int64_t stride, int64_t t,
+                                                   uint32_t cnt, const float4* __restrict__ xm,
+                                                   const uint32_t* __restrict__ xlo, uint32_t sub, float& hdrho) {
   const float hinv = 1.0f / hh;
   const float sig = (8.0f / kPI32) * hinv * hinv * hinv;
   const float h2 = hh * hh;
@@ -369,13 +393,17 @@ __device__ __forceinline__ float own_density_group(const float4 me, float hh, co
     const uint4 q = nbr4[(int64_t)c * stride + t];
     const uint32_t jj[4] = {q.x, q.y, q.z, q.w};
     float4 src[4];
+    uint32_t lc[4] = {0u, 0u, 0u, 0u};
 #pragma unroll
-    for (int e = 0; e < 4; ++e) src[e] = xm[jj[e]];
+    for (int e = 0; e < 4; ++e) {
+      src[e] = xm[jj[e]];
+      if (REL) lc[e] = xlo[jj[e]];
+    }
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const float4 qv = src[e];
-      const float dx = qv.x - me.x, dy = qv.y - me.y, dz = qv.z - me.z;
-      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float3 d = pair_d<REL>(me, melo, qv, lc[e]);
+      const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
       if (4u * c + (uint32_t)e < cnt && r2 < h2) {
         const float qq = __builtin_amdgcn_sqrtf(r2) * hinv;
         float wq, dq;
@@ -404,14 +432,16 @@ __device__ __forceinline__ float own_density_group(const float4 me, float hh, co
 }
 
 // hsolve_loop for a group of HS2_G lanes sharing one particle (every lane of the group holds the same state).
-__device__ __forceinline__ void hsolve_loop_group(HsolveState& st, const float4 me, float hsi,
+template <bool REL>
+__device__ __forceinline__ void hsolve_loop_group(HsolveState& st, const float4 me, const float3 melo, float hsi,
                                                   const uint4* __restrict__ nbr4, int64_t stride, int64_t t,
-                                                  uint32_t cnt, const float4* __restrict__ xm, uint32_t sub,
-                                                  float eta3, float tol, int max_iter) {
+                                                  uint32_t cnt, const float4* __restrict__ xm,
+                                                  const uint32_t* __restrict__ xlo, uint32_t sub, float eta3,
+                                                  float tol, int max_iter) {
   float hh = st.h;
   for (;;) {
     float hd;
-    const float rho = own_density_group(me, hh, nbr4, stride, t, cnt, xm, sub, hd);
+    const float rho = own_density_group<REL>(me, melo, hh, nbr4, stride, t, cnt, xm, xlo, sub, hd);
     ++st.evals;
     const float hinv = 1.0f / hh;
     const float target = me.w * eta3 * hinv * hinv * hinv;
@@ -439,15 +469,17 @@ __device__ __forceinline__ void hsolve_loop_group(HsolveState& st, const float4 
   st.h = hh;
 }
 
-__device__ __forceinline__ void hsolve_loop(HsolveState& st, const float4 me, float hsi, uint32_t max_evals,
-                                            const uint4* __restrict__ nbr4, int64_t stride, int64_t t, uint32_t cnt,
-                                            const float4* __restrict__ xm, const XWin& W, float eta3, float tol,
+template <bool REL>
+__device__ __forceinline__ void hsolve_loop(HsolveState& st, const float4 me, const float3 melo, float hsi,
+                                            uint32_t max_evals, const uint4* __restrict__ nbr4, int64_t stride,
+                                            int64_t t, uint32_t cnt, const float4* __restrict__ xm,
+                                            const uint32_t* __restrict__ xlo, const XWin& W, float eta3, float tol,
                                             int max_iter) {
   float hh = st.h;
   for (;;) {
     float hd;
     uint32_t nn;
-    const float rho = own_density(me, hh, nbr4, stride, t, cnt, xm, W, hd, nn);
+    const float rho = own_density<REL>(me, melo, hh, nbr4, stride, t, cnt, xm, xlo, W, hd, nn);
     ++st.evals;
     const float hinv = 1.0f / hh;
     const float target = me.w * eta3 * hinv * hinv * hinv;
@@ -511,7 +543,9 @@ __global__ void k_hsolve_begin(const uint32_t* __restrict__ gate, uint32_t* __re
 // shard b mod PEND_SHARDS, whose entries start at shard * region) with their bracket, so that the second pass runs
 // full waves of unconverged particles instead of every wave waiting for its slowest lane. One shared counter took
 // an L2 atomic from every wave with a pending lane, one after another (140 us per seeded evaluation at 1M).
-__global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, const float* __restrict__ hs,
+template <bool REL>
+__global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, const uint32_t* __restrict__ xlo,
+                                                 const float* __restrict__ hs,
                                                  float* __restrict__ h, uint8_t* __restrict__ iters,
                                                  const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
                                                  int64_t stride, int64_t n, float eta3, float tol, int max_iter,
@@ -521,12 +555,16 @@ __global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, 
                                                  uint32_t* __restrict__ flags, unsigned long long* __restrict__ counters) {
   if (gate && *gate == 0u) return;
   __shared__ float4 wx[DWIN];
+  __shared__ uint32_t wl[REL ? DWIN : 1];
   const int64_t t0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
   int64_t wlo, whi;
   win_bounds(t0, n, DWIN_HALO, wlo, whi);
-  for (int64_t k = wlo + threadIdx.x; k < whi; k += blockDim.x) wx[k - wlo] = xm[k];
+  for (int64_t k = wlo + threadIdx.x; k < whi; k += blockDim.x) {
+    wx[k - wlo] = xm[k];
+    if (REL) wl[k - wlo] = xlo[k];
+  }
   __syncthreads();
-  const XWin W{wx, (uint32_t)wlo, (uint32_t)(whi - wlo)};
+  const XWin W{wx, wl, (uint32_t)wlo, (uint32_t)(whi - wlo)};
   const int64_t t = t0 + threadIdx.x;
   const bool valid = t < n;
   HsolveState st{0.f, 0.f, INFINITY, 0u, 0u, HS_DONE};
@@ -535,7 +573,9 @@ __global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, 
     hsi = hs[t];
     hcap = sv.hcap[sv.of(t)];
     st.h = fminf(h[t], hsi);
-    hsolve_loop(st, xm[t], hsi, 1u, nbr4, stride, t, ncount[t], xm, W, eta3, tol, max_iter);
+    const float4 me = xm[t];
+    hsolve_loop<REL>(st, me, own_lo<REL>(xlo, t, me), hsi, 1u, nbr4, stride, t, ncount[t], xm, xlo, W, eta3, tol,
+                     max_iter);
     h[t] = st.h;
   }
   const bool more = valid && st.state == HS_PENDING;
@@ -556,7 +596,9 @@ __global__ __launch_bounds__(256) void k_hsolve1(const float4* __restrict__ xm, 
 
 // Second pass over the pending lists (persistent grid-stride loop over their concatenation): resumes each solve
 // from its first update.
-__global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, const float* __restrict__ hs,
+template <bool REL>
+__global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, const uint32_t* __restrict__ xlo,
+                                                 const float* __restrict__ hs,
                                                  float* __restrict__ h, uint8_t* __restrict__ iters,
                                                  const uint4* __restrict__ nbr4, const uint32_t* __restrict__ ncount,
                                                  int64_t stride, float eta3, float tol, int max_iter,
@@ -607,7 +649,9 @@ __global__ __launch_bounds__(256) void k_hsolve2(const float4* __restrict__ xm, 
       st.h = h[t];
       st.lo = br.x;
       st.hi = br.y;
-      hsolve_loop_group(st, xm[t], hsi, nbr4, stride, t, ncount[t], xm, sub, eta3, tol, max_iter);
+      const float4 me = xm[t];
+      hsolve_loop_group<REL>(st, me, own_lo<REL>(xlo, t, me), hsi, nbr4, stride, t, ncount[t], xm, xlo, sub, eta3,
+                             tol, max_iter);
       if (sub == 0) {
         h[t] = st.h;
         iters[t] = (uint8_t)min(st.it, 255u);
@@ -680,8 +724,9 @@ __global__ __launch_bounds__(256) void k_eos(int64_t n, const float* __restrict_
 // mask (optional): the support mask of each list, bit k set when entry k is inside the kernel support
 // (r < h_ij), the pairs the force pass sums; entries between h_ij and the searched radius are then not fetched
 // again. The words are assembled in LDS (thread-private) and written once, after the loop.
-template <class PT, bool FEAT>
-__global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, const float* __restrict__ h,
+template <class PT, bool FEAT, bool REL>
+__global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, const uint32_t* __restrict__ xlo,
+                                                 const float* __restrict__ h,
                                                  const PT* __restrict__ u, const uint8_t* __restrict__ mat,
                                                  const EosDev* __restrict__ eos, const uint4* __restrict__ nbr4,
                                                  const uint32_t* __restrict__ ncount, int64_t stride, int64_t n,
@@ -691,20 +736,23 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
   __shared__ uint32_t smask[NMASK_WORDS * 256];
   __shared__ float4 wx[DWIN];
   __shared__ float wh[DWIN];
+  __shared__ uint32_t wl[REL ? DWIN : 1];
   const int64_t t0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
   int64_t wlo, whi;
   win_bounds(t0, n, DWIN_HALO, wlo, whi);
   for (int64_t k = wlo + threadIdx.x; k < whi; k += blockDim.x) {
     wx[k - wlo] = xm[k];
     wh[k - wlo] = h[k];
+    if (REL) wl[k - wlo] = xlo[k];
   }
   __syncthreads();
-  const uint32_t wl = (uint32_t)wlo, wn = (uint32_t)(whi - wlo);
+  const uint32_t wlb = (uint32_t)wlo, wn = (uint32_t)(whi - wlo);
   const int64_t t = t0 + threadIdx.x;
   const bool valid = t < n;
   uint32_t key = 0xffffffffu;
   if (valid) {
     const float4 me = xm[t];
+    const float3 melo = own_lo<REL>(xlo, t, me);
     const float hi = h[t];
     const float4 vi = vh[t];
     const uint32_t cnt = ncount[t];
@@ -718,17 +766,18 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
     struct Src {
       float4 q;
       float hj;
+      uint32_t c;
     };
     for_neighbours(nbr4, stride, t, cnt,
                    [&](uint32_t j) {
-                     const uint32_t o = j - wl;
-                     if (o < wn) return Src{wx[o], wh[o]};
-                     return Src{xm[j], h[j]};
+                     const uint32_t o = j - wlb;
+                     if (o < wn) return Src{wx[o], wh[o], REL ? wl[o] : 0u};
+                     return Src{xm[j], h[j], REL ? xlo[j] : 0u};
                    },
                    [&](uint32_t, const Src src, bool ok) {
       const float4 q = src.q;
-      const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
-      const float r2 = dx * dx + dy * dy + dz * dz;
+      const float3 d = pair_d<REL>(me, melo, q, src.c);
+      const float r2 = d.x * d.x + d.y * d.y + d.z * d.z;
       const float hij = 0.5f * (hi + src.hj);
       if (FEAT && ok && r2 < hi * hi) ++nn;
       if (ok && r2 < hij * hij) {
@@ -764,14 +813,16 @@ __global__ __launch_bounds__(256) void k_density(const float4* __restrict__ xm, 
 // coalesced loads; neighbours inside it are read from LDS and only the others are gathered from global memory
 // (each gathered lane costs its own L2 request, the bound of these passes). In Morton order about two thirds of
 // the pairs of the impact fall inside such a window.
-template <bool KAHAN, bool DIVV>
-__global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, const float4* __restrict__ vh,
+template <bool KAHAN, bool DIVV, bool REL>
+__global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, const uint32_t* __restrict__ xlo,
+                                                const float4* __restrict__ vh,
                                                 const float4* __restrict__ rpc, const uint4* __restrict__ nbr4,
                                                 const uint32_t* __restrict__ ncount,
                                                 const uint32_t* __restrict__ mask, int64_t stride, int64_t n,
                                                 const float* __restrict__ av, float4* __restrict__ asph,
                                                 float4* __restrict__ feat) {
   __shared__ float4 wx[FWIN], wv[FWIN], wr[FWIN];
+  __shared__ uint32_t wl[REL ? FWIN : 1];
   const int64_t t0 = (int64_t)xcd_block(blockIdx.x, gridDim.x) * blockDim.x;
   int64_t wlo, whi;
   win_bounds(t0, n, FWIN_HALO, wlo, whi);
@@ -779,13 +830,15 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
     wx[k - wlo] = xm[k];
     wv[k - wlo] = vh[k];
     wr[k - wlo] = rpc[k];
+    if (REL) wl[k - wlo] = xlo[k];
   }
   __syncthreads();
-  const uint32_t wl = (uint32_t)wlo, wn = (uint32_t)(whi - wlo);
+  const uint32_t wlb = (uint32_t)wlo, wn = (uint32_t)(whi - wlo);
   const int64_t t = t0 + threadIdx.x;
   if (t >= n) return;
   const float alpha = av[0], beta = av[1];
   const float4 me = xm[t];
+  const float3 melo = own_lo<REL>(xlo, t, me);
   const float4 vi = vh[t];
   const float4 ri = rpc[t];
   const uint32_t cnt = ncount[t];
@@ -794,15 +847,17 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
   ka.zero();
   struct Src {
     float4 q, vj, rj;
+    uint32_t c;
   };
   auto fetch = [&](uint32_t j) {
-    const uint32_t o = j - wl;
-    if (o < wn) return Src{wx[o], wv[o], wr[o]};
-    return Src{xm[j], vh[j], rpc[j]};
+    const uint32_t o = j - wlb;
+    if (o < wn) return Src{wx[o], wv[o], wr[o], REL ? wl[o] : 0u};
+    return Src{xm[j], vh[j], rpc[j], REL ? xlo[j] : 0u};
   };
   auto use = [&](uint32_t, const Src& src, bool ok) {
     const float4 q = src.q, vj = src.vj, rj = src.rj;
-    const float dx = me.x - q.x, dy = me.y - q.y, dz = me.z - q.z;
+    const float3 d = pair_d<REL>(me, melo, q, src.c);
+    const float dx = d.x, dy = d.y, dz = d.z;
     const float r2 = dx * dx + dy * dy + dz * dz;
     const float hij = 0.5f * (vi.w + vj.w);
     if (!ok || !(r2 < hij * hij) || !(r2 > 1.0e-18f)) return;
@@ -870,51 +925,73 @@ void launch_nbr_build(hipStream_t s, int64_t n, int K, const float4* xm, const f
                                              tb.ngroups, tb.wq, nbr4, ncount, cap, stride, flags, counters, gate);
 }
 
-void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, float* h, uint8_t* iters,
-                   const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol, int max_iter,
-                   const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters, const SegView& sv,
-                   const uint32_t* gate, uint32_t* retry) {
+template <bool REL>
+void hsolve_launch(hipStream_t s, int64_t n, const float4* xm, const uint32_t* xlo, const float* hs, float* h,
+                   uint8_t* iters, const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol,
+                   int max_iter, const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters,
+                   const SegView& sv, const uint32_t* gate, uint32_t* retry) {
   static unsigned nblk2 = 0;
-  if (!nblk2) nblk2 = persistent_blocks((const void*)k_hsolve2, 256);
+  if (!nblk2) nblk2 = persistent_blocks((const void*)k_hsolve2<REL>, 256);
   // Shard r holds the pending particles of blocks r, r + PEND_SHARDS, ...: at most region entries.
   const uint32_t region = (uint32_t)((g256(n) + PEND_SHARDS - 1) / PEND_SHARDS) * 256u;
   k_hsolve_begin<<<1, 64, 0, s>>>(gate, hw.npend);
-  k_hsolve1<<<g256(n), 256, 0, s>>>(xm, hs, h, iters, (const uint4*)nbr, ncount, stride, n, eta3, tol, max_iter, gate,
-                                    retry, sv, hw.pend, hw.bracket, hw.npend, region, flags, counters);
-  k_hsolve2<<<nblk2, 256, 0, s>>>(xm, hs, h, iters, (const uint4*)nbr, ncount, stride, eta3, tol, max_iter, gate, retry,
-                                  sv, hw.pend, hw.bracket, hw.npend, region, flags, counters);
+  k_hsolve1<REL><<<g256(n), 256, 0, s>>>(xm, xlo, hs, h, iters, (const uint4*)nbr, ncount, stride, n, eta3, tol,
+                                         max_iter, gate, retry, sv, hw.pend, hw.bracket, hw.npend, region, flags,
+                                         counters);
+  k_hsolve2<REL><<<nblk2, 256, 0, s>>>(xm, xlo, hs, h, iters, (const uint4*)nbr, ncount, stride, eta3, tol, max_iter,
+                                       gate, retry, sv, hw.pend, hw.bracket, hw.npend, region, flags, counters);
 }
 
-void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const void* u,
-                    const uint8_t* mat, const void* eos, const uint32_t* nbr, const uint32_t* ncount, int64_t stride,
-                    float4* rpc, float4* vh, float* rho, float4* featv, uint32_t* mask, uint32_t* flags,
-                    const SegView& sv) {
+void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const uint32_t* xlo, const float* hs, float* h,
+                   uint8_t* iters, const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol,
+                   int max_iter, const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters,
+                   const SegView& sv, const uint32_t* gate, uint32_t* retry) {
+  if (xlo)
+    hsolve_launch<true>(s, n, xm, xlo, hs, h, iters, nbr, ncount, stride, eta3, tol, max_iter, hw, flags, counters, sv,
+                        gate, retry);
+  else
+    hsolve_launch<false>(s, n, xm, xlo, hs, h, iters, nbr, ncount, stride, eta3, tol, max_iter, hw, flags, counters,
+                         sv, gate, retry);
+}
+
+void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const uint32_t* xlo, const float* h,
+                    const void* u, const uint8_t* mat, const void* eos, const uint32_t* nbr, const uint32_t* ncount,
+                    int64_t stride, float4* rpc, float4* vh, float* rho, float4* featv, uint32_t* mask,
+                    uint32_t* flags, const SegView& sv) {
   const EosDev* e = (const EosDev*)eos;
-#define IHS_DENS(PT, F)                                                                                              \
-  k_density<PT, F><<<g256(n), 256, 0, s>>>(xm, h, (const PT*)u, mat, e, (const uint4*)nbr, ncount, stride, n, rpc, \
-                                           vh, rho, featv, mask, flags, sv)
-  if (split) {
-    if (feat) IHS_DENS(double, true);
-    else IHS_DENS(double, false);
+#define IHS_DENS(PT, F, R)                                                                                            \
+  k_density<PT, F, R><<<g256(n), 256, 0, s>>>(xm, xlo, h, (const PT*)u, mat, e, (const uint4*)nbr, ncount, stride, \
+                                              n, rpc, vh, rho, featv, mask, flags, sv)
+  if (split && xlo) {
+    if (feat) IHS_DENS(double, true, true);
+    else IHS_DENS(double, false, true);
+  } else if (split) {
+    if (feat) IHS_DENS(double, true, false);
+    else IHS_DENS(double, false, false);
   } else {
-    if (feat) IHS_DENS(float, true);
-    else IHS_DENS(float, false);
+    if (feat) IHS_DENS(float, true, false);
+    else IHS_DENS(float, false, false);
   }
 #undef IHS_DENS
 }
 
-void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const float4* vh, const float4* rpc,
-                   const uint32_t* nbr, const uint32_t* ncount, const uint32_t* mask, int64_t stride, const float* av,
-                   float4* asph, float4* divv_feat) {
+void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const uint32_t* xlo, const float4* vh,
+                   const float4* rpc, const uint32_t* nbr, const uint32_t* ncount, const uint32_t* mask,
+                   int64_t stride, const float* av, float4* asph, float4* divv_feat) {
   const uint4* nbr4 = (const uint4*)nbr;
   const unsigned g = g256(n);
+#define IHS_FORCES(K, D, R) \
+  k_forces<K, D, R><<<g, 256, 0, s>>>(xm, xlo, vh, rpc, nbr4, ncount, mask, stride, n, av, asph, divv_feat)
   if (divv_feat) {
-    if (kahan) k_forces<true, true><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, mask, stride, n, av, asph, divv_feat);
-    else k_forces<false, true><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, mask, stride, n, av, asph, divv_feat);
+    if (kahan && xlo) IHS_FORCES(true, true, true);
+    else if (kahan) IHS_FORCES(true, true, false);
+    else IHS_FORCES(false, true, false);
   } else {
-    if (kahan) k_forces<true, false><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, mask, stride, n, av, asph, nullptr);
-    else k_forces<false, false><<<g, 256, 0, s>>>(xm, vh, rpc, nbr4, ncount, mask, stride, n, av, asph, nullptr);
+    if (kahan && xlo) IHS_FORCES(true, false, true);
+    else if (kahan) IHS_FORCES(true, false, false);
+    else IHS_FORCES(false, false, false);
   }
+#undef IHS_FORCES
 }
 
 void launch_eos(hipStream_t s, int split, int64_t n, const float* rho, const float* h, const float* vx, const float* vy,

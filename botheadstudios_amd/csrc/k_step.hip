@@ -32,7 +32,8 @@ __global__ __launch_bounds__(256) void k_retry_hs(int64_t n, const float* __rest
 template <class PT>
 __global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ perm, int64_t n, MasterBufs S, MasterBufs D,
                                                 float4* __restrict__ xm, float4* __restrict__ vh,
-                                                float* __restrict__ hs, float margin, SegView sv) {
+                                                float* __restrict__ hs, float margin, SegView sv,
+                                                uint32_t* __restrict__ xlo) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const uint32_t s = perm[t];
@@ -55,7 +56,9 @@ __global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ per
   D.rho[t] = S.rho[s];
   D.feat[t] = S.feat[s];
   D.hiter[t] = S.hiter[s];
-  xm[t] = make_float4((float)x, (float)y, (float)z, m);
+  const float hx = (float)x, hy = (float)y, hz = (float)z;
+  xm[t] = make_float4(hx, hy, hz, m);
+  if (xlo) xlo[t] = lo_code((double)x, hx) | (lo_code((double)y, hy) << 10) | (lo_code((double)z, hz) << 20);
   vh[t] = make_float4(vx, vy, vz, h);
   hs[t] = fminf(h * margin, fmaxf(sv.hcap[sv.of(t)], h));
 }
@@ -300,9 +303,10 @@ void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float 
 }
 
 void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,
-                   const MasterBufs& dst, float4* xm, float4* vh, float* hs, float h_margin, const SegView& sv) {
-  if (split) k_gather<double><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, vh, hs, h_margin, sv);
-  else k_gather<float><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, vh, hs, h_margin, sv);
+                   const MasterBufs& dst, float4* xm, float4* vh, float* hs, float h_margin, const SegView& sv,
+                   uint32_t* xlo) {
+  if (split) k_gather<double><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, vh, hs, h_margin, sv, xlo);
+  else k_gather<float><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, vh, hs, h_margin, sv, nullptr);
 }
 
 void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph,

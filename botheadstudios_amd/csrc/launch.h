@@ -193,8 +193,11 @@ struct HsolveScratch {
   float2* bracket = nullptr;   // n: their (lo, hi) after the first update, by particle
   uint32_t* npend = nullptr;   // PEND_SHARDS list lengths
 };
+// The SPH passes below take pair displacements from xm alone (xlo == null) or, in split precision, from the hi/lo
+// positions xm + xlo (pair_d in common.h).
 // retry != null: solves stopped by a search radius below hcap increment *retry instead of FLAG_H_CAPPED.
-void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const float* hs, float* h, uint8_t* iters,
+void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const uint32_t* xlo, const float* hs, float* h,
+                   uint8_t* iters,
                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol, int max_iter,
                    const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters, const SegView& sv,
                    const uint32_t* gate = nullptr, uint32_t* retry = nullptr);
@@ -204,12 +207,14 @@ void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float 
 // Also reduces each segment's Courant signal into sv.clk[s].signal. vh enters as (v, h_prev) and leaves as (v, h).
 // mask (optional, lists of at most NMASK_WORDS * 32 entries): receives each list's support mask, bit k of word
 // k / 32 at (k / 32) * stride + t set when entry k lies inside the kernel support r < h_ij.
-void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const float* h, const void* u,
+void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const uint32_t* xlo, const float* h,
+                    const void* u,
                     const uint8_t* mat, const void* eos, const uint32_t* nbr, const uint32_t* ncount, int64_t stride,
                     float4* rpc, float4* vh, float* rho, float4* featv, uint32_t* mask, uint32_t* flags,
                     const SegView& sv);
 // divv_feat != null: also write the velocity divergence into divv_feat[t].y.
-void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const float4* vh, const float4* rpc,
+void launch_forces(hipStream_t s, int kahan, int64_t n, const float4* xm, const uint32_t* xlo, const float4* vh,
+                   const float4* rpc,
                    const uint32_t* nbr, const uint32_t* ncount, const uint32_t* mask, int64_t stride, const float* av,
                    float4* asph, float4* divv_feat = nullptr);
 // Velocity-only evaluation at unchanged positions: EOS from the stored density and the current u, rpc / vh for
@@ -261,9 +266,11 @@ struct SphParticle48 {
 };
 
 void launch_fe_init(hipStream_t s, uint32_t* flags, const SegView& sv);
-// Permutes the state into the new order and writes the working copies xm (x, m), vh (v, h) and hs.
+// Permutes the state into the new order and writes the working copies xm (x, m), vh (v, h) and hs; with xlo (split
+// precision) also each position's remainder below its FP32 rounding (lo_code in common.h).
 void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,
-                   const MasterBufs& dst, float4* xm, float4* vh, float* hs, float h_margin, const SegView& sv);
+                   const MasterBufs& dst, float4* xm, float4* vh, float* hs, float h_margin, const SegView& sv,
+                   uint32_t* xlo = nullptr);
 void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph,
                        const float4* accg, const StepCtl* ctl, const SegView& sv);
 void launch_kick(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,

@@ -3,7 +3,8 @@
 steps.
 
 The oracle recomputes the SPH density, acceleration and du/dt in FP64 on the device's own state (the FP64 positions
-of split precision, the converged h). Gravity is measured against the FP64 direct sum on 8,192 sampled targets.
+of split precision, the converged h). Split precision must be at least as accurate as FP32 mode on density: its
+neighbour loops take each pair's displacement from hi/lo positions (the FP32 rounding plus the remainder). Gravity is measured against the FP64 direct sum on 8,192 sampled targets.
 The printed table is the per-quantity error report."""
 import numpy as np
 import pytest
@@ -61,6 +62,11 @@ def test_precision_report_cfg3():
     for p in ("fp32", "split"):
         r = rows[p]
         assert r["density max rel"] < 1e-4 and r["density rms rel"] < 1e-5, (p, r)
+    # Split precision takes each pair's displacement from its hi/lo positions, so its density is as accurate as
+    # FP32 mode's on FP32 positions (the rounding of a 1e7 m position to FP32 alone would cost ~1e-5). The two runs
+    # follow slightly different trajectories: 1.25 allows for that.
+    assert rows["split"]["density max rel"] <= 1.25 * rows["fp32"]["density max rel"], rows
+    assert rows["split"]["density rms rel"] <= 1.25 * rows["fp32"]["density rms rel"], rows
         # sph-verify's bounds are 3e-2 (acceleration) and 5e-2 (du/dt), sph-verify:342; the FP32 sound speed's
         # 1e-3 finite difference (sph_step.wgsl:96-100) leaves ~1e-4 relative noise in the viscosity terms.
         assert r["SPH acc rms"] < 3e-3 and r["du/dt rms"] < 3e-3, (p, r)
