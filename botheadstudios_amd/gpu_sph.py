@@ -262,7 +262,7 @@ class GpuSph:
         r2 = (ref ** 2).sum(1)
         err = np.sqrt(e2) / np.maximum(np.sqrt(r2), 1e-300)
         return {"rms": float(np.sqrt(np.mean(err ** 2))), "rms_ref_norm": float(np.sqrt(e2.sum() / max(r2.sum(), 1e-300))),
-                "max": float(err.max()), "targets": int(len(ids))}
+                "max": float(err.max()), "targets": int(len(ids)), "ids": ids, "ref": ref}
 
     # ---- learned smoothing-length seed ----
     def set_mlp_weights(self, w):

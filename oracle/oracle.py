@@ -35,6 +35,24 @@ def lib():
     return _lib
 
 
+_native = None
+
+
+def native_lib():
+    """The same oracle compiled with -march=native for this host (built on first use; the CPU baseline's timing
+    library). Falls back to the portable build when the compiler cannot target this host."""
+    global _native
+    if _native is None:
+        path = os.path.join(_HERE, "liboracle_sph_native.so")
+        try:
+            subprocess.run(["make", "-s", "-C", _HERE, "liboracle_sph_native.so"], check=True, capture_output=True)
+            _native = C.CDLL(path)
+            _declare(_native)
+        except (OSError, subprocess.CalledProcessError):
+            _native = lib()
+    return _native
+
+
 _dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
 _fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 _up = np.ctypeslib.ndpointer(dtype=np.uint32, flags="C_CONTIGUOUS")
@@ -93,6 +111,7 @@ def _declare(L):
     L.or_karras_ex.argtypes = [_up, i64, i32, _up, _up, _up]
     L.or_upsweep_f32.argtypes = [_fp, i64, i64, _up, _up, i64, _fp, _fp, _fp]
     L.or_bh_walk_f32.argtypes = [_fp, _up, i64, i64, _up, _up, _fp, _fp, _fp, i64, f32, f32, _fp]
+    L.or_bh_walk_count.argtypes = [_fp, i64, i64, _up, _up, _fp, _fp, _fp, i64, f32, f32, _lp, i64, _lp, C.c_void_p]
     L.or_direct_f64.argtypes = [_dp, _dp, i64, d, C.c_void_p, i64, _dp, C.c_void_p, C.c_int]
     L.or_octree_f64.argtypes = [_dp, _dp, i64, d, d, C.c_int, _dp]
     L.or_grid_pairs.argtypes = [_dp, i64, d, d, C.c_void_p, i64]
@@ -217,6 +236,17 @@ class Lbvh:
         self.bmax = np.zeros((nn, 3), np.float32)
         lib().or_upsweep_f32(self.sbody, n, k, self.left, self.right, self.L, self.com4, self.bmin, self.bmax)
 
+    def walk_counts(self, theta, soft, targets, with_acc=False):
+        """Per sorted target: (accepted nodes, leaf-particle interactions, opened nodes) of the oracle walk, and with
+        with_acc its FP32 acceleration (G included)."""
+        t = np.ascontiguousarray(targets, np.int64)
+        counts = np.zeros((t.shape[0], 3), np.int64)
+        acc = np.zeros((t.shape[0], 3), np.float32) if with_acc else None
+        lib().or_bh_walk_count(self.sbody, self.n, self.k, self.left, self.right, self.com4, self.bmin, self.bmax,
+                               self.L, float(theta), float(np.float32(soft * soft)), t, t.shape[0], counts,
+                               acc.ctypes.data if with_acc else None)
+        return (counts, acc) if with_acc else counts
+
     def walk(self, theta, soft):
         acc = np.zeros((self.n, 3), np.float32)
         lib().or_bh_walk_f32(self.sbody, self.order, self.n, self.k, self.left, self.right, self.com4,
@@ -265,7 +295,7 @@ class Hydro:
 
     def __init__(self, xyz, vel, u, h, m, mat, eos_table, softening, av_alpha=1.0, av_beta=2.0, theta=0.5,
                  gravity_mode=1, cs_fd=1, brute=False, leaf_cap=1, nthreads=0, h_solve=False, h_eta=2.2545,
-                 h_tol=1e-4):
+                 h_tol=1e-4, native=False):
         self.xyz = _f64(xyz).copy()
         self.vel = _f64(vel).copy()
         self.u = _f64(u).copy()
@@ -273,6 +303,7 @@ class Hydro:
         self.m = _f64(m).copy()
         self.mat = np.ascontiguousarray(mat, np.int32).copy()
         self.eos = _f64(eos_table).copy()
+        self._L = native_lib() if native else lib()
         self.st = OrState()
         s = self.st
         s.n = self.xyz.shape[0]
@@ -288,7 +319,7 @@ class Hydro:
 
     def density(self):
         rho = np.zeros(self.st.n)
-        lib().or_density(C.byref(self.st), rho)
+        self._L.or_density(C.byref(self.st), rho)
         return rho
 
     def forces(self, rho=None):
@@ -296,24 +327,24 @@ class Hydro:
             rho = self.density()
         acc = np.zeros((self.st.n, 3))
         dudt = np.zeros(self.st.n)
-        lib().or_forces(C.byref(self.st), _f64(rho), acc, dudt)
+        self._L.or_forces(C.byref(self.st), _f64(rho), acc, dudt)
         return acc, dudt
 
     def courant_dt(self, cfl, rho=None):
         if rho is None:
             rho = self.density()
-        return lib().or_courant_dt(C.byref(self.st), _f64(rho), float(cfl))
+        return self._L.or_courant_dt(C.byref(self.st), _f64(rho), float(cfl))
 
     def kdk(self, steps, dt=0.0, cfl=0.1):
-        return lib().or_kdk(C.byref(self.st), float(dt), float(cfl), int(steps))
+        return self._L.or_kdk(C.byref(self.st), float(dt), float(cfl), int(steps))
 
     def relax(self, steps, dt, damp):
         """hydrostatic.rs:283-291 relax_step, `steps` times (AV off, u fixed)."""
-        lib().or_relax(C.byref(self.st), float(dt), float(damp), int(steps))
+        self._L.or_relax(C.byref(self.st), float(dt), float(damp), int(steps))
 
     def energy(self):
         out = np.zeros(9)
-        lib().or_energy(C.byref(self.st), out)
+        self._L.or_energy(C.byref(self.st), out)
         return out
 
 

@@ -508,58 +508,86 @@ void or_upsweep_f32(const float* sb, int64_t n, int64_t k, const uint32_t* left,
   }
 }
 
-// ie/shaders/bh_gravity.wgsl:337-394 — per sorted particle stack walk with the robust MAC
-// dist >= |bmax-bmin|/theta + |centre-com|, exact leaf sums (s != t), scatter to acc[order[t]].
+// ie/shaders/bh_gravity.wgsl:337-394 — one sorted particle's stack walk with the robust MAC
+// dist >= |bmax-bmin|/theta + |centre-com| and exact leaf sums (s != t). cnt (may be null) receives the accepted
+// nodes, the leaf-particle interactions and the opened nodes of the walk.
+namespace {
+void bh_walk_one(const float* sb, int64_t t, int64_t n, int64_t k, const uint32_t* left, const uint32_t* right,
+                 const float* com4, const float* bmin3, const float* bmax3, int64_t L, float theta, float s2,
+                 float* a3, int64_t* cnt) {
+  const int64_t leaf0 = L - 1;
+  const float px = sb[4 * t], py = sb[4 * t + 1], pz = sb[4 * t + 2];
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  int64_t n_node = 0, n_pp = 0, n_open = 0;
+  uint32_t stack[64];
+  uint32_t sp = 0;
+  stack[sp++] = 0;
+  while (sp != 0) {
+    const uint32_t ni = stack[--sp];
+    if ((int64_t)ni >= leaf0) {
+      const int64_t c = ni - leaf0;
+      const int64_t start = c * k, end = std::min(start + k, n);
+      for (int64_t s = start; s < end; ++s) {
+        if (s == t) continue;
+        const float dx = sb[4 * s] - px, dy = sb[4 * s + 1] - py, dz = sb[4 * s + 2] - pz;
+        const float r2 = (dx * dx + dy * dy + dz * dz) + s2;
+        const float g = kG32 * sb[4 * s + 3] / (r2 * std::sqrt(r2));
+        ax += dx * g; ay += dy * g; az += dz * g;
+        ++n_pp;
+      }
+      continue;
+    }
+    const float* c4 = com4 + 4 * ni;
+    const float dx = c4[0] - px, dy = c4[1] - py, dz = c4[2] - pz;
+    const float r2 = (dx * dx + dy * dy + dz * dz) + s2;
+    const float dist = std::sqrt(r2);
+    const float sx = bmax3[3 * ni] - bmin3[3 * ni], sy = bmax3[3 * ni + 1] - bmin3[3 * ni + 1],
+                sz = bmax3[3 * ni + 2] - bmin3[3 * ni + 2];
+    const float size = std::sqrt(sx * sx + sy * sy + sz * sz);
+    const float ex = 0.5f * (bmin3[3 * ni] + bmax3[3 * ni]) - c4[0];
+    const float ey = 0.5f * (bmin3[3 * ni + 1] + bmax3[3 * ni + 1]) - c4[1];
+    const float ez = 0.5f * (bmin3[3 * ni + 2] + bmax3[3 * ni + 2]) - c4[2];
+    const float delta = std::sqrt(ex * ex + ey * ey + ez * ez);
+    if (dist >= size / theta + delta) {
+      const float g = kG32 * c4[3] / (r2 * dist);
+      ax += dx * g; ay += dy * g; az += dz * g;
+      ++n_node;
+      continue;
+    }
+    ++n_open;
+    if (sp + 2 <= 64) {
+      stack[sp++] = left[ni];
+      stack[sp++] = right[ni];
+    }
+  }
+  a3[0] = ax;
+  a3[1] = ay;
+  a3[2] = az;
+  if (cnt) {
+    cnt[0] = n_node;
+    cnt[1] = n_pp;
+    cnt[2] = n_open;
+  }
+}
+}  // namespace
+
 void or_bh_walk_f32(const float* sb, const uint32_t* order, int64_t n, int64_t k, const uint32_t* left,
                     const uint32_t* right, const float* com4, const float* bmin3, const float* bmax3, int64_t L,
                     float theta, float s2, float* acc) {
-  const int64_t leaf0 = L - 1;
 #pragma omp parallel for schedule(dynamic, 256)
-  for (int64_t t = 0; t < n; ++t) {
-    const float px = sb[4 * t], py = sb[4 * t + 1], pz = sb[4 * t + 2];
-    float ax = 0.f, ay = 0.f, az = 0.f;
-    uint32_t stack[64];
-    uint32_t sp = 0;
-    stack[sp++] = 0;
-    while (sp != 0) {
-      const uint32_t ni = stack[--sp];
-      if ((int64_t)ni >= leaf0) {
-        const int64_t c = ni - leaf0;
-        const int64_t start = c * k, end = std::min(start + k, n);
-        for (int64_t s = start; s < end; ++s) {
-          if (s == t) continue;
-          const float dx = sb[4 * s] - px, dy = sb[4 * s + 1] - py, dz = sb[4 * s + 2] - pz;
-          const float r2 = (dx * dx + dy * dy + dz * dz) + s2;
-          const float g = kG32 * sb[4 * s + 3] / (r2 * std::sqrt(r2));
-          ax += dx * g; ay += dy * g; az += dz * g;
-        }
-        continue;
-      }
-      const float* c4 = com4 + 4 * ni;
-      const float dx = c4[0] - px, dy = c4[1] - py, dz = c4[2] - pz;
-      const float r2 = (dx * dx + dy * dy + dz * dz) + s2;
-      const float dist = std::sqrt(r2);
-      const float sx = bmax3[3 * ni] - bmin3[3 * ni], sy = bmax3[3 * ni + 1] - bmin3[3 * ni + 1],
-                  sz = bmax3[3 * ni + 2] - bmin3[3 * ni + 2];
-      const float size = std::sqrt(sx * sx + sy * sy + sz * sz);
-      const float ex = 0.5f * (bmin3[3 * ni] + bmax3[3 * ni]) - c4[0];
-      const float ey = 0.5f * (bmin3[3 * ni + 1] + bmax3[3 * ni + 1]) - c4[1];
-      const float ez = 0.5f * (bmin3[3 * ni + 2] + bmax3[3 * ni + 2]) - c4[2];
-      const float delta = std::sqrt(ex * ex + ey * ey + ez * ez);
-      if (dist >= size / theta + delta) {
-        const float g = kG32 * c4[3] / (r2 * dist);
-        ax += dx * g; ay += dy * g; az += dz * g;
-        continue;
-      }
-      if (sp + 2 <= 64) {
-        stack[sp++] = left[ni];
-        stack[sp++] = right[ni];
-      }
-    }
-    const uint32_t o = order[t];
-    acc[3 * o] = ax;
-    acc[3 * o + 1] = ay;
-    acc[3 * o + 2] = az;
+  for (int64_t t = 0; t < n; ++t) bh_walk_one(sb, t, n, k, left, right, com4, bmin3, bmax3, L, theta, s2,
+                                              acc + 3 * (int64_t)order[t], nullptr);
+}
+
+void or_bh_walk_count(const float* sb, int64_t n, int64_t k, const uint32_t* left, const uint32_t* right,
+                      const float* com4, const float* bmin3, const float* bmax3, int64_t L, float theta, float s2,
+                      const int64_t* targets, int64_t nt, int64_t* counts, float* acc3) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t q = 0; q < nt; ++q) {
+    float a3[3];
+    bh_walk_one(sb, targets[q], n, k, left, right, com4, bmin3, bmax3, L, theta, s2, a3, counts + 3 * q);
+    if (acc3)
+      for (int c = 0; c < 3; ++c) acc3[3 * q + c] = a3[c];
   }
 }
 

@@ -75,6 +75,12 @@ void or_upsweep_f32(const float* sbody, int64_t n, int64_t k, const uint32_t* le
 void or_bh_walk_f32(const float* sbody, const uint32_t* order, int64_t n, int64_t k, const uint32_t* left,
                     const uint32_t* right, const float* com4, const float* bmin3, const float* bmax3,
                     int64_t L, float theta, float soft2, float* acc);
+/* The same walk on sorted targets, counting per target: accepted nodes, leaf-particle interactions and opened nodes
+ * (counts: nt*3 int64); acc3 (may be NULL) receives each target's acceleration (nt*3). The algorithmic work of a
+ * Barnes-Hut step is priced on these counts. */
+void or_bh_walk_count(const float* sbody, int64_t n, int64_t k, const uint32_t* left, const uint32_t* right,
+                      const float* com4, const float* bmin3, const float* bmax3, int64_t L, float theta,
+                      float soft2, const int64_t* targets, int64_t nt, int64_t* counts, float* acc3);
 
 /* FP64 gravity. Direct softened sum on targets (all if targets == NULL, nt = n). acc nt*3, pot nt
  * (pot may be NULL; pot_i = -sum_j G m_j / sqrt(r^2+eps^2), j != i). */

@@ -186,3 +186,23 @@ def test_frozen_lbvh_fixture(cloud):
         assert np.array_equal(lb.order, z["order"])
         assert np.array_equal(lb.left, z[f"left_k{k}"]) and np.array_equal(lb.right, z[f"right_k{k}"])
         assert np.array_equal(lb.parent, z[f"parent_k{k}"])
+
+
+def test_walk_counts_are_consistent():
+    """The oracle walk's counts (accepted nodes, leaf-particle interactions, opened nodes) on the 1500-cloud with
+    one-particle leaves: theta -> 0 opens every internal node and meets every other particle; a larger theta
+    opens fewer nodes and interacts less; each target's work is accounted (every opened node pushes two children)."""
+    pos, mass = bh_cloud(1500)
+    t = oracle.Lbvh(pos.astype(np.float32), mass.astype(np.float32), 1)
+    tg = np.arange(0, 1500, 7)
+    c0 = t.walk_counts(1e-6, 1.0e3, tg)
+    assert np.all(c0[:, 0] == 0) and np.all(c0[:, 1] == 1499) and np.all(c0[:, 2] == 1499)
+    c5 = t.walk_counts(0.5, 1.0e3, tg)
+    c9 = t.walk_counts(0.9, 1.0e3, tg)
+    assert c9.sum(0)[2] < c5.sum(0)[2] < c0.sum(0)[2]
+    assert (c9[:, 0] + c9[:, 1]).sum() < (c5[:, 0] + c5[:, 1]).sum()
+    # Every node visited is either accepted, a leaf (one particle; the target's own leaf is skipped) or opened:
+    # visits = 1 (root) + 2 * opened.
+    for c in (c5, c9):
+        own = 1  # the target's own one-particle leaf contributes no interaction
+        assert np.all(c[:, 0] + c[:, 1] + own + c[:, 2] == 1 + 2 * c[:, 2])
