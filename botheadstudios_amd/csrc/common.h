@@ -65,6 +65,9 @@ __device__ __forceinline__ void wave_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// A wave-uniform value moved to a scalar register (the first active lane's copy).
+__device__ __forceinline__ float sgpr_f(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
 template <class T>
 __device__ __forceinline__ T wave_min(T v) {
 #pragma unroll

@@ -150,8 +150,9 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
     bx1 = fmaxf(bx1, me[q].x); by1 = fmaxf(by1, me[q].y); bz1 = fmaxf(bz1, me[q].z);
     nv += valid[q] ? 1u : 0u;
   }
-  bx0 = wave_min(bx0); by0 = wave_min(by0); bz0 = wave_min(bz0);
-  bx1 = wave_max(bx1); by1 = wave_max(by1); bz1 = wave_max(bz1);
+  // The group box is wave-uniform: held in scalar registers, it frees vector registers for the walk.
+  bx0 = sgpr_f(wave_min(bx0)); by0 = sgpr_f(wave_min(by0)); bz0 = sgpr_f(wave_min(bz0));
+  bx1 = sgpr_f(wave_max(bx1)); by1 = sgpr_f(wave_max(by1)); bz1 = sgpr_f(wave_max(bz1));
   const uint64_t nvalid = (uint64_t)wave_sum((unsigned long long)nv);
 
   Acc A[T];
@@ -164,7 +165,7 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
   // Neighbour-list state (NBR): this lane's search radius, the group's largest, the 4-entry staging register.
   const float myhs = NBR && valid[0] ? io.hs[t[0]] : 0.f;
   const uint32_t myhs2u = __float_as_uint(myhs * myhs);
-  const float ghs = NBR ? wave_max(myhs) : 0.f;
+  const float ghs = NBR ? sgpr_f(wave_max(myhs)) : 0.f;
   uint32_t b0 = 0, b1 = 0, b2 = 0, b3 = 0, nb = 0, nchunk = 0, cntn = 0;
   const uint32_t cap4 = (uint32_t)io.cap >> 2;
 
