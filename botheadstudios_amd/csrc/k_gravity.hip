@@ -32,6 +32,14 @@ struct WalkLds {
   uint32_t phs2[64];  // with neighbour lists: the staged particles' search radius squared (float bits)
 };
 
+// A 16-byte neighbour-list chunk, stored non-temporally: the lists stream out of the walk, and outside the L2's
+// normal replacement they do not evict the tree nodes that other groups still read.
+__device__ __forceinline__ void store_chunk(uint4* p, uint4 v) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 x = {v.x, v.y, v.z, v.w};
+  __builtin_nontemporal_store(x, reinterpret_cast<u32x4*>(p));
+}
+
 // Padding entry of the source rings: massless and far from every particle, so that 0 * 1/r stays 0 even without
 // softening.
 constexpr float4 kPadSource = {1.0e18f, 0.f, 0.f, 0.f};
@@ -281,7 +289,7 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
         b3 = j;
         ++cntn;
         if (++nb == 4) {
-          if (nchunk < cap4) io.nbr4[(int64_t)nchunk * io.stride + t[0]] = make_uint4(b0, b1, b2, b3);
+          if (nchunk < cap4) store_chunk(io.nbr4 + (int64_t)nchunk * io.stride + t[0], make_uint4(b0, b1, b2, b3));
           ++nchunk;
           nb = 0;
         }
@@ -400,8 +408,8 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
   if (NBR) {
     // A partial last chunk holds its nb entries in b[4-nb..3]: move them to the front, padding with the last.
     if (valid[0] && nb > 0 && nchunk < cap4)
-      io.nbr4[(int64_t)nchunk * io.stride + t[0]] = nb == 1 ? make_uint4(b3, b3, b3, b3)
-                                                  : nb == 2 ? make_uint4(b2, b3, b3, b3) : make_uint4(b1, b2, b3, b3);
+      store_chunk(io.nbr4 + (int64_t)nchunk * io.stride + t[0],
+                  nb == 1 ? make_uint4(b3, b3, b3, b3) : nb == 2 ? make_uint4(b2, b3, b3, b3) : make_uint4(b1, b2, b3, b3));
     const uint32_t capn = cap4 * 4u;
     if (valid[0]) io.ncount[t[0]] = min(cntn, capn);
     const uint64_t over = __ballot(valid[0] && cntn > capn);
