@@ -27,7 +27,7 @@ class IhsStats(C.Structure):
         ("t", C.c_double), ("dt", C.c_double), ("steps", C.c_int64), ("force_evals", C.c_int64),
         ("mean_h_iters", C.c_double), ("mean_h_evals", C.c_double), ("mean_neighbours", C.c_double),
         ("nbr_overflow", C.c_uint32), ("walk_overflow", C.c_uint32), ("h_capped", C.c_uint32),
-        ("h_unconverged", C.c_uint32), ("nonfinite", C.c_uint32), ("_pad", C.c_uint32),
+        ("h_unconverged", C.c_uint32), ("nonfinite", C.c_uint32), ("tie_runs", C.c_uint32),
         ("grav_cell_interactions", C.c_int64), ("grav_pp_interactions", C.c_int64),
         ("nbr_candidates", C.c_int64), ("walk_nodes", C.c_int64),
     ]

@@ -273,6 +273,7 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &s.g_clk, 1));
   HIP_TRY(h, dalloc(h, &s.g_root, 1));
   HIP_TRY(h, dalloc(h, &t.codes0, C));
+  HIP_TRY(h, dalloc(h, &t.tie_runs, C / 65 + 1));
   HIP_TRY(h, alloc_raw(h, &s.eos, 16 * sizeof(ihs_eos)));
   HIP_TRY(h, dalloc(h, &s.staging, C));
   HIP_TRY(h, dalloc(h, &h->hw.pend, C + PEND_SHARDS * 256));
@@ -362,7 +363,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   launch_bbox_morton(s, h->split, S.px, S.py, S.pz, n, h->tb, sv);
   launch_radix_sort(s, n, 30, h->tb);
   launch_segment_order(s, n, h->tb, sv);
-  launch_fix_ties(s, n, S.id, h->tb);
+  launch_fix_ties(s, n, S.id, h->tb, sc.flags);
   prof_mark(h, IHS_STAGE_SORT);
   float margin = 1.0f;
   if (newton) margin = (!after_drift && h->h_settled) ? kMarginSettled : h->cfg.h_margin;
@@ -907,6 +908,7 @@ int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset) {
   out->h_capped = fl[FLAG_H_CAPPED];
   out->h_unconverged = fl[FLAG_H_UNCONVERGED];
   out->nonfinite = fl[FLAG_NONFINITE];
+  out->tie_runs = fl[FLAG_TIE_RUNS];
   out->grav_cell_interactions = (int64_t)cn[CNT_GRAV_CELL];
   out->grav_pp_interactions = (int64_t)cn[CNT_GRAV_PP];
   out->nbr_candidates = (int64_t)cn[CNT_NBR_CAND];

@@ -224,17 +224,32 @@ __global__ __launch_bounds__(RADIX_THREADS) void k_radix_scatter(const uint32_t*
   }
 }
 
-// Runs of equal codes are ordered by storage position after the stable sort; reorder each run by
-// the particle id so the final order is (code, id). Runs are short (a handful) in practice.
+// Runs of equal codes are ordered by storage position after the stable sort; each run is reordered by particle id
+// so the final order is (code, id). Storage holds the previous evaluation's sorted order, so a run is usually short
+// and nearly in order: its first thread insertion-sorts runs of up to kShortRun entries. A longer run (a dense clump
+// inside one Morton cell, as when escaping particles stretch the box) is listed for k_fix_long_ties, which sorts it
+// with a whole workgroup in O(r log^2 r).
+constexpr int kShortRun = 64;
+constexpr int kLongThreads = 1024;
+constexpr int kLongLds = 4096;  // runs up to this length sort in LDS
+constexpr int kLongBlocks = 64;
+
 __global__ __launch_bounds__(256) void k_fix_ties(const uint32_t* __restrict__ skeys, uint32_t* __restrict__ perm,
-                                                  const uint32_t* __restrict__ ids, int64_t n) {
+                                                  const uint32_t* __restrict__ ids, int64_t n,
+                                                  uint32_t* __restrict__ flags, uint32_t* __restrict__ long_runs) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
   const uint32_t c = skeys[t];
   if (t > 0 && skeys[t - 1] == c) return;          // not a run start
   if (t + 1 >= n || skeys[t + 1] != c) return;     // run of one
+  const int64_t lim = min(n, t + kShortRun + 1);
   int64_t e = t + 1;
-  while (e < n && skeys[e] == c) ++e;
+  while (e < lim && skeys[e] == c) ++e;
+  if (e - t > kShortRun) {
+    // At most n / (kShortRun + 1) runs are this long, the capacity of long_runs.
+    long_runs[atomicAdd(&flags[FLAG_TIE_RUNS], 1u)] = (uint32_t)t;
+    return;
+  }
   for (int64_t a = t + 1; a < e; ++a) {
     const uint32_t pv = perm[a];
     const uint32_t idv = ids[pv];
@@ -244,6 +259,117 @@ __global__ __launch_bounds__(256) void k_fix_ties(const uint32_t* __restrict__ s
       --b;
     }
     perm[b + 1] = pv;
+  }
+}
+
+// Ascending bitonic network on r keyed entries (any r: the entries past r count as +infinity, and every comparison
+// with one of them leaves the pair as it is). Each merge stage first compares mirrored pairs (i, i ^ (k - 1)), then
+// half-cleans with distances k/4 ... 1, so all comparisons order the pair ascending. ACC provides key(i), swap(i, j).
+template <class ACC>
+__device__ void bitonic_sort_block(ACC& a, uint32_t r) {
+  uint32_t P = 1;
+  while (P < r) P <<= 1;
+  for (uint32_t k = 2; k <= P; k <<= 1) {
+    for (uint32_t p = threadIdx.x; p < P / 2; p += blockDim.x) {
+      const uint32_t half = k >> 1;
+      const uint32_t i = (p / half) * k + (p % half);
+      const uint32_t j = i ^ (k - 1);
+      if (j < r && a.key(j) < a.key(i)) a.swap(i, j);
+    }
+    a.sync();
+    for (uint32_t d = k >> 2; d >= 1; d >>= 1) {
+      for (uint32_t p = threadIdx.x; p < P / 2; p += blockDim.x) {
+        const uint32_t i = (p / d) * 2 * d + (p % d);
+        const uint32_t j = i + d;
+        if (j < r && a.key(j) < a.key(i)) a.swap(i, j);
+      }
+      a.sync();
+    }
+  }
+}
+
+struct LdsRun {
+  uint32_t* k;
+  uint32_t* v;
+  __device__ uint32_t key(uint32_t i) const { return k[i]; }
+  __device__ void swap(uint32_t i, uint32_t j) {
+    const uint32_t ki = k[i], vi = v[i];
+    k[i] = k[j];
+    v[i] = v[j];
+    k[j] = ki;
+    v[j] = vi;
+  }
+  __device__ void sync() { __syncthreads(); }
+};
+
+// Global-memory run: (id, storage index) pairs in the sort's idle ping-pong buffers at the run's own offsets, so runs
+// sorted by different workgroups never overlap.
+struct GlobalRun {
+  uint32_t* k;
+  uint32_t* v;
+  __device__ uint32_t key(uint32_t i) const { return k[i]; }
+  __device__ void swap(uint32_t i, uint32_t j) {
+    const uint32_t ki = k[i], vi = v[i];
+    k[i] = k[j];
+    v[i] = v[j];
+    k[j] = ki;
+    v[j] = vi;
+  }
+  __device__ void sync() {
+    __threadfence_block();
+    __syncthreads();
+  }
+};
+
+// One workgroup per listed run: find the run's end, sort its (id, storage index) pairs by id, write the storage
+// indices back. kLongBlocks workgroups stride over the list; with no long runs they return at once.
+__global__ __launch_bounds__(kLongThreads) void k_fix_long_ties(const uint32_t* __restrict__ skeys,
+                                                                uint32_t* __restrict__ perm,
+                                                                const uint32_t* __restrict__ ids, int64_t n,
+                                                                const uint32_t* __restrict__ flags,
+                                                                const uint32_t* __restrict__ long_runs,
+                                                                uint32_t* __restrict__ tkey, uint32_t* __restrict__ tval) {
+  __shared__ uint32_t s_key[kLongLds], s_val[kLongLds];
+  __shared__ uint32_t s_end;
+  const uint32_t nl = flags[FLAG_TIE_RUNS];
+  for (uint32_t q = blockIdx.x; q < nl; q += gridDim.x) {
+    const uint32_t t = long_runs[q];
+    const uint32_t c = skeys[t];
+    if (threadIdx.x == 0) s_end = (uint32_t)n;
+    __syncthreads();
+    // The run's end: the first position past t whose code differs, found kLongThreads positions at a time.
+    for (uint32_t base = t;; base += kLongThreads) {
+      const uint32_t j = base + threadIdx.x;
+      if (j < (uint32_t)n && skeys[j] != c) atomicMin(&s_end, j);
+      __syncthreads();
+      const bool done = s_end != (uint32_t)n || base + kLongThreads >= (uint32_t)n;
+      __syncthreads();
+      if (done) break;
+    }
+    const uint32_t r = s_end - t;
+    if (r <= (uint32_t)kLongLds) {
+      for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) {
+        const uint32_t pv = perm[t + i];
+        s_val[i] = pv;
+        s_key[i] = ids[pv];
+      }
+      __syncthreads();
+      LdsRun acc{s_key, s_val};
+      bitonic_sort_block(acc, r);
+      for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) perm[t + i] = s_val[i];
+    } else {
+      for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) {
+        const uint32_t pv = perm[t + i];
+        tval[t + i] = pv;
+        tkey[t + i] = ids[pv];
+      }
+      __threadfence_block();
+      __syncthreads();
+      GlobalRun acc{tkey + t, tval + t};
+      bitonic_sort_block(acc, r);
+      for (uint32_t i = threadIdx.x; i < r; i += blockDim.x) perm[t + i] = tval[t + i];
+    }
+    __syncthreads();
   }
 }
 
@@ -789,8 +915,13 @@ int launch_radix_pairs(hipStream_t s, int64_t n, int bits, uint32_t* keys[2], ui
   return cur;
 }
 
-void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb) {
-  k_fix_ties<<<grid_for(n, 256), 256, 0, s>>>(tb.skeys, tb.perm, ids, n);
+void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb, uint32_t* flags) {
+  k_fix_ties<<<grid_for(n, 256), 256, 0, s>>>(tb.skeys, tb.perm, ids, n, flags, tb.tie_runs);
+  // The ping-pong buffers that hold neither the sorted codes nor the permutation are idle once the sort is done:
+  // scratch for runs longer than LDS holds (with scenarios the two results can sit in different slots).
+  uint32_t* tkey = tb.skeys == tb.keys[0] ? tb.keys[1] : tb.keys[0];
+  uint32_t* tval = tb.perm == tb.vals[0] ? tb.vals[1] : tb.vals[0];
+  k_fix_long_ties<<<kLongBlocks, kLongThreads, 0, s>>>(tb.skeys, tb.perm, ids, n, flags, tb.tie_runs, tkey, tval);
 }
 
 void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv) {

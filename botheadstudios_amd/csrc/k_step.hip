@@ -14,7 +14,10 @@ namespace {
 inline unsigned g256(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 __global__ void k_fe_init(uint32_t* flags, SegView sv) {
-  if (threadIdx.x == 0) flags[FLAG_H_RETRY] = 0u;
+  if (threadIdx.x == 0) {
+    flags[FLAG_H_RETRY] = 0u;
+    flags[FLAG_TIE_RUNS] = 0u;
+  }
   for (int sg = threadIdx.x; sg < sv.S; sg += blockDim.x) sv.clk[sg].signal = 0xffffffffu;
 }
 

@@ -12,7 +12,7 @@ enum FlagIndex : int {
   FLAG_WALK_OVERFLOW = 1,  // tree-walk stack overflows (interactions dropped)
   FLAG_H_CAPPED = 2,       // h-solves stopped at the searched radius (isolated particles)
   FLAG_H_UNCONVERGED = 3,  // h-solves that hit the iteration limit
-  FLAG_RESERVED4 = 4,      // (the Courant signal lives in the segment clocks)
+  FLAG_TIE_RUNS = 4,       // this evaluation: runs of equal Morton codes too long for the short-run sort
   FLAG_NONFINITE = 5,      // particles with non-finite results
   FLAG_H_RETRY = 6,        // this evaluation: seeded solves stopped by the tight search radius (gate of the
                            // wider-radius pass)
@@ -102,6 +102,7 @@ struct TreeBufs {
   uint32_t* skeys = nullptr;
   uint32_t* perm = nullptr;
   uint32_t* codes0 = nullptr;  // n: Morton codes by storage position (the segment pass re-gathers them)
+  uint32_t* tie_runs = nullptr;  // n / 65 + 1: first sorted positions of the runs of equal codes longer than 64
 };
 
 constexpr int BBOX_PARTS = 1024;          // blocks of the bbox pass (their partial boxes fold in one workgroup)
@@ -132,7 +133,7 @@ void launch_radix_sort(hipStream_t s, int64_t n, int bits, TreeBufs& tb);
 int launch_radix_pairs(hipStream_t s, int64_t n, int bits, uint32_t* keys[2], uint32_t* vals[2], int cur,
                        uint32_t* hist, uint32_t* dtot);
 // Reorders runs of equal codes in tb.perm by ids[perm] (so the order is (code, id)).
-void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb);
+void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb, uint32_t* flags);
 // Leaf cluster codes + Karras topology + counters reset.
 // With segments (sv.segof) the tree is a forest: one Karras tree per segment, rooted at sv.root[s].
 void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv);

@@ -188,7 +188,7 @@ class GpuSph:
             "dt": s.dt, "steps": s.steps, "force_evals": s.force_evals, "mean_h_iters": s.mean_h_iters,
             "mean_h_evals": s.mean_h_evals, "mean_neighbours": s.mean_neighbours, "nbr_overflow": s.nbr_overflow,
             "walk_overflow": s.walk_overflow, "h_capped": s.h_capped, "h_unconverged": s.h_unconverged,
-            "nonfinite": s.nonfinite, "grav_cell_interactions": s.grav_cell_interactions,
+            "nonfinite": s.nonfinite, "tie_runs": s.tie_runs, "grav_cell_interactions": s.grav_cell_interactions,
             "grav_pp_interactions": s.grav_pp_interactions, "nbr_candidates": s.nbr_candidates,
             "walk_nodes": s.walk_nodes,
         }

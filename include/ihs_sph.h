@@ -94,7 +94,9 @@ typedef struct {
   double mean_h_iters;    /* Newton updates per particle per force evaluation */
   double mean_h_evals;    /* density evaluations per particle per force evaluation */
   double mean_neighbours; /* neighbour-list entries per particle */
-  uint32_t nbr_overflow, walk_overflow, h_capped, h_unconverged, nonfinite, _pad;
+  uint32_t nbr_overflow, walk_overflow, h_capped, h_unconverged, nonfinite;
+  uint32_t tie_runs;       /* last evaluation's sort: runs of equal Morton codes longer than 64 (sorted by a
+                              workgroup each) */
   int64_t grav_cell_interactions; /* target x accepted-node interactions evaluated (since reset) */
   int64_t grav_pp_interactions;   /* target x leaf-particle interactions evaluated (since reset) */
   int64_t nbr_candidates;         /* target x candidate tests of the neighbour search (since reset) */

@@ -98,5 +98,24 @@ def rms_rel(approx, exact):
 def particles_from_ic(ic) -> np.ndarray:
     return ic.particles48()
 
+def checkpoint_sections(data):
+    """Byte offsets of a single-scenario FP32 checkpoint's sections (the order ihs_checkpoint_save writes)."""
+    from botheadstudios_amd import checkpoint
+
+    hd = checkpoint.header(data)
+    n, nseg = hd["n"], hd["scenarios"]
+    assert not hd["split"]
+    sizes = [("seg_start", 4 * (nseg + 1)), ("segof", 4 * n), ("soft2", 4 * nseg), ("soft2d", 8 * nseg),
+             ("hcap", 4 * nseg), ("clk", 32 * nseg), ("root", 4 * nseg), ("eos", 16 * 48), ("mlp", 4 * 1313),
+             ("px", 4 * n), ("py", 4 * n), ("pz", 4 * n), ("u", 4 * n), ("vx", 4 * n), ("vy", 4 * n), ("vz", 4 * n),
+             ("h", 4 * n), ("m", 4 * n), ("mat", n), ("id", 4 * n), ("prov", 4 * n), ("rho", 4 * n), ("feat", 16 * n),
+             ("hiter", n)]
+    off, out = hd["header_bytes"], {}
+    for name, size in sizes:
+        out[name] = off
+        off += size
+    assert off == len(data)
+    return out, n
+
 
 from botheadstudios_amd.ics import G, cold_blob, disk_scene  # noqa: E402,F401  (scene builders live in the package)

@@ -8,6 +8,7 @@ import pytest
 from botheadstudios_amd import checkpoint, ics
 from botheadstudios_amd import mlp as mlp_mod
 from botheadstudios_amd.gpu_sph import GpuSph, IhsError
+from tests.helpers import checkpoint_sections as _sections
 
 pytestmark = pytest.mark.gpu
 
@@ -71,24 +72,6 @@ def test_checkpoint_refusals(tmp_path):
         checkpoint.from_bytes(data[:-100])
     with pytest.raises(ValueError, match="not a checkpoint"):
         checkpoint.from_bytes(b"x" * 200)
-
-
-def _sections(data):
-    """Byte offsets of a single-scenario FP32 checkpoint's sections (the order ihs_checkpoint_save writes)."""
-    hd = checkpoint.header(data)
-    n, nseg = hd["n"], hd["scenarios"]
-    assert not hd["split"]
-    sizes = [("seg_start", 4 * (nseg + 1)), ("segof", 4 * n), ("soft2", 4 * nseg), ("soft2d", 8 * nseg),
-             ("hcap", 4 * nseg), ("clk", 32 * nseg), ("root", 4 * nseg), ("eos", 16 * 48), ("mlp", 4 * 1313),
-             ("px", 4 * n), ("py", 4 * n), ("pz", 4 * n), ("u", 4 * n), ("vx", 4 * n), ("vy", 4 * n), ("vz", 4 * n),
-             ("h", 4 * n), ("m", 4 * n), ("mat", n), ("id", 4 * n), ("prov", 4 * n), ("rho", 4 * n), ("feat", 16 * n),
-             ("hiter", n)]
-    off, out = hd["header_bytes"], {}
-    for name, size in sizes:
-        out[name] = off
-        off += size
-    assert off == len(data)
-    return out, n
 
 
 @pytest.mark.parametrize("corruption", ["id_duplicate", "id_range", "mat", "seg_start", "h"])

@@ -143,3 +143,43 @@ def test_tree_single_leaf_and_tiny():
         with GpuSph(64, leaf_size=16, h_mode="fixed") as sph:
             sph.upload(parts, eos_table(["granite"]), 5.0e3)
             _compare_tree(sph, parts, 16)
+
+
+def test_long_tie_runs_follow_particle_ids():
+    """Dense clumps inside single Morton cells (two corner particles stretch the box to 1e12 m, so a cell is ~1e9 m
+    wide) give runs of equal codes of 70, 3,000 and 10,000 particles: past the short-run insertion sort, through the
+    workgroup sort in LDS and in global memory. The ids are permuted against the storage order (a checkpoint written
+    with shuffled ids), so every run arrives scrambled; the order must still be the oracle's (code, id)."""
+    from botheadstudios_amd import checkpoint
+    from tests.helpers import checkpoint_sections
+
+    rng = np.random.default_rng(11)
+    cell = 1.0e12 / 1024.0
+    parts_pos = [np.array([[0.0, 0.0, 0.0], [1.0e12, 1.0e12, 1.0e12]]), rng.uniform(0.0, 1.0e12, (500, 3))]
+    for size, c in ((70, (3, 5, 7)), (3000, (100, 200, 300)), (10000, (512, 511, 513))):
+        d = rng.normal(size=(size, 3))
+        d *= (rng.uniform(0.0, 1.0, size) ** (1 / 3) / np.linalg.norm(d, axis=1))[:, None]
+        parts_pos.append((np.asarray(c) + 0.5) * cell + 1.0e6 * d)
+    pos = np.concatenate(parts_pos)
+    n = len(pos)
+    parts = _cloud_particles(pos, np.full(n, 1.0e18), 1.0e3)
+    with GpuSph(16384, leaf_size=8, h_mode="fixed") as a:
+        a.upload(parts, eos_table(["granite"]), 5.0e3)
+        data = bytearray(checkpoint.to_bytes(a))
+    at, nn = checkpoint_sections(bytes(data))
+    assert nn == n
+    data[at["id"]:at["id"] + 4 * n] = rng.permutation(n).astype(np.uint32).tobytes()
+    with checkpoint.from_bytes(bytes(data)) as b:
+        b.force_eval()
+        got = b.debug_tree()
+        st = b.stats()
+        back = b.readback()
+    assert st["tie_runs"] == 3, st["tie_runs"]
+    ref = oracle.Lbvh(np.ascontiguousarray(back["pos"], np.float32), back["mass"], 8)
+    assert np.array_equal(got["sorted_codes"], ref.sorted_codes)
+    runs = np.diff(np.flatnonzero(np.diff(np.r_[-1, ref.sorted_codes.astype(np.int64), -1]) != 0))
+    assert sorted(runs[runs > 64].tolist()) == [70, 3000, 10000]
+    assert np.array_equal(got["order"], ref.order), "(code, id) order differs inside long runs"
+    L = ref.L
+    assert np.array_equal(got["left"][: L - 1], ref.left[: L - 1])
+    assert np.array_equal(got["parent"], ref.parent)
