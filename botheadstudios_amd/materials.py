@@ -1,7 +1,7 @@
 """Tillotson parameter sets — the table of ie/data/materials.json `tillotson` blocks.
 
 The numbers are data and must equal the reference catalogue exactly (pinned in eos.rs:352-377;
-re-pinned by tests/test_oracle_eos.py). Order of the 10-field block (SphEos, gpu_sph.rs:39-55):
+re-pinned by tests/test_oracle_eos_kernel.py). Order of the 10-field block (SphEos, gpu_sph.rs:39-55):
 rho0, a, b, A, B, E0, E_iv, E_cv, alpha, beta.
 """
 from __future__ import annotations
