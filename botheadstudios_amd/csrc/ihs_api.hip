@@ -400,7 +400,11 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
     launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags, sc.counters);
     prof_mark(h, IHS_STAGE_NBR);
   }
-  if (newton) {
+  // At positions unchanged since the last solve (the first evaluation of a KDK step: the kick changed only v and
+  // u) every h already meets the tolerance, and a solve would only confirm it: it is skipped. A particle the last
+  // solve left capped or unconverged is solved again at the next evaluation after a drift.
+  const bool solve = newton && !(h->h_settled && !after_drift);
+  if (solve) {
     const float eta3 = eta * eta * eta;
     if (record) mlp_record_features(s, h->mlp, n, D, sc.eos, sv, eta);
     launch_hsolve(s, n, sc.xm, sc.xlo, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol,
