@@ -225,6 +225,7 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &t.ready, Lc));
   HIP_TRY(h, dalloc(h, &t.pend, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.npend, 1));
+  HIP_TRY(h, dalloc(h, &t.bstart, Lc));
   HIP_TRY(h, dalloc(h, &t.ncom, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.nbmin, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.nbmax, 2 * Lc));

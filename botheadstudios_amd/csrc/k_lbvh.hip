@@ -377,11 +377,16 @@ __global__ __launch_bounds__(kLongThreads) void k_fix_long_ties(const uint32_t* 
 // Karras 2012 over the L leaf cluster codes (first code of each bucket of K sorted particles).
 __global__ __launch_bounds__(256) void k_leaf_codes(const uint32_t* __restrict__ skeys, int64_t L, int K,
                                                     uint32_t* __restrict__ ccode, uint32_t* __restrict__ parent,
-                                                    uint32_t* __restrict__ gstart) {
+                                                    uint32_t* __restrict__ gstart, uint32_t* __restrict__ bstart,
+                                                    const uint32_t* __restrict__ lseg) {
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= L) return;
   ccode[c] = skeys[c * K];
   gstart[c] = IHS_NONE;
+  // A leaf that is a whole segment (or the whole tree) is its own upsweep block root; k_karras marks the others.
+  const bool alone = (c == 0 || (lseg && lseg[c * K] != lseg[(c - 1) * K])) &&
+                     (c == L - 1 || (lseg && lseg[c * K] != lseg[(c + 1) * K]));
+  bstart[c] = alone ? (uint32_t)(L - 1 + c) : IHS_NONE;
   // Roots (one per segment) keep NONE; k_karras overwrites the parent of every child it creates.
   parent[L - 1 + c] = IHS_NONE;
   if (c < L - 1) parent[c] = IHS_NONE;
@@ -401,7 +406,8 @@ __device__ __forceinline__ int kdelta(const uint32_t* __restrict__ cc, int64_t L
 __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cc, int64_t L, int2* __restrict__ child,
                                                 int2* __restrict__ range, uint32_t* __restrict__ parent,
                                                 uint32_t* __restrict__ ready, uint32_t* __restrict__ npend,
-                                                const uint32_t* __restrict__ lseg, int K) {
+                                                const uint32_t* __restrict__ lseg, int K, uint32_t* __restrict__ bstart,
+                                                int64_t bk) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) *npend = 0u;
   if (i >= L - 1) return;
@@ -442,17 +448,24 @@ __global__ __launch_bounds__(256) void k_karras(const uint32_t* __restrict__ cc,
   parent[li] = (uint32_t)i;
   parent[ri] = (uint32_t)i;
   ready[i] = 0u;
+  // Upsweep block roots: a child of at most bk leaves under a node of more, or a segment's root of at most bk.
+  if (last - first + 1 > bk) {
+    if (split - first + 1 <= bk) bstart[first] = li;
+    if (last - split <= bk) bstart[split + 1] = ri;
+  } else if ((first == 0 || (lseg && lseg[first * K] != lseg[(first - 1) * K])) &&
+             (last == L - 1 || (lseg && lseg[last * K] != lseg[(last + 1) * K]))) {
+    bstart[first] = (uint32_t)i;
+  }
 }
 
 // ---------------------------------------------------------------------------------------------
-// Upsweep. Phase 1: one thread per leaf sums its bucket, then climbs; a node is merged by the second
-// child to arrive (arrival counter with workgroup-scope acquire/release) as long as the node's leaf
-// range lies inside this block's leaves. A climb that reaches a node spanning several blocks records that
-// arrival instead. Phase 2 (one workgroup) replays the recorded arrivals up the remaining top nodes.
+// Upsweep. Phase 1: one workgroup per upsweep block root (a subtree of at most UB_LEAVES_* leaves whose parent holds
+// more; k_karras marks them) sums its leaves and climbs the subtree with the node values, topology and arrival
+// counters in LDS; a node is merged by the second child to arrive. The climb that completes a block root records its
+// arrival at the parent. Phase 2 (one workgroup) replays the recorded arrivals up the nodes above the block roots.
 
-constexpr int UPS_THREADS = 1024;
-constexpr int UPS_PER_THREAD = 4;
-constexpr int UPS_LEAVES = UPS_THREADS * UPS_PER_THREAD;  // leaves per block of the first phase
+constexpr int UB_LEAVES_1 = 1024;  // block-root leaf capacity with one particle per leaf (1,024 threads, ~112 KB LDS)
+constexpr int UB_LEAVES_K = 512;   // with buckets of K > 1 particles (their full leaf values in LDS, ~82 KB)
 constexpr int TOP_SLOTS = 8192;                          // top-phase arrival table in LDS (node ids)
 constexpr int TOP_PROBES = 32;                           // probes before a node counts in global memory
 
@@ -504,16 +517,13 @@ __device__ __forceinline__ NodeVals merge_vals(const NodeVals& l, const NodeVals
   const float2 qlb = l.qb, qrb = r.qb;
   const float m = cl.w + cr.w;
   float4 c = make_float4(0.f, 0.f, 0.f, m);
+  float4 qa = make_float4(0.f, 0.f, 0.f, 0.f);
+  float2 qb = make_float2(0.f, 0.f);
   if (m > 0.f) {
     const float im = 1.0f / m;
     c.x = (cl.x * cl.w + cr.x * cr.w) * im;
     c.y = (cl.y * cl.w + cr.y * cr.w) * im;
     c.z = (cl.z * cl.w + cr.z * cr.w) * im;
-  }
-  float4 qa = make_float4(0.f, 0.f, 0.f, 0.f);
-  float2 qb = make_float2(0.f, 0.f);
-  if (m > 0.f) {
-    const float im = 1.0f / m;
     quad_shift(qa, qb, cl, qla, qlb, c, cl.w * im);
     quad_shift(qa, qb, cr, qra, qrb, c, cr.w * im);
   }
@@ -525,35 +535,50 @@ __device__ __forceinline__ NodeVals merge_vals(const NodeVals& l, const NodeVals
   return NodeVals{c, b0, b1, qa, qb};
 }
 
+// The parent of a held node and its sibling: the operands are put in (left, right) order first, so lanes that hold
+// left and right children run one merge, not both branches.
+__device__ __forceinline__ NodeVals sel(bool a, const NodeVals& x, const NodeVals& y) {
+  return NodeVals{a ? x.c : y.c, a ? x.b0 : y.b0, a ? x.b1 : y.b1, a ? x.qa : y.qa, a ? x.qb : y.qb};
+}
+__device__ __forceinline__ NodeVals merge_pair(bool held_left, const NodeVals& held, const NodeVals& sib,
+                                               float inv_theta) {
+  return merge_vals(sel(held_left, held, sib), sel(held_left, sib, held), inv_theta);
+}
+
 // Parent p of `from`, whose values the caller holds: the sibling is loaded and the children merged in order.
 __device__ __forceinline__ NodeVals merge_with_sibling(const NodeVals& cur, uint32_t from, const int2 ch,
                                                        const float4* ncom, const float4* nbmin, const float4* nbmax,
                                                        const float4* nqa, const float2* nqb, float inv_theta) {
   const bool left = (uint32_t)ch.x == from;
   const NodeVals sib = load_node(left ? (uint32_t)ch.y : (uint32_t)ch.x, ncom, nbmin, nbmax, nqa, nqb);
-  return left ? merge_vals(cur, sib, inv_theta) : merge_vals(sib, cur, inv_theta);
+  return merge_pair(left, cur, sib, inv_theta);
 }
 
-// Leaf moments of leaf c (K particles) into the node arrays.
-__device__ __forceinline__ NodeVals leaf_moments(int64_t c, const float4* __restrict__ xm, const float* __restrict__ hs,
-                                                 int64_t n, int K, int64_t L, float4* ncom, float4* nbmin,
-                                                 float4* nbmax, float4* nqa, float2* nqb, float inv_theta) {
-  const int64_t start = c * K, end = min(start + (int64_t)K, n);
+// Particles of a leaf read from the sorted working copies (search radius 0 without them).
+struct GlobalParticles {
+  const float4* xm;
+  const float* hs;
+  __device__ __forceinline__ float4 x(int64_t s) const { return xm[s]; }
+  __device__ __forceinline__ float h(int64_t s) const { return hs ? hs[s] : 0.f; }
+};
+// Moments and box of the leaf holding particles [start, end), read through src.
+template <class P>
+__device__ __forceinline__ NodeVals leaf_vals(const P& src, int64_t start, int64_t end, float inv_theta) {
   // The centre of mass is formed relative to the leaf's first particle: a one-particle leaf's is then exactly
   // that particle's position (m x / m is not, in FP32), so a walk never sees a particle's own leaf as a source
   // an ulp away from it.
-  const float4 b0 = xm[start];
+  const float4 b0 = src.x(start);
   float m = 0.f, mx = 0.f, my = 0.f, mz = 0.f, hm = 0.f;
   float lo[3] = {1.0e30f, 1.0e30f, 1.0e30f}, hi[3] = {-1.0e30f, -1.0e30f, -1.0e30f};
   for (int64_t s = start; s < end; ++s) {
-    const float4 b = xm[s];
+    const float4 b = src.x(s);
     m += b.w;
     mx += (b.x - b0.x) * b.w;
     my += (b.y - b0.y) * b.w;
     mz += (b.z - b0.z) * b.w;
     lo[0] = fminf(lo[0], b.x); lo[1] = fminf(lo[1], b.y); lo[2] = fminf(lo[2], b.z);
     hi[0] = fmaxf(hi[0], b.x); hi[1] = fmaxf(hi[1], b.y); hi[2] = fmaxf(hi[2], b.z);
-    if (hs) hm = fmaxf(hm, hs[s]);
+    hm = fmaxf(hm, src.h(s));
   }
   float4 cm = make_float4(b0.x, b0.y, b0.z, m);
   if (m > 0.f) {
@@ -567,71 +592,137 @@ __device__ __forceinline__ NodeVals leaf_moments(int64_t c, const float4* __rest
   if (end - start > 1 && m > 0.f) {
     const float im = 1.0f / m;
     for (int64_t s = start; s < end; ++s) {
-      const float4 b = xm[s];
+      const float4 b = src.x(s);
       quad_shift(qa, qb, b, make_float4(0.f, 0.f, 0.f, 0.f), make_float2(0.f, 0.f), cm, b.w * im);
     }
   }
   const float sx = hi[0] - lo[0], sy = hi[1] - lo[1], sz = hi[2] - lo[2];
   const float ex = 0.5f * (lo[0] + hi[0]) - cm.x, ey = 0.5f * (lo[1] + hi[1]) - cm.y, ez = 0.5f * (lo[2] + hi[2]) - cm.z;
-  const NodeVals v{cm, make_float4(lo[0], lo[1], lo[2], hm),
-                   make_float4(hi[0], hi[1], hi[2],
-                               sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta + sqrtf(ex * ex + ey * ey + ez * ez)),
-                   qa, qb};
-  store_node((uint32_t)(L - 1 + c), v, ncom, nbmin, nbmax, nqa, nqb);
-  return v;
+  return NodeVals{cm, make_float4(lo[0], lo[1], lo[2], hm),
+                  make_float4(hi[0], hi[1], hi[2],
+                              sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta + sqrtf(ex * ex + ey * ey + ez * ez)),
+                  qa, qb};
 }
 
-// Phase 1. Arrival counters of the block's internal nodes live in LDS (node i of the block at i - blo: a node
-// whose leaf range lies inside the block has its index there). Pass j takes leaves blo + j * UPS_THREADS + t, so
-// each wave's leaf loads and node stores are contiguous. A climb keeps the node it merged in registers and loads
-// only the sibling; the next level's parent, children and range (read-only here) are fetched before the arrival
-// counter.
-__global__ __launch_bounds__(UPS_THREADS) void k_upsweep_local(const float4* __restrict__ xm, const float* __restrict__ hs,
-                                                               int64_t n, int K, int64_t L, const int2* __restrict__ child,
-                                                               const int2* __restrict__ range,
-                                                               const uint32_t* __restrict__ parent, float4* ncom,
-                                                               float4* nbmin, float4* nbmax, float4* nqa, float2* nqb,
-                                                               float inv_theta, uint32_t* __restrict__ pend,
-                                                               uint32_t* __restrict__ npend,
-                                                               const uint32_t* __restrict__ gate) {
+// LDS of one phase-1 workgroup: values of the subtree's internal nodes (node x at x - f, f = the root's first leaf;
+// a subtree's internal nodes are numbered inside its leaf range), its leaves (leaf c at c - f; a one-particle leaf's
+// quadrupole is zero and not stored), the internal nodes' children and parents, their arrival counters, and the
+// tile's block roots.
+template <int BK, bool KONE>
+struct UbLds {
+  float4 ic[BK], ib0[BK], ib1[BK], iqa[BK];
+  float2 iqb[BK];
+  float4 lc[BK], lb0[BK], lb1[BK];
+  float4 lqa[KONE ? 1 : BK];
+  float2 lqb[KONE ? 1 : BK];
+  int2 child[BK];
+  uint32_t parent[BK];
+  uint32_t arrive[BK];
+  uint32_t roots[BK];
+  uint32_t wcnt[BK / 64];
+};
+
+// Phase 1. Workgroup b takes the block roots whose first leaf lies in leaves [b BK, (b + 1) BK), one after another.
+// For a root over leaves [f, l], thread t sums leaf f + t (stored to the node arrays as well as to LDS) and climbs:
+// it publishes the node it holds in LDS, counts its arrival at the parent (LDS atomic; the fences order LDS only, so
+// no wait for the global stores sits on the climb), and the second arrival merges its held values with the
+// sibling's from LDS in (left, right) order. Merging is not contracted into FMAs, so every node gets the same bits
+// whichever thread or kernel merges it. The thread that completes the root records the arrival at its parent for
+// phase 2.
+template <bool KONE>
+__global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_blocks(
+    const float4* __restrict__ xm, const float* __restrict__ hs, int64_t n, int K, int64_t L,
+    const int2* __restrict__ child, const int2* __restrict__ range, const uint32_t* __restrict__ parent,
+    const uint32_t* __restrict__ bstart, float4* __restrict__ ncom, float4* __restrict__ nbmin,
+    float4* __restrict__ nbmax, float4* __restrict__ nqa, float2* __restrict__ nqb, float inv_theta,
+    uint32_t* __restrict__ pend, uint32_t* __restrict__ npend, const uint32_t* __restrict__ gate) {
+  constexpr int BK = KONE ? UB_LEAVES_1 : UB_LEAVES_K;
   if (gate && *gate == 0u) return;
-  __shared__ uint32_t lready[UPS_LEAVES];
-  for (int i = threadIdx.x; i < UPS_LEAVES; i += UPS_THREADS) lready[i] = 0u;
+  __shared__ UbLds<BK, KONE> S;
+  const int t = (int)threadIdx.x;
+  const int w = t >> 6;
+  // The tile's block roots in leaf order (a root holds at most BK leaves, so every tile holds at least one start).
+  const int64_t c0 = (int64_t)blockIdx.x * BK + t;
+  const uint32_t rs = c0 < L ? bstart[c0] : IHS_NONE;
+  const uint64_t bal = __ballot(rs != IHS_NONE);
+  if ((t & 63) == 0) S.wcnt[w] = (uint32_t)__popcll(bal);
   __syncthreads();
-  const int64_t blo = (int64_t)blockIdx.x * UPS_LEAVES, bhi = blo + UPS_LEAVES;
-  for (int j = 0; j < UPS_PER_THREAD; ++j) {
-    const int64_t c = blo + (int64_t)j * UPS_THREADS + threadIdx.x;
-    if (c >= L) break;
-    NodeVals cur = leaf_moments(c, xm, hs, n, K, L, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
-    uint32_t from = (uint32_t)(L - 1 + c);
-    uint32_t p = parent[from];
-    if (p == IHS_NONE) continue;
-    int2 rg = range[p], ch = child[p];
-    uint32_t pp = parent[p];
-    for (;;) {
-      if (rg.x < blo || rg.y >= bhi) {  // top node: phase 2 replays this arrival
-        pend[atomicAdd(npend, 1u)] = p;
-        break;
-      }
-      int2 rg2 = make_int2(0, 0), ch2 = make_int2(0, 0);
-      uint32_t pp2 = IHS_NONE;
-      if (pp != IHS_NONE) {
-        rg2 = range[pp];
-        ch2 = child[pp];
-        pp2 = parent[pp];
-      }
-      const uint32_t old =
-          __hip_atomic_fetch_add(&lready[p - blo], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0u) break;
-      cur = merge_with_sibling(cur, from, ch, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
-      store_node(p, cur, ncom, nbmin, nbmax, nqa, nqb);
-      if (pp == IHS_NONE) break;
-      from = p;
-      p = pp;
-      rg = rg2;
-      ch = ch2;
-      pp = pp2;
+  uint32_t base = 0, nroots = 0;
+#pragma unroll
+  for (int k = 0; k < BK / 64; ++k) {
+    const uint32_t v = S.wcnt[k];
+    base += k < w ? v : 0u;
+    nroots += v;
+  }
+  if (rs != IHS_NONE) S.roots[base + lanes_below(bal)] = rs;
+  __syncthreads();
+  const uint32_t Lm1 = (uint32_t)(L - 1);
+  for (uint32_t q = 0; q < nroots; ++q) {
+    const uint32_t r = S.roots[q];
+    int64_t f = (int64_t)r - (L - 1), l = f;
+    if (r < Lm1) {
+      const int2 rg = range[r];
+      f = rg.x;
+      l = rg.y;
     }
+    const int64_t cnt = l - f + 1;
+    NodeVals cur;
+    uint32_t from = IHS_NONE, p = IHS_NONE;
+    if (t < cnt) {
+      const int64_t x = f + t;
+      if (x < L - 1) {
+        S.child[t] = child[x];
+        S.parent[t] = parent[x];
+      }
+      S.arrive[t] = 0u;
+      cur = leaf_vals(GlobalParticles{xm, hs}, x * K, min((x + 1) * K, n), inv_theta);
+      from = (uint32_t)(L - 1 + x);
+      store_node(from, cur, ncom, nbmin, nbmax, nqa, nqb);
+      p = parent[from];
+      S.lc[t] = cur.c;
+      S.lb0[t] = cur.b0;
+      S.lb1[t] = cur.b1;
+      if (!KONE) {
+        S.lqa[t] = cur.qa;
+        S.lqb[t] = cur.qb;
+      }
+    }
+    __syncthreads();
+    if (t < cnt) {
+      while (from != r) {
+        const int pi = (int)((int64_t)p - f);
+        if (from < Lm1) {
+          const int fi = (int)((int64_t)from - f);
+          S.ic[fi] = cur.c;
+          S.ib0[fi] = cur.b0;
+          S.ib1[fi] = cur.b1;
+          S.iqa[fi] = cur.qa;
+          S.iqb[fi] = cur.qb;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+        const uint32_t old = __hip_atomic_fetch_add(&S.arrive[pi], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (old == 0u) break;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+        const int2 ch = S.child[pi];
+        const bool left = (uint32_t)ch.x == from;
+        const uint32_t sib = left ? (uint32_t)ch.y : (uint32_t)ch.x;
+        NodeVals sv;
+        if (sib >= Lm1) {
+          const int si = (int)((int64_t)(sib - Lm1) - f);
+          sv = NodeVals{S.lc[si], S.lb0[si], S.lb1[si], KONE ? make_float4(0.f, 0.f, 0.f, 0.f) : S.lqa[si],
+                        KONE ? make_float2(0.f, 0.f) : S.lqb[si]};
+        } else {
+          const int si = (int)((int64_t)sib - f);
+          sv = NodeVals{S.ic[si], S.ib0[si], S.ib1[si], S.iqa[si], S.iqb[si]};
+        }
+        cur = merge_pair(left, cur, sv, inv_theta);
+        store_node(p, cur, ncom, nbmin, nbmax, nqa, nqb);
+        from = p;
+        p = S.parent[pi];
+      }
+      if (from == r && p != IHS_NONE) pend[atomicAdd(npend, 1u)] = p;
+    }
+    __syncthreads();
   }
 }
 
@@ -926,10 +1017,11 @@ void launch_fix_ties(hipStream_t s, int64_t n, const uint32_t* ids, TreeBufs& tb
 
 void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv) {
   const int64_t L = (n + K - 1) / K;
-  k_leaf_codes<<<grid_for(L, 256), 256, 0, s>>>(tb.skeys, L, K, tb.ccode, tb.parent, tb.gstart);
+  tb.ub_leaves = K == 1 ? UB_LEAVES_1 : UB_LEAVES_K;
+  k_leaf_codes<<<grid_for(L, 256), 256, 0, s>>>(tb.skeys, L, K, tb.ccode, tb.parent, tb.gstart, tb.bstart, sv.segof);
   if (L > 1)
     k_karras<<<grid_for(L - 1, 256), 256, 0, s>>>(tb.ccode, L, tb.child, tb.range, tb.parent, tb.ready, tb.npend,
-                                                  sv.segof, K);
+                                                  sv.segof, K, tb.bstart, tb.ub_leaves);
 }
 
 void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv, const uint32_t* gate,
@@ -951,10 +1043,15 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
   const int64_t L = (n + K - 1) / K;
   const float inv_theta = theta > 0.f ? 1.0f / theta : 3.0e38f;
   if (gate) k_tree_rearm<<<grid_for(L, 256), 256, 0, s>>>(L, tb.ready, tb.npend, gate);
-  k_upsweep_local<<<grid_for(L, UPS_LEAVES), UPS_THREADS, 0, s>>>(xm, hs, n, K, L, tb.child, tb.range, tb.parent,
-                                                                 tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb, inv_theta,
-                                                                 tb.pend, tb.npend, gate);
-  if (L > UPS_LEAVES)
+  if (K == 1)
+    k_upsweep_blocks<true><<<grid_for(L, UB_LEAVES_1), UB_LEAVES_1, 0, s>>>(
+        xm, hs, n, K, L, tb.child, tb.range, tb.parent, tb.bstart, tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb,
+        inv_theta, tb.pend, tb.npend, gate);
+  else
+    k_upsweep_blocks<false><<<grid_for(L, UB_LEAVES_K), UB_LEAVES_K, 0, s>>>(
+        xm, hs, n, K, L, tb.child, tb.range, tb.parent, tb.bstart, tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb,
+        inv_theta, tb.pend, tb.npend, gate);
+  if (L > tb.ub_leaves)
     k_upsweep_top<<<1, 1024, 0, s>>>(tb.pend, tb.npend, tb.child, tb.parent, tb.ready, tb.ncom, tb.nbmin, tb.nbmax,
                                      tb.nqa, tb.nqb, inv_theta, gate);
 }

@@ -80,8 +80,11 @@ struct TreeBufs {
   int2* range = nullptr;     // L-1 (first, last leaf)
   uint32_t* parent = nullptr;  // 2L-1
   uint32_t* ready = nullptr;   // L-1
-  uint32_t* pend = nullptr;    // 2L: upsweep arrivals at nodes above the upsweep's leaf blocks
+  uint32_t* pend = nullptr;    // 2L: upsweep arrivals at nodes above the upsweep's block roots
   uint32_t* npend = nullptr;   // 1: arrivals recorded in pend
+  uint32_t* bstart = nullptr;  // L: the upsweep block root whose first leaf is leaf c (none otherwise): block roots
+                               // are the subtrees of at most ub_leaves leaves whose parent holds more
+  int ub_leaves = 1024;        // leaf capacity of an upsweep block root (set by launch_karras from K)
   float4* ncom = nullptr;   // 2L-1: com xyz, mass
   float4* nbmin = nullptr;  // 2L-1: bmin xyz, max search radius
   float4* nbmax = nullptr;  // 2L-1: bmax xyz, critical radius |bmax-bmin|/theta + |centre-com|

@@ -155,11 +155,12 @@ def test_quadrupole_never_worse_on_the_cloud(cloud):
 
 
 def test_tree_top_phase_beyond_its_lds_table():
-    """4,194,304 bodies, one per leaf (bh_cloud seed 7): 12,623 tree nodes cross the upsweep's 4,096-leaf blocks
-    (counted on the oracle's tree), more than the top phase's 8,192-entry LDS table holds, so part of the top tree
-    counts its arrivals in global memory. A lost or doubled merge there would leave wrong moments in the largest
-    nodes; the tree force at 256 targets agrees with the FP64 direct sum to the walk's accuracy at theta 0.5."""
-    n = 1 << 22
+    """8,388,608 bodies, one per leaf (bh_cloud seed 7): 12,222 tree nodes hold more than the upsweep's 1,024-leaf
+    block roots (counted on the oracle's tree), more than the top phase's 8,192-entry LDS table holds, so part of the
+    top tree counts its arrivals in global memory. A lost or doubled merge there would leave wrong moments in the
+    largest nodes; the tree force at 256 targets agrees with the FP64 direct sum to the walk's accuracy at theta
+    0.5."""
+    n = 1 << 23
     pos, mass = bh_cloud(n, seed=7)
     f = GravityField(n, leaf_size=1, knee=1000)
     try:
