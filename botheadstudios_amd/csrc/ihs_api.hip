@@ -59,11 +59,13 @@ struct Scratch {
   float4* vh = nullptr;
   float4* asph = nullptr;
   float4* accg = nullptr;
+  float4* apot = nullptr;  // the potential walk's output (an energy read after steps that skipped the potential)
   uint32_t* nbr = nullptr;
   uint32_t* ncount = nullptr;
   uint32_t* nmask = nullptr;  // support masks of the lists (null when lists exceed NMASK_WORDS * 32 entries)
   uint32_t* flags = nullptr;
   unsigned long long* counters = nullptr;
+  unsigned long long* counters_aux = nullptr;  // interaction counters of the potential walk (not reported)
   double* epart = nullptr;
   double* eout = nullptr;
   StepCtl* ctl = nullptr;
@@ -114,6 +116,7 @@ struct ihs_handle {
   bool zero_soft = false;               // some scenario has no softening: the walk skips self pairs by index
   int64_t force_evals = 0;
   bool have_fe = false;
+  bool pot_fresh = false;  // accg.w holds the potential of the last force evaluation
   // The last force evaluation solved h at the current positions (nothing has drifted since), so the next
   // evaluation's search radius only needs room for round-off.
   bool h_settled = false;
@@ -252,11 +255,13 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &s.vh, C));
   HIP_TRY(h, dalloc(h, &s.asph, C));
   HIP_TRY(h, dalloc(h, &s.accg, C));
+  HIP_TRY(h, dalloc(h, &s.apot, C));
   HIP_TRY(h, dalloc(h, &s.nbr, (size_t)C * h->cfg.nbr_capacity));
   HIP_TRY(h, dalloc(h, &s.ncount, C));
   if (h->cfg.nbr_capacity <= NMASK_WORDS * 32) HIP_TRY(h, dalloc(h, &s.nmask, (size_t)NMASK_WORDS * C));
   HIP_TRY(h, dalloc(h, &s.flags, FLAG_COUNT));
   HIP_TRY(h, dalloc(h, &s.counters, CNT_SLOTS * CNT_STRIDE));
+  HIP_TRY(h, dalloc(h, &s.counters_aux, CNT_SLOTS * CNT_STRIDE));
   HIP_TRY(h, dalloc(h, &s.epart, 9 * 1024));
   HIP_TRY(h, dalloc(h, &s.eout, 16));
   HIP_TRY(h, dalloc(h, &s.ctl, 1));
@@ -442,6 +447,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   }
   h->force_evals++;
   h->have_fe = true;
+  h->pot_fresh = need_pot;
   h->pos_fresh = true;
 }
 
@@ -467,12 +473,13 @@ void force_eval_velocity(ihs_handle* h) {
 
 void kdk_step(ihs_handle* h) {
   if (h->cfg.reuse_positions && h->pos_fresh) force_eval_velocity(h);
-  else force_eval(h, false, false);  // the potential is read only after the step's second evaluation
+  else force_eval(h, false, false);
   launch_kick_drift(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, seg_view(h));
   h->h_settled = false;
   h->pos_fresh = false;
   prof_mark(h, IHS_STAGE_KICK);
-  force_eval(h, true);
+  // The step needs accelerations only; the potential is a diagnostic, summed when an energy is read (potential()).
+  force_eval(h, true, false);
   launch_kick(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, seg_view(h));
   prof_mark(h, IHS_STAGE_KICK);
   prof_flush(h);
@@ -798,6 +805,7 @@ int ihs_step_kdk(ihs_handle* h, int32_t substeps) {
     h->cur = h->graph_next[par];
     h->force_evals += 2;
     h->have_fe = true;
+    h->pot_fresh = false;
   }
   return IHS_OK;
 }
@@ -873,11 +881,23 @@ int ihs_sync(ihs_handle* h) {
   return IHS_OK;
 }
 
+// The potential of the last force evaluation's positions (in .w): accg when that evaluation summed it; otherwise one
+// gravity walk into apot over that evaluation's tree and walk groups, which still describe the positions (a step ends
+// with a kick, which moves only v and u). That walk does not open the nodes the neighbour search needs, so its
+// potential is the plain walk's at the same theta; its interaction counts are not reported.
+const float4* potential(ihs_handle* h) {
+  if (h->pot_fresh) return h->sc.accg;
+  hipMemsetAsync(h->tb.wq, 0, 8 * sizeof(uint32_t), h->stream);
+  launch_gravity_walk(h->stream, h->n, h->cfg.leaf_size, h->split != 0, h->sc.xm, h->tb, h->tb.ggroups,
+                      h->tb.nggroups, seg_view(h), h->sc.apot, h->sc.flags, h->sc.counters_aux, true, h->zero_soft);
+  return h->sc.apot;
+}
+
 int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset) {
   if (!h || !out) return IHS_ERR_ARG;
   hipSetDevice(h->device);
   std::memset(out, 0, sizeof *out);
-  if (h->have_fe) launch_energy(h->stream, h->split, h->n, h->st[h->cur], h->sc.accg, h->sc.epart, h->sc.eout);
+  if (h->have_fe) launch_energy(h->stream, h->split, h->n, h->st[h->cur], potential(h), h->sc.epart, h->sc.eout);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   double e[9] = {0};
   if (h->have_fe) HIP_TRY(h, hipMemcpy(e, h->sc.eout, sizeof e, hipMemcpyDeviceToHost));
@@ -945,7 +965,7 @@ int ihs_segment_energy(ihs_handle* h, int32_t seg, double* out9) {
   V.vy = S.vy + t0;
   V.vz = S.vz + t0;
   V.m = S.m + t0;
-  launch_energy(h->stream, h->split, t1 - t0, V, h->sc.accg + t0, h->sc.epart, h->sc.eout);
+  launch_energy(h->stream, h->split, t1 - t0, V, potential(h) + t0, h->sc.epart, h->sc.eout);
   HIP_TRY(h, hipGetLastError());
   HIP_TRY(h, hipStreamSynchronize(h->stream));
   HIP_TRY(h, hipMemcpy(out9, h->sc.eout, 9 * sizeof(double), hipMemcpyDeviceToHost));

@@ -84,7 +84,8 @@ typedef struct {
 typedef struct ihs_handle ihs_handle;
 
 typedef struct {
-  double ke, ie, pe;      /* FP64: kinetic, internal, gravitational (1/2 sum m phi, tree potential) */
+  double ke, ie, pe;      /* FP64: kinetic, internal, gravitational (1/2 sum m phi, tree potential; after KDK
+                             steps, which sum no potential, one gravity walk over the last tree forms it) */
   double p[3];            /* FP64 momentum */
   double L[3];            /* FP64 angular momentum about the origin */
   double t;               /* simulated time (scenario 0 of a batch) */
@@ -123,7 +124,8 @@ enum {
   IHS_FIELD_RHO = 1,     /* float */
   IHS_FIELD_RPC = 2,     /* float4: rho, P/rho^2, c, P */
   IHS_FIELD_ASPH = 3,    /* float4: SPH acceleration xyz, du/dt */
-  IHS_FIELD_AGRAV = 4,   /* float4: gravitational acceleration xyz, potential */
+  IHS_FIELD_AGRAV = 4,   /* float4: gravitational acceleration xyz, potential (after ihs_force_eval or ihs_relax;
+                            KDK steps sum no potential and leave 0) */
   IHS_FIELD_HITER = 5,   /* uint32: Newton updates in the last h-solve */
   IHS_FIELD_NCOUNT = 6,  /* uint32: neighbour-list length */
   IHS_FIELD_FEAT = 7,    /* float4: n_ngb within h, div v, |grad rho|, 0 */
