@@ -384,8 +384,8 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   const float eff_margin = seeded ? h->cfg.h_margin_seeded : margin;
   const float diag_scale = newton ? h->cfg.h_margin / eff_margin : 1.0f;
   launch_karras(s, n, K, h->tb, sv);
-  launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb);
-  launch_groups(s, n, K, h->tb, sv, nullptr, NBR_DIAG_HS * diag_scale);
+  launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, nullptr, NBR_DIAG_HS * diag_scale);
+  launch_groups(s, n, K, h->tb, sv, nullptr, NBR_DIAG_HS * diag_scale, false, true);
   if (h->tb.ggroups != h->tb.groups) launch_groups(s, n, K, h->tb, sv, nullptr, GROUP_DIAG_HS * diag_scale, true);
   prof_mark(h, IHS_STAGE_TREE);
   // One traversal for gravity and the neighbour lists when the walk and neighbour groups coincide: the walk opens
@@ -421,8 +421,8 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
       // Wider-radius pass, gated on the device: every kernel returns at once unless a solve hit the tight radius.
       const uint32_t* gate = sc.flags + FLAG_H_RETRY;
       launch_retry_hs(s, n, D.h, sc.hs, h->cfg.h_margin, sv, sc.flags);
-      launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, gate);
-      launch_groups(s, n, K, h->tb, sv, gate, NBR_DIAG_HS);
+      launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, gate, NBR_DIAG_HS);
+      launch_groups(s, n, K, h->tb, sv, gate, NBR_DIAG_HS, false, true);
       launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags,
                        sc.counters, gate);
       launch_hsolve(s, n, sc.xm, sc.xlo, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol,

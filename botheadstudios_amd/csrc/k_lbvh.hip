@@ -545,6 +545,13 @@ __device__ __forceinline__ NodeVals merge_pair(bool held_left, const NodeVals& h
   return merge_vals(sel(held_left, held, sib), sel(held_left, sib, held), inv_theta);
 }
 
+// A walk group's box bound: diagonal at most diag_hs times the node's largest search radius (none: no bound).
+__device__ __forceinline__ bool group_box_ok(const float4 b0, const float4 b1, float diag_hs) {
+  if (!(b0.w > 0.f)) return true;
+  const float dx = b1.x - b0.x, dy = b1.y - b0.y, dz = b1.z - b0.z;
+  return dx * dx + dy * dy + dz * dz <= (diag_hs * b0.w) * (diag_hs * b0.w);
+}
+
 // Parent p of `from`, whose values the caller holds: the sibling is loaded and the children merged in order.
 __device__ __forceinline__ NodeVals merge_with_sibling(const NodeVals& cur, uint32_t from, const int2 ch,
                                                        const float4* ncom, const float4* nbmin, const float4* nbmax,
@@ -617,10 +624,16 @@ struct UbLds {
   float2 lqb[KONE ? 1 : BK];
   int2 child[BK];
   uint32_t parent[BK];
+  uint32_t icount[BK];  // internal nodes' leaf counts (walk-group marking)
   uint32_t arrive[BK];
   uint32_t roots[BK];
   uint32_t wcnt[BK / 64];
 };
+
+// A leaf of K particles is within the walk-group size limit.
+__device__ __forceinline__ bool hc_leaf_ok(int64_t c, int K, int64_t n, int gmax) {
+  return min((c + 1) * K, n) - c * K <= (int64_t)gmax;
+}
 
 // Phase 1. Workgroup b takes the block roots whose first leaf lies in leaves [b BK, (b + 1) BK), one after another.
 // For a root over leaves [f, l], thread t sums leaf f + t (stored to the node arrays as well as to LDS) and climbs:
@@ -635,7 +648,8 @@ __global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_bl
     const int2* __restrict__ child, const int2* __restrict__ range, const uint32_t* __restrict__ parent,
     const uint32_t* __restrict__ bstart, float4* __restrict__ ncom, float4* __restrict__ nbmin,
     float4* __restrict__ nbmax, float4* __restrict__ nqa, float2* __restrict__ nqb, float inv_theta,
-    uint32_t* __restrict__ pend, uint32_t* __restrict__ npend, const uint32_t* __restrict__ gate) {
+    uint32_t* __restrict__ pend, uint32_t* __restrict__ npend, const uint32_t* __restrict__ gate,
+    uint32_t* __restrict__ gstart, float diag_hs, int gmax) {
   constexpr int BK = KONE ? UB_LEAVES_1 : UB_LEAVES_K;
   if (gate && *gate == 0u) return;
   __shared__ UbLds<BK, KONE> S;
@@ -667,9 +681,10 @@ __global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_bl
     }
     const int64_t cnt = l - f + 1;
     NodeVals cur;
-    uint32_t from = IHS_NONE, p = IHS_NONE;
+    uint32_t from = IHS_NONE, p = IHS_NONE, p0 = IHS_NONE, hc = 0;
     if (t < cnt) {
       const int64_t x = f + t;
+      hc = (uint32_t)(min((x + 1) * K, n) - x * K);
       if (x < L - 1) {
         S.child[t] = child[x];
         S.parent[t] = parent[x];
@@ -679,6 +694,7 @@ __global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_bl
       from = (uint32_t)(L - 1 + x);
       store_node(from, cur, ncom, nbmin, nbmax, nqa, nqb);
       p = parent[from];
+      p0 = p;
       S.lc[t] = cur.c;
       S.lb0[t] = cur.b0;
       S.lb1[t] = cur.b1;
@@ -698,6 +714,7 @@ __global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_bl
           S.ib1[fi] = cur.b1;
           S.iqa[fi] = cur.qa;
           S.iqb[fi] = cur.qb;
+          S.icount[fi] = hc;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
         const uint32_t old = __hip_atomic_fetch_add(&S.arrive[pi], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -708,19 +725,60 @@ __global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_bl
         const uint32_t sib = left ? (uint32_t)ch.y : (uint32_t)ch.x;
         NodeVals sv;
         if (sib >= Lm1) {
-          const int si = (int)((int64_t)(sib - Lm1) - f);
+          const int64_t sc = (int64_t)(sib - Lm1);
+          const int si = (int)(sc - f);
           sv = NodeVals{S.lc[si], S.lb0[si], S.lb1[si], KONE ? make_float4(0.f, 0.f, 0.f, 0.f) : S.lqa[si],
                         KONE ? make_float2(0.f, 0.f) : S.lqb[si]};
+          hc += (uint32_t)(min((sc + 1) * K, n) - sc * K);
         } else {
           const int si = (int)((int64_t)sib - f);
           sv = NodeVals{S.ic[si], S.ib0[si], S.ib1[si], S.iqa[si], S.iqb[si]};
+          hc += S.icount[si];
         }
         cur = merge_pair(left, cur, sv, inv_theta);
         store_node(p, cur, ncom, nbmin, nbmax, nqa, nqb);
         from = p;
         p = S.parent[pi];
       }
-      if (from == r && p != IHS_NONE) pend[atomicAdd(npend, 1u)] = p;
+      if (from == r) {
+        if (p != IHS_NONE) pend[atomicAdd(npend, 1u)] = p;
+        if (gmax > 0 && r < Lm1) {
+          const int ri = (int)((int64_t)r - f);
+          S.ib0[ri] = cur.b0;
+          S.ib1[ri] = cur.b1;
+          S.icount[ri] = hc;
+        }
+      }
+    }
+    if (gmax > 0) {
+      // Walk groups (as k_mark_groups, from LDS): the topmost node on leaf c's root path holding at most gmax
+      // particles whose box passes group_box_ok; its first leaf records it. Such a node lies inside the block root
+      // (the root's parent holds more than BK >= gmax leaves).
+      __syncthreads();
+      if (t < cnt) {
+        const int64_t c = f + t;
+        uint32_t best = IHS_NONE;
+        bool first = true, best_first = false;
+        if (hc_leaf_ok(c, K, n, gmax)) {
+          if (group_box_ok(S.lb0[t], S.lb1[t], diag_hs)) {
+            best = (uint32_t)(L - 1 + c);
+            best_first = true;
+          }
+          uint32_t at = (uint32_t)(L - 1 + c), x = p0;
+          while (at != r && x != IHS_NONE) {
+            const int xi = (int)((int64_t)x - f);
+            if (S.icount[xi] > (uint32_t)gmax) break;
+            first = first && (uint32_t)S.child[xi].x == at;
+            if (group_box_ok(S.ib0[xi], S.ib1[xi], diag_hs)) {
+              best = x;
+              best_first = first;
+            }
+            at = x;
+            x = S.parent[xi];
+          }
+        }
+        gstart[c] = best == IHS_NONE ? GROUP_SPLIT : (best_first ? best : IHS_NONE);
+      }
     }
     __syncthreads();
   }
@@ -815,10 +873,7 @@ __device__ __forceinline__ bool group_ok(const int2* __restrict__ range, int64_t
                                          float diag_hs, int gmax) {
   int64_t first;
   if (node_count(range, L, K, n, x, first) > gmax) return false;
-  const float4 b0 = nbmin[x], b1 = nbmax[x];
-  if (!(b0.w > 0.f)) return true;
-  const float dx = b1.x - b0.x, dy = b1.y - b0.y, dz = b1.z - b0.z;
-  return dx * dx + dy * dy + dz * dz <= (diag_hs * b0.w) * (diag_hs * b0.w);
+  return group_box_ok(nbmin[x], nbmax[x], diag_hs);
 }
 
 // One thread per leaf: the topmost acceptable node on the leaf's root path is its group; the group's first
@@ -1025,32 +1080,33 @@ void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView&
 }
 
 void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv, const uint32_t* gate,
-                   float diag_hs, bool gravity) {
+                   float diag_hs, bool gravity, bool marked) {
   const int64_t L = (n + K - 1) / K;
   const int nblk = (int)grid_for(L, GRP_BLK);
   const int gmax = gravity ? tb.group_max : GROUP_MAX;
   int4* out = gravity ? tb.ggroups : tb.groups;
   uint32_t* nout = gravity ? tb.nggroups : tb.ngroups;
-  k_mark_groups<<<grid_for(L, 256), 256, 0, s>>>(n, K, L, tb.range, tb.parent, tb.nbmin, tb.nbmax, tb.gstart, gate,
-                                                 diag_hs, gmax);
+  if (!marked)
+    k_mark_groups<<<grid_for(L, 256), 256, 0, s>>>(n, K, L, tb.range, tb.parent, tb.nbmin, tb.nbmax, tb.gstart, gate,
+                                                   diag_hs, gmax);
   k_group_count<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.gblk, gate);
   k_group_offsets<<<1, GRP_BLK, 0, s>>>(tb.gblk, nblk, nout, tb.wq, gate);
   k_group_emit<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.range, tb.gblk, out, gate, sv.segof, sv.root);
 }
 
 void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb,
-                    const uint32_t* gate) {
+                    const uint32_t* gate, float group_diag_hs) {
   const int64_t L = (n + K - 1) / K;
   const float inv_theta = theta > 0.f ? 1.0f / theta : 3.0e38f;
   if (gate) k_tree_rearm<<<grid_for(L, 256), 256, 0, s>>>(L, tb.ready, tb.npend, gate);
   if (K == 1)
     k_upsweep_blocks<true><<<grid_for(L, UB_LEAVES_1), UB_LEAVES_1, 0, s>>>(
         xm, hs, n, K, L, tb.child, tb.range, tb.parent, tb.bstart, tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb,
-        inv_theta, tb.pend, tb.npend, gate);
+        inv_theta, tb.pend, tb.npend, gate, tb.gstart, group_diag_hs, group_diag_hs > 0.f ? GROUP_MAX : 0);
   else
     k_upsweep_blocks<false><<<grid_for(L, UB_LEAVES_K), UB_LEAVES_K, 0, s>>>(
         xm, hs, n, K, L, tb.child, tb.range, tb.parent, tb.bstart, tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb,
-        inv_theta, tb.pend, tb.npend, gate);
+        inv_theta, tb.pend, tb.npend, gate, tb.gstart, group_diag_hs, group_diag_hs > 0.f ? GROUP_MAX : 0);
   if (L > tb.ub_leaves)
     k_upsweep_top<<<1, 1024, 0, s>>>(tb.pend, tb.npend, tb.child, tb.parent, tb.ready, tb.ncom, tb.nbmin, tb.nbmax,
                                      tb.nqa, tb.nqb, inv_theta, gate);

@@ -145,13 +145,16 @@ void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView&
 // diagonal is at most
 // diag_hs times its largest search radius (no radius: no box limit); leaves without one are split into
 // single-particle groups. Needs the upsweep's boxes.
-// Groups record their segment's root (int4.w), where their walks start.
+// Groups record their segment's root (int4.w), where their walks start. marked: the upsweep has already marked the
+// neighbour-search groups (GROUP_MAX, this diag_hs) in tb.gstart.
 void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv, const uint32_t* gate = nullptr,
-                   float diag_hs = GROUP_DIAG_HS, bool gravity = false);
+                   float diag_hs = GROUP_DIAG_HS, bool gravity = false, bool marked = false);
 // Leaf moments and ordered upsweep; xm = sorted (x,y,z,m), hs = sorted search radius (or null). With a gate
-// the arrival counters are re-armed first (a second upsweep over the same topology).
+// the arrival counters are re-armed first (a second upsweep over the same topology). group_diag_hs > 0: the upsweep
+// also marks the neighbour-search groups (GROUP_MAX particles, that box bound) in tb.gstart, for launch_groups
+// with marked = true.
 void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb,
-                    const uint32_t* gate = nullptr);
+                    const uint32_t* gate = nullptr, float group_diag_hs = 0.f);
 
 // --- k_gravity.hip ---
 // Wave-coherent Barnes-Hut group walk over sorted particles; accg = (a/G xyz... see kernel), sorted order.
