@@ -221,7 +221,6 @@ int alloc_all(ihs_handle* h) {
   }
   HIP_TRY(h, dalloc(h, &t.hist, (size_t)256 * t.nb_cap));
   HIP_TRY(h, dalloc(h, &t.dtot, 256));
-  HIP_TRY(h, dalloc(h, &t.ccode, Lc));
   HIP_TRY(h, dalloc(h, &t.child, Lc));
   HIP_TRY(h, dalloc(h, &t.range, Lc));
   HIP_TRY(h, dalloc(h, &t.parent, 2 * Lc));
@@ -365,8 +364,12 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   const float eta = std::cbrt(3.0f * h->cfg.n_ngb / (4.0f * kPi));
   const SegView sv = seg_view(h);
   prof_mark(h, -1);
-  launch_fe_init(s, sc.flags, sv);
-  launch_bbox_morton(s, h->split, S.px, S.py, S.pz, n, h->tb, sv);
+  EvalInit fe;
+  fe.flags = sc.flags;
+  fe.clk = sv.clk;
+  fe.hs_npend = h->hw.npend;
+  fe.n_hs_npend = PEND_SHARDS;
+  launch_bbox_morton(s, h->split, S.px, S.py, S.pz, n, K, h->tb, sv, fe);
   launch_radix_sort(s, n, 30, h->tb);
   launch_segment_order(s, n, h->tb, sv);
   launch_fix_ties(s, n, S.id, h->tb, sc.flags);
@@ -414,7 +417,8 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
     const float eta3 = eta * eta * eta;
     if (record) mlp_record_features(s, h->mlp, n, D, sc.eos, sv, eta);
     launch_hsolve(s, n, sc.xm, sc.xlo, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol,
-                  h->cfg.h_max_iter, h->hw, sc.flags, sc.counters, sv, nullptr, seeded ? sc.flags + FLAG_H_RETRY : nullptr);
+                  h->cfg.h_max_iter, h->hw, sc.flags, sc.counters, sv, nullptr, seeded ? sc.flags + FLAG_H_RETRY : nullptr,
+                  false);  // k_bbox cleared the pending lists at the start of this evaluation
     if (record) mlp_record_targets(s, h->mlp, n, D);
     prof_mark(h, IHS_STAGE_HSOLVE);
     if (seeded) {
@@ -1039,7 +1043,7 @@ int ihs_gravity(ihs_handle* h, const double* pos3, const double* mass, int64_t n
     gv.bbox = h->tb.bbox;
     gv.root = sc.g_root;
     launch_fe_init(s, sc.flags, gv);
-    launch_bbox_morton(s, 1, sc.gpx, sc.gpy, sc.gpz, n, h->tb, gv);
+    launch_bbox_morton(s, 1, sc.gpx, sc.gpy, sc.gpz, n, K, h->tb, gv);
     launch_radix_sort(s, n, 30, h->tb);  // identity payload: ties already ordered by index
     launch_gather_xm(s, h->tb.perm, n, sc.gpx, sc.gpy, sc.gpz, sc.gm, sc.xm);
     launch_karras(s, n, K, h->tb, gv);
@@ -1067,7 +1071,9 @@ int64_t ihs_debug_tree(ihs_handle* h, uint32_t* sorted_codes, uint32_t* order, u
   const int64_t n = h->n, K = h->cfg.leaf_size, L = (n + K - 1) / K;
   if (sorted_codes) HIP_TRY(h, hipMemcpy(sorted_codes, h->tb.skeys, n * 4, hipMemcpyDeviceToHost));
   if (order) HIP_TRY(h, hipMemcpy(order, h->st[h->cur].id, n * 4, hipMemcpyDeviceToHost));
-  if (ccodes) HIP_TRY(h, hipMemcpy(ccodes, h->tb.ccode, L * 4, hipMemcpyDeviceToHost));
+  // Leaf c's cluster code is the first sorted code of its bucket.
+  if (ccodes)
+    HIP_TRY(h, hipMemcpy2D(ccodes, 4, h->tb.skeys, (size_t)4 * h->cfg.leaf_size, 4, (size_t)L, hipMemcpyDeviceToHost));
   std::vector<int2> ch(L > 1 ? L - 1 : 1);
   if (L > 1) HIP_TRY(h, hipMemcpy(ch.data(), h->tb.child, (L - 1) * sizeof(int2), hipMemcpyDeviceToHost));
   for (int64_t i = 0; i < 2 * L - 1; ++i) {

@@ -52,6 +52,14 @@ struct SegClock {
   uint32_t pad;
 };
 
+// Per-evaluation resets folded into the first kernel of a force evaluation (k_bbox's block 0).
+struct EvalInit {
+  uint32_t* flags = nullptr;     // FLAG_H_RETRY and FLAG_TIE_RUNS are cleared (null: nothing is reset)
+  SegClock* clk = nullptr;       // every scenario's Courant signal is re-armed
+  uint32_t* hs_npend = nullptr;  // the h-solve's pending-list lengths are cleared (n_hs_npend of them)
+  int n_hs_npend = 0;
+};
+
 struct SegView {
   int S = 1;
   const uint32_t* segof = nullptr;  // n: segment of storage position t (null when S == 1)
@@ -75,7 +83,6 @@ struct TreeBufs {
   uint32_t* vals[2] = {nullptr, nullptr};
   uint32_t* hist = nullptr;  // 256 * nb
   uint32_t* dtot = nullptr;  // 256
-  uint32_t* ccode = nullptr; // L
   int2* child = nullptr;     // L-1 (left, right)
   int2* range = nullptr;     // L-1 (first, last leaf)
   uint32_t* parent = nullptr;  // 2L-1
@@ -125,8 +132,8 @@ inline int radix_tiles(int64_t n) { return (int)((n + RADIX_TILE - 1) / RADIX_TI
 // --- k_lbvh.hip ---
 // bbox + Morton codes from positions (PT = float or double, SoA), vals = identity.
 // Per-segment bbox (sv.bbox) and codes normalised in each segment's box.
-void launch_bbox_morton(hipStream_t s, int split, const void* px, const void* py, const void* pz, int64_t n,
-                        TreeBufs& tb, const SegView& sv);
+void launch_bbox_morton(hipStream_t s, int split, const void* px, const void* py, const void* pz, int64_t n, int K,
+                        TreeBufs& tb, const SegView& sv, const EvalInit& fe = EvalInit{});
 // S > 1: stable pass by segment after the Morton passes, so the order is (segment, code); re-gathers skeys.
 void launch_segment_order(hipStream_t s, int64_t n, TreeBufs& tb, const SegView& sv);
 // Stable LSD radix sort of (keys[0], vals[0]) over `bits` low bits; sets tb.skeys / tb.perm.
@@ -207,7 +214,7 @@ void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const uint32_t* x
                    uint8_t* iters,
                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol, int max_iter,
                    const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters, const SegView& sv,
-                   const uint32_t* gate = nullptr, uint32_t* retry = nullptr);
+                   const uint32_t* gate = nullptr, uint32_t* retry = nullptr, bool clear_pending = true);
 // Wider-radius pass set-up (gated): hs = min(h * margin, max(hcap, h)); counts the pass in FLAG_RETRIES.
 void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float margin, const SegView& sv,
                      uint32_t* flags);
