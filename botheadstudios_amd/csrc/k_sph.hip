@@ -866,10 +866,12 @@ __global__ __launch_bounds__(256) void k_forces(const float4* __restrict__ xm, c
     const float vr = dvx * dx + dvy * dy + dvz * dz;
     float pij = 0.f;
     if (vr < 0.f) {
-      const float mu = hij * vr / (r2 + 0.01f * hij * hij);
+      // Monaghan's viscosity; the two quotients use the hardware reciprocal (1 ulp) instead of IEEE divisions,
+      // which were a fifth of the pair's instructions.
+      const float mu = hij * vr * __builtin_amdgcn_rcpf(r2 + 0.01f * hij * hij);
       const float cbar = 0.5f * (ri.z + rj.z);
       const float rbar = 0.5f * (ri.x + rj.x);
-      pij = (-alpha * cbar * mu + beta * mu * mu) / rbar;
+      pij = (-alpha * cbar * mu + beta * mu * mu) * __builtin_amdgcn_rcpf(rbar);
     }
     const float coeff = ri.y + rj.y + pij;
     const float hinv = __builtin_amdgcn_rcpf(hij);
