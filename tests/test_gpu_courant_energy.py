@@ -171,3 +171,39 @@ def test_potential_energy_of_a_uniform_sphere():
     _, phi = oracle.direct_f64(pos, np.full(n, M / n), soft, with_pot=True)
     assert pe == pytest.approx(0.5 * (M / n) * float(phi.sum()), rel=1e-3)
     assert math.isfinite(pe)
+
+
+@pytest.mark.parametrize("precision", ["fp32", "split"])
+def test_energy_read_after_steps_forms_the_potential(precision):
+    """KDK steps sum no potential; an energy read after them forms it with one gravity walk over the last tree. Its PE
+    equals the oracle's FP64 direct-sum potential of the same positions to the walk's tolerance, and the PE an explicit
+    force evaluation at those positions sums (the fused walk's potential); the read leaves the trajectory untouched: a
+    twin that does not read the energy between steps ends in the same state, bit for bit."""
+    ic = ics.config_ic(1, n=8192)
+
+    def run(mode):
+        with GpuSph(ic.n, leaf_size=1, precision=precision, h_mode="newton", theta=0.9) as sph:
+            sph.upload(ic.particles48(), ic.eos, ic.softening)
+            sph.set_dt(0.0)
+            sph.encode_kdk(3)
+            if mode == "force_eval":
+                sph.force_eval()  # the same positions: the fused walk sums the potential
+                return sph.stats(), None, None
+            st = sph.stats() if mode == "read" else None
+            pos = sph.field("pos64")
+            sph.encode_kdk(2)
+            return st, pos, sph.readback()
+
+    st, pos, back = run("read")
+    _, _, back_twin = run("no read")
+    st_fe, _, _ = run("force_eval")
+    m = ic.m.astype(np.float32).astype(np.float64)
+    _, phi = oracle.direct_f64(pos, m, float(np.float32(ic.softening)), with_pot=True)
+    pe_ref = 0.5 * float((m * phi).sum())
+    assert st["pe"] < 0
+    assert st["pe"] == pytest.approx(pe_ref, rel=1e-3)
+    assert st["pe"] == pytest.approx(st_fe["pe"], rel=1e-3)
+    assert (st["ke"], st["ie"]) == (st_fe["ke"], st_fe["ie"])
+    for k in back.dtype.names:
+        a, b = np.ascontiguousarray(back[k]), np.ascontiguousarray(back_twin[k])
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), k
