@@ -19,7 +19,9 @@ ARCH = os.environ.get("IHS_OFFLOAD_ARCH", "gfx950")
 SOURCES = ["k_lbvh.hip", "k_gravity.hip", "k_sph.hip", "k_step.hip", "k_mlp.hip", "k_measure.hip", "ihs_api.hip"]
 
 # Per-file flags: the measurement keeps the reference's f64 rounding (no fused multiply-add).
-EXTRA = {"k_measure.hip": ["-ffp-contract=off"]}
+# k_mlp: the inference kernels keep their matrix accumulators in ordinary VGPRs, which the GELU reads directly
+# (without the flag the compiler puts them in AGPRs and copies each one out: 64 extra moves per tile).
+EXTRA = {"k_measure.hip": ["-ffp-contract=off"], "k_mlp.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 # Packed FP32 issue is disabled for all device code; hipcc's host pass warns that it does not know
 # the feature, which is harmless.

@@ -1,9 +1,9 @@
 // Learned smoothing-length seed: BF16-weight / FP32-accumulate inference and training kernels (gfx950).
 //
-// Inference runs one particle per thread with the weights expanded from BF16 into LDS. Training computes the
-// exact gradient of the mean squared error (forward, backward, then per-parameter wave reductions in a fixed
-// order, so the gradient is deterministic) and applies Adam to FP32 master weights; the forward passes always
-// use the BF16-rounded weights, exactly as inference does.
+// Inference runs on the matrix cores, 32 particles per wave, with the weights expanded from BF16 into LDS. Training
+// (one particle per thread) computes the exact gradient of the mean squared error (forward, backward, then
+// per-parameter wave reductions in a fixed order, so the gradient is deterministic) and applies Adam to FP32 master
+// weights; the forward passes always use the BF16-rounded weights, as inference does.
 #include "common.h"
 #include "mlp.h"
 
@@ -19,13 +19,18 @@ __device__ __forceinline__ uint16_t f32_to_bf16(float f) {  // round to nearest 
   return (uint16_t)(u >> 16);
 }
 
-// GELU (tanh form) and its derivative. tanh u = 1 - 2 / (e^{2u} + 1), with the hardware reciprocal (1 ulp; the
-// IEEE division was a third of the inference kernel's instructions); it saturates to +-1 as e^{2u} leaves range.
+// GELU (tanh form) and its derivative. With u = K (z + C z^3), 0.5 z (1 + tanh u) = z s, s = 1 / (1 + e^{-2u}) =
+// 1 / (1 + 2^{z (g1 + g3 z^2)}), g1 = -2 K log2 e, g3 = g1 C: a multiply, an FMA and a multiply, then the hardware
+// exp2 and reciprocal (1 ulp each; an IEEE division here was a third of the inference kernel's instructions). Far
+// below zero the power overflows to +inf and s = 0, far above it s = 1: the function saturates without a NaN.
+constexpr float kGelu1 = -2.0f * kGeluK * 1.4426950408889634f;
+constexpr float kGelu3 = kGelu1 * kGeluC;
+
 __device__ __forceinline__ float gelu(float z, float& dg) {
-  const float u = kGeluK * (z + kGeluC * z * z * z);
-  const float t = 1.0f - 2.0f * __builtin_amdgcn_rcpf(__expf(2.0f * u) + 1.0f);
-  dg = 0.5f * (1.0f + t) + 0.5f * z * (1.0f - t * t) * kGeluK * (1.0f + 3.0f * kGeluC * z * z);
-  return 0.5f * z * (1.0f + t);
+  const float z2 = z * z;
+  const float sg = __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(z * fmaf(kGelu3, z2, kGelu1)));
+  dg = sg + 2.0f * z * sg * (1.0f - sg) * kGeluK * (1.0f + 3.0f * kGeluC * z2);
+  return z * sg;
 }
 
 __device__ __forceinline__ void load_weights(float* sw, const uint16_t* __restrict__ wb) {
@@ -55,6 +60,86 @@ __device__ __forceinline__ float forward(const float* sw, const float x[MLP_IN])
   return y;
 }
 
+// The forward pass on the matrix cores. v_mfma_f32_32x32x2f32 takes A[i][k] from lane i + 32 k and B[k][j] from lane
+// j + 32 k, and leaves D[i][j] in lane j + 32 (i / 4 mod 2), register 4 (i / 8) + i mod 4. Both layers are computed
+// transposed, hidden units as rows and the 32 particles of a pass as columns, so A holds weights (the same for every
+// pass: each lane keeps its 21 in registers) and B the particles' values. Layer 1 is W1, with b1 as the weight of a
+// constant input 1, times the inputs: K = 8 in 4 steps. Each lane then holds 16 hidden activations of its column's
+// particle, rows 8 (r / 4) + 4 half + r mod 4, which are exactly layer 2's B operands when layer 2's K index runs over
+// those rows in that order (16 steps, no lane exchange); b2 enters as the weight of a 17th step's constant input 1.
+// The output unit sums its lane's 16 rows and the other half's.
+// FP32 inputs and accumulation, the weights as the BF16 copy holds them; only the summation order differs from
+// forward().
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int W2T_STRIDE = 40;  // W2 transposed in LDS, rows of 40: a wave's two halves read 32 banks apart
+
+struct MfmaWeights {
+  float a1[4];   // layer 1's A operand for steps 0..3
+  float a2[17];  // layer 2's A operand for steps 0..16
+};
+
+// sw: the parameters in order; w2t[kin * W2T_STRIDE + row] = W2[row][kin].
+__device__ __forceinline__ void load_weights_mfma(float* sw, float* w2t, const uint16_t* __restrict__ wb) {
+  for (int p = threadIdx.x; p < MLP_NPARAM; p += blockDim.x) {
+    const float v = bf16_to_f32(wb[p]);
+    sw[p] = v;
+    const int q = p - MLP_W2;
+    if (q >= 0 && q < MLP_H * MLP_H) w2t[(q % MLP_H) * W2T_STRIDE + q / MLP_H] = v;
+  }
+  __syncthreads();
+}
+
+__device__ __forceinline__ MfmaWeights lane_weights(const float* sw, const float* w2t, int lane) {
+  const int half = lane >> 5, col = lane & 31;
+  MfmaWeights w;
+#pragma unroll
+  for (int s = 0; s < 4; ++s) {
+    const int k = 2 * s + half;  // the input this lane supplies
+    w.a1[s] = k < MLP_IN ? sw[MLP_W1 + col * MLP_IN + k] : (k == MLP_IN ? sw[MLP_B1 + col] : 0.f);
+  }
+#pragma unroll
+  for (int r = 0; r < 16; ++r) w.a2[r] = w2t[((r >> 2) * 8 + half * 4 + (r & 3)) * W2T_STRIDE + col];
+  w.a2[16] = half ? 0.f : sw[MLP_B2 + col];
+  return w;
+}
+
+// x: the inputs of particle (lane mod 32) of the pass, in both halves; returns that particle's output in both halves.
+__device__ __forceinline__ float forward_mfma(const float* sw, const MfmaWeights& w, const float x[MLP_IN], int half) {
+  const float xx[8] = {x[0], x[1], x[2], x[3], x[4], x[5], 1.0f, 0.0f};
+  f32x16 z1 = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+    z1 = __builtin_amdgcn_mfma_f32_32x32x2f32(w.a1[s], half ? xx[2 * s + 1] : xx[2 * s], z1, 0, 0, 0);
+  f32x16 z2 = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    float dg;
+    z2 = __builtin_amdgcn_mfma_f32_32x32x2f32(w.a2[r], gelu(z1[r], dg), z2, 0, 0, 0);
+  }
+  z2 = __builtin_amdgcn_mfma_f32_32x32x2f32(w.a2[16], half ? 0.f : 1.f, z2, 0, 0, 0);
+  float part = 0.f;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    float dg;
+    part = fmaf(sw[MLP_W3 + (r >> 2) * 8 + half * 4 + (r & 3)], gelu(z2[r], dg), part);
+  }
+  return sw[MLP_B3] + (part + __shfl_xor(part, 32, 64));
+}
+
+// A wave's 64 particles in two passes of 32: lane l holds particle l's inputs and receives particle l's output.
+__device__ __forceinline__ float forward_wave(const float* sw, const MfmaWeights& w, const float x[MLP_IN], int lane) {
+  const int half = lane >> 5, col = lane & 31;
+  float xa[MLP_IN], xb[MLP_IN];
+#pragma unroll
+  for (int j = 0; j < MLP_IN; ++j) {
+    xa[j] = __shfl(x[j], col, 64);
+    xb[j] = __shfl(x[j], col + 32, 64);
+  }
+  const float ya = forward_mfma(sw, w, xa, half);
+  const float yb = forward_mfma(sw, w, xb, half);
+  return half ? yb : ya;
+}
+
 __device__ __forceinline__ void features(int64_t t, const MasterBufs& D, const EosDev* __restrict__ eos, double dt,
                                          float eta, float x[MLP_IN], float& h) {
   h = D.h[t];
@@ -70,23 +155,34 @@ __device__ __forceinline__ void features(int64_t t, const MasterBufs& D, const E
   x[5] = (float)mat;
 }
 
+// Waves stride over tiles of 64 particles, one particle per lane (forward_wave); the grid is capped so that a wave
+// loads its weights once for several tiles. The loop bound is uniform over the wave: the matrix instructions need
+// every lane, and lanes past the end repeat the tile's first particle without writing.
+constexpr int MFMA_BLOCKS = 1024;
+
 __global__ __launch_bounds__(256) void k_mlp_seed(int64_t n, MasterBufs D, float* __restrict__ hs,
                                                   const EosDev* __restrict__ eos, SegView sv, float eta,
                                                   const uint16_t* __restrict__ wb, float margin) {
   __shared__ float sw[MLP_NPARAM];
-  load_weights(sw, wb);
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  const uint32_t sg = sv.of(t);
-  const float hcap = sv.hcap[sg];
-  float x[MLP_IN], h;
-  features(t, D, eos, sv.clk[sg].dt, eta, x, h);
-  const float y = forward(sw, x) + MLP_PHYS * x[0];
-  float seed = h * __expf(fminf(fmaxf(y, -50.f), 50.f) * (1.0f / MLP_YSCALE));
-  seed = fminf(fmaxf(seed, h * (1.0f / MLP_CLAMP)), h * MLP_CLAMP);
-  seed = fminf(seed, fmaxf(hcap, h));
-  D.h[t] = seed;
-  hs[t] = fminf(seed * margin, fmaxf(hcap, seed));
+  __shared__ float w2t[MLP_H * W2T_STRIDE];
+  load_weights_mfma(sw, w2t, wb);
+  const int lane = threadIdx.x & 63;
+  const MfmaWeights w = lane_weights(sw, w2t, lane);
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * 64;
+  for (int64_t tile = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; tile < n; tile += stride) {
+    const int64_t t = tile + lane < n ? tile + lane : tile;
+    const uint32_t sg = sv.of(t);
+    const float hcap = sv.hcap[sg];
+    float x[MLP_IN], h;
+    features(t, D, eos, sv.clk[sg].dt, eta, x, h);
+    const float y = forward_wave(sw, w, x, lane) + MLP_PHYS * x[0];
+    if (tile + lane >= n) continue;
+    float seed = h * __expf(fminf(fmaxf(y, -50.f), 50.f) * (1.0f / MLP_YSCALE));
+    seed = fminf(fmaxf(seed, h * (1.0f / MLP_CLAMP)), h * MLP_CLAMP);
+    seed = fminf(seed, fmaxf(hcap, h));
+    D.h[t] = seed;
+    hs[t] = fminf(seed * margin, fmaxf(hcap, seed));
+  }
 }
 
 __global__ __launch_bounds__(256) void k_mlp_features(int64_t n, MasterBufs D, const EosDev* __restrict__ eos,
@@ -215,16 +311,24 @@ __global__ __launch_bounds__(256) void k_mlp_to_bf16(const float* __restrict__ w
 __global__ __launch_bounds__(256) void k_mlp_forward(const uint16_t* __restrict__ wb, const float* __restrict__ x,
                                                      int64_t n, float* __restrict__ out) {
   __shared__ float sw[MLP_NPARAM];
-  load_weights(sw, wb);
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  float xx[MLP_IN];
+  __shared__ float w2t[MLP_H * W2T_STRIDE];
+  load_weights_mfma(sw, w2t, wb);
+  const int lane = threadIdx.x & 63;
+  const MfmaWeights w = lane_weights(sw, w2t, lane);
+  const int64_t stride = (int64_t)gridDim.x * (blockDim.x >> 6) * 64;
+  for (int64_t tile = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 64; tile < n; tile += stride) {
+    const int64_t t = tile + lane < n ? tile + lane : tile;
+    float xx[MLP_IN];
 #pragma unroll
-  for (int j = 0; j < MLP_IN; ++j) xx[j] = x[t * MLP_IN + j];
-  out[t] = forward(sw, xx);
+    for (int j = 0; j < MLP_IN; ++j) xx[j] = x[t * MLP_IN + j];
+    const float y = forward_wave(sw, w, xx, lane);
+    if (tile + lane < n) out[t] = y;
+  }
 }
 
 inline unsigned g256(int64_t n) { return (unsigned)((n + 255) / 256); }
+// Blocks of 4 waves, 64 particles per wave and tile, at most MFMA_BLOCKS blocks.
+inline unsigned gmfma(int64_t n) { return (unsigned)std::min<int64_t>((n + 255) / 256, MFMA_BLOCKS); }
 
 }  // namespace
 
@@ -255,7 +359,7 @@ void mlp_free(MlpDev& m) {
 void mlp_seed_h(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, float* hs, const void* eos,
                 const SegView& sv, float eta, float margin) {
   if (!m.trained) return;
-  k_mlp_seed<<<g256(n), 256, 0, s>>>(n, D, hs, (const EosDev*)eos, sv, eta, m.wb, margin);
+  k_mlp_seed<<<gmfma(n), 256, 0, s>>>(n, D, hs, (const EosDev*)eos, sv, eta, m.wb, margin);
 }
 
 void mlp_record_features(hipStream_t s, MlpDev& m, int64_t n, const MasterBufs& D, const void* eos, const SegView& sv,
@@ -282,7 +386,7 @@ void mlp_adam(hipStream_t s, MlpDev& m, float lr) {
 }
 
 void mlp_forward(hipStream_t s, MlpDev& m, const float* x, int64_t n, float* out) {
-  k_mlp_forward<<<g256(n), 256, 0, s>>>(m.wb, x, n, out);
+  k_mlp_forward<<<gmfma(n), 256, 0, s>>>(m.wb, x, n, out);
 }
 
 void mlp_sync_bf16(hipStream_t s, MlpDev& m) { k_mlp_to_bf16<<<g256(MLP_NPARAM), 256, 0, s>>>(m.w, m.wb); }
