@@ -123,6 +123,9 @@ struct ihs_handle {
   // The last full force evaluation saw the current positions (no drift since): its tree, neighbour lists,
   // density and gravity still hold.
   bool pos_fresh = false;
+  // tb holds the sort and topology of st[cur]'s positions, in st[cur]'s order (set by every force evaluation,
+  // cleared by whatever moves, reorders or replaces the particles or builds another tree in tb).
+  bool tree_fresh = false;
   HsolveScratch hw;
   // One captured KDK step per starting parity of the ping-pong state buffers.
   hipGraph_t graph[2] = {nullptr, nullptr};
@@ -350,12 +353,16 @@ constexpr float kMarginSettled = 1.02f;
 // after_drift: the positions moved since the previous evaluation (the second evaluation of a KDK step). Then a
 // trained network seeds the h-solve (LEARNED) and sets a tight search radius around its seed, with a wider-radius
 // pass that runs only if some solve reached that radius; a recording run stores (features, converged h) rows.
+// At positions the previous evaluation sorted, none moved since (the first evaluation of a KDK step: the kick
+// changed only v and u; or a repeated evaluation), the sort would return the identity and the Karras pass the same
+// topology: both are kept and the state stays in its buffers (k_stage), with results identical to sorting again.
 void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   hipStream_t s = h->stream;
   const int64_t n = h->n;
   const int K = h->cfg.leaf_size;
+  const bool sorted = h->tree_fresh && !after_drift;
   MasterBufs& S = h->st[h->cur];
-  MasterBufs& D = h->st[h->cur ^ 1];
+  MasterBufs& D = sorted ? S : h->st[h->cur ^ 1];
   Scratch& sc = h->sc;
   const bool kahan = h->split != 0;
   const bool newton = h->cfg.h_mode != IHS_H_FIXED;
@@ -369,15 +376,20 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   fe.clk = sv.clk;
   fe.hs_npend = h->hw.npend;
   fe.n_hs_npend = PEND_SHARDS;
-  launch_bbox_morton(s, h->split, S.px, S.py, S.pz, n, K, h->tb, sv, fe);
-  launch_radix_sort(s, n, 30, h->tb);
-  launch_segment_order(s, n, h->tb, sv);
-  launch_fix_ties(s, n, S.id, h->tb, sc.flags);
-  prof_mark(h, IHS_STAGE_SORT);
   float margin = 1.0f;
   if (newton) margin = (!after_drift && h->h_settled) ? kMarginSettled : h->cfg.h_margin;
-  launch_gather(s, h->split, n, h->tb.perm, S, D, sc.xm, sc.vh, sc.hs, margin, sv, sc.xlo);
-  h->cur ^= 1;
+  if (sorted) {
+    launch_stage(s, h->split, n, S, sc.xm, sc.vh, sc.hs, margin, sv, sc.xlo, fe, (n + K - 1) / K, h->tb.ready,
+                 h->tb.npend);
+  } else {
+    launch_bbox_morton(s, h->split, S.px, S.py, S.pz, n, K, h->tb, sv, fe);
+    launch_radix_sort(s, n, 30, h->tb);
+    launch_segment_order(s, n, h->tb, sv);
+    launch_fix_ties(s, n, S.id, h->tb, sc.flags);
+    prof_mark(h, IHS_STAGE_SORT);
+    launch_gather(s, h->split, n, h->tb.perm, S, D, sc.xm, sc.vh, sc.hs, margin, sv, sc.xlo);
+    h->cur ^= 1;
+  }
   prof_mark(h, IHS_STAGE_GATHER);
   if (seeded) {
     mlp_seed_h(s, h->mlp, n, D, sc.hs, sc.eos, sv, eta, h->cfg.h_margin_seeded);
@@ -386,7 +398,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   // Walk groups are bounded in units of h, whatever margin set this evaluation's search radius.
   const float eff_margin = seeded ? h->cfg.h_margin_seeded : margin;
   const float diag_scale = newton ? h->cfg.h_margin / eff_margin : 1.0f;
-  launch_karras(s, n, K, h->tb, sv);
+  if (!sorted) launch_karras(s, n, K, h->tb, sv);
   launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, nullptr, NBR_DIAG_HS * diag_scale);
   launch_groups(s, n, K, h->tb, sv, nullptr, NBR_DIAG_HS * diag_scale, false, true);
   if (h->tb.ggroups != h->tb.groups) launch_groups(s, n, K, h->tb, sv, nullptr, GROUP_DIAG_HS * diag_scale, true);
@@ -453,6 +465,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   h->have_fe = true;
   h->pot_fresh = need_pot;
   h->pos_fresh = true;
+  h->tree_fresh = true;
 }
 
 // Velocity-only force evaluation (configuration reuse_positions): the first evaluation of a KDK step sees the
@@ -481,6 +494,7 @@ void kdk_step(ihs_handle* h) {
   launch_kick_drift(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, seg_view(h));
   h->h_settled = false;
   h->pos_fresh = false;
+  h->tree_fresh = false;
   prof_mark(h, IHS_STAGE_KICK);
   // The step needs accelerations only; the potential is a diagnostic, summed when an energy is read (potential()).
   force_eval(h, true, false);
@@ -663,6 +677,7 @@ int ihs_upload_batch(ihs_handle* h, const ihs_particle* p, const int64_t* counts
   h->have_fe = false;
   h->h_settled = false;
   h->pos_fresh = false;
+  h->tree_fresh = false;
   Scratch& sc = h->sc;
   HIP_TRY(h, hipMemcpy(sc.eos, eos, n_mat * sizeof(ihs_eos), hipMemcpyHostToDevice));
   HIP_TRY(h, hipMemcpy(sc.segof, segof.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice));
@@ -779,9 +794,11 @@ int ihs_step_kdk(ihs_handle* h, int32_t substeps) {
   hipSetDevice(h->device);
   const bool graph = h->cfg.use_graph && !h->mlp.record && !h->prof_on;
   const bool fresh_needed = h->cfg.reuse_positions && !h->pos_fresh;
-  if (graph && substeps > 0 && ((h->cfg.h_mode != IHS_H_FIXED && !h->h_settled) || fresh_needed)) {
+  if (graph && substeps > 0 &&
+      ((h->cfg.h_mode != IHS_H_FIXED && !h->h_settled) || fresh_needed || !h->tree_fresh)) {
     // Captured steps assume h was solved (and, reusing positions, the forces evaluated) at the current
-    // positions, as after any step; a step after an upload without a solve or after relaxation runs directly.
+    // positions, and the tree sorted for them, as after any step; a step after an upload without a solve, after
+    // relaxation or after another tree was built in the handle runs directly.
     kdk_step(h);
     --substeps;
   }
@@ -823,6 +840,7 @@ int ihs_relax(ihs_handle* h, int32_t steps) {
     launch_relax(h->stream, h->split, h->n, h->st[h->cur], h->sc.asph, h->sc.accg, h->sc.ctl, seg_view(h));
     h->h_settled = false;
     h->pos_fresh = false;
+    h->tree_fresh = false;
   }
   HIP_TRY(h, hipGetLastError());
   return IHS_OK;
@@ -1007,6 +1025,12 @@ int ihs_gravity(ihs_handle* h, const double* pos3, const double* mass, int64_t n
   if (mode != IHS_GRAVITY_DIRECT && mode != IHS_GRAVITY_TREE) return fail(h, IHS_ERR_ARG, "bad gravity mode");
   hipSetDevice(h->device);
   HIP_TRY(h, hipStreamSynchronize(h->stream));
+  // The bodies take over the working copies, the tree and the accelerations: nothing a force evaluation left for
+  // the particles stands until the next one.
+  h->have_fe = false;
+  h->pot_fresh = false;
+  h->pos_fresh = false;
+  h->tree_fresh = false;
   Scratch& sc = h->sc;
   if (!sc.gpos) {
     HIP_TRY(h, dalloc(h, &sc.gpos, 3 * h->cap));
@@ -1598,6 +1622,7 @@ int ihs_assemble(ihs_handle* h, int32_t segment, const double* placements, int32
   h->have_fe = false;
   h->h_settled = false;
   h->pos_fresh = false;
+  h->tree_fresh = false;
   return IHS_OK;
 }
 
@@ -1803,6 +1828,7 @@ int ihs_checkpoint_load(ihs_handle* h, const void* buf, int64_t bytes) {
   h->zero_soft = hd.zero_soft != 0;
   h->h_settled = hd.h_settled != 0;
   h->pos_fresh = false;
+  h->tree_fresh = false;
   h->have_fe = false;
   h->force_evals = hd.force_evals;
   MlpDev& m = h->mlp;

@@ -29,14 +29,7 @@ template <class PT>
 __global__ __launch_bounds__(256) void k_bbox(const PT* __restrict__ px, const PT* __restrict__ py,
                                               const PT* __restrict__ pz, const uint32_t* __restrict__ start, int bps,
                                               uint32_t* __restrict__ part, int S, EvalInit fe) {
-  if (blockIdx.x == 0 && fe.flags) {
-    if (threadIdx.x == 0) {
-      fe.flags[FLAG_H_RETRY] = 0u;
-      fe.flags[FLAG_TIE_RUNS] = 0u;
-    }
-    for (int q = threadIdx.x; q < S; q += blockDim.x) fe.clk[q].signal = 0xffffffffu;
-    if (fe.hs_npend && threadIdx.x < fe.n_hs_npend) fe.hs_npend[threadIdx.x] = 0u;
-  }
+  if (blockIdx.x == 0) eval_init(fe, S);
   const int sg = blockIdx.x / bps, b = blockIdx.x % bps;
   const int64_t i0 = start[sg], i1 = start[sg + 1];
   uint32_t lo[3] = {0xffffffffu, 0xffffffffu, 0xffffffffu}, hi[3] = {0u, 0u, 0u};

@@ -66,6 +66,29 @@ __global__ __launch_bounds__(256) void k_gather(const uint32_t* __restrict__ per
   hs[t] = fminf(h * margin, fmaxf(sv.hcap[sv.of(t)], h));
 }
 
+// An evaluation at positions the previous evaluation sorted, none of which has moved since: a sort would return the
+// identity, so the state stays in its buffers and only the working copies are written, as k_gather writes them.
+// Block 0 also performs the evaluation's reset (k_bbox's, in a sorting evaluation), and the threads below L - 1
+// re-arm the internal nodes' arrival counters for the upsweep over the kept topology (k_karras's, otherwise).
+template <class PT>
+__global__ __launch_bounds__(256) void k_stage(int64_t n, MasterBufs S, float4* __restrict__ xm,
+                                               float4* __restrict__ vh, float* __restrict__ hs, float margin,
+                                               SegView sv, uint32_t* __restrict__ xlo, EvalInit fe, int64_t L,
+                                               uint32_t* __restrict__ ready, uint32_t* __restrict__ npend) {
+  if (blockIdx.x == 0) eval_init(fe, sv.S);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) *npend = 0u;
+  if (t < L - 1) ready[t] = 0u;
+  if (t >= n) return;
+  const PT x = ((const PT*)S.px)[t], y = ((const PT*)S.py)[t], z = ((const PT*)S.pz)[t];
+  const float h = S.h[t];
+  const float hx = (float)x, hy = (float)y, hz = (float)z;
+  xm[t] = make_float4(hx, hy, hz, S.m[t]);
+  if (xlo) xlo[t] = lo_code((double)x, hx) | (lo_code((double)y, hy) << 10) | (lo_code((double)z, hz) << 20);
+  vh[t] = make_float4(S.vx[t], S.vy[t], S.vz[t], h);
+  hs[t] = fminf(h * margin, fmaxf(sv.hcap[sv.of(t)], h));
+}
+
 // The step of segment sg: fixed, or the Courant dt from the segment's signal of this evaluation.
 __device__ __forceinline__ double step_dt(const StepCtl* ctl, const SegView& sv, uint32_t sg) {
   return ctl->dt_fixed > 0.0 ? ctl->dt_fixed : ctl->cfl * (double)key2f(sv.clk[sg].signal);
@@ -310,6 +333,13 @@ void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, co
                    uint32_t* xlo) {
   if (split) k_gather<double><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, vh, hs, h_margin, sv, xlo);
   else k_gather<float><<<g256(n), 256, 0, s>>>(perm, n, src, dst, xm, vh, hs, h_margin, sv, nullptr);
+}
+
+void launch_stage(hipStream_t s, int split, int64_t n, const MasterBufs& st, float4* xm, float4* vh, float* hs,
+                  float h_margin, const SegView& sv, uint32_t* xlo, const EvalInit& fe, int64_t L, uint32_t* ready,
+                  uint32_t* npend) {
+  if (split) k_stage<double><<<g256(n), 256, 0, s>>>(n, st, xm, vh, hs, h_margin, sv, xlo, fe, L, ready, npend);
+  else k_stage<float><<<g256(n), 256, 0, s>>>(n, st, xm, vh, hs, h_margin, sv, nullptr, fe, L, ready, npend);
 }
 
 void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph,

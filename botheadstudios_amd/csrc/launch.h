@@ -60,6 +60,17 @@ struct EvalInit {
   int n_hs_npend = 0;
 };
 
+// The reset, by the threads of block 0 of the evaluation's first kernel (k_bbox, or k_stage when the sort is kept).
+__device__ __forceinline__ void eval_init(const EvalInit& fe, int nseg) {
+  if (!fe.flags) return;
+  if (threadIdx.x == 0) {
+    fe.flags[FLAG_H_RETRY] = 0u;
+    fe.flags[FLAG_TIE_RUNS] = 0u;
+  }
+  for (int q = threadIdx.x; q < nseg; q += blockDim.x) fe.clk[q].signal = 0xffffffffu;
+  if (fe.hs_npend && (int)threadIdx.x < fe.n_hs_npend) fe.hs_npend[threadIdx.x] = 0u;
+}
+
 struct SegView {
   int S = 1;
   const uint32_t* segof = nullptr;  // n: segment of storage position t (null when S == 1)
@@ -285,6 +296,11 @@ void launch_fe_init(hipStream_t s, uint32_t* flags, const SegView& sv);
 void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,
                    const MasterBufs& dst, float4* xm, float4* vh, float* hs, float h_margin, const SegView& sv,
                    uint32_t* xlo = nullptr);
+// The same working copies in the state's own order (k_stage: positions and order unchanged since the last sort),
+// with the evaluation's reset fe and the re-arming of the upsweep's arrival counters (ready, npend) over L leaves.
+void launch_stage(hipStream_t s, int split, int64_t n, const MasterBufs& st, float4* xm, float4* vh, float* hs,
+                  float h_margin, const SegView& sv, uint32_t* xlo, const EvalInit& fe, int64_t L, uint32_t* ready,
+                  uint32_t* npend);
 void launch_kick_drift(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph,
                        const float4* accg, const StepCtl* ctl, const SegView& sv);
 void launch_kick(hipStream_t s, int split, int64_t n, const MasterBufs& st, const float4* asph, const float4* accg,

@@ -189,7 +189,8 @@ int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset);
 int ihs_segment_energy(ihs_handle* h, int32_t seg, double* out9);
 /* GravityField::accelerations / GpuGravity::accelerations[_tree] (gpu_gravity.rs:181-230,556-578):
  * softened self-gravity on n bodies (pos3 and acc3 are n*3 doubles). mode = IHS_GRAVITY_*;
- * pot (may be NULL) receives the potential. Synchronous. Uses the handle's device and leaf size. */
+ * pot (may be NULL) receives the potential. Synchronous. Uses the handle's device and leaf size, and its tree and
+ * working buffers: the particles need a force evaluation (or a step) before their energies or tree are read. */
 int ihs_gravity(ihs_handle* h, const double* pos3, const double* mass, int64_t n, double softening, double theta,
                 int32_t mode, double* acc3, double* pot);
 /* Inspection for parity tests: the LBVH of the last force evaluation. sorted_codes/order have n

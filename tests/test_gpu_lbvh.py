@@ -4,7 +4,7 @@ the tree passes verify_tree (gpu-bh-verify:420-459), and the walk stays coherent
 import numpy as np
 import pytest
 
-from botheadstudios_amd import ics
+from botheadstudios_amd import ics, mlp
 from botheadstudios_amd.gpu_sph import GpuSph
 from botheadstudios_amd.materials import eos_table
 from oracle import oracle
@@ -134,6 +134,45 @@ def test_tree_bitexact_after_steps():
         assert np.array_equal(got["left"][: L - 1], ref.left[: L - 1])
         assert np.array_equal(got["right"][: L - 1], ref.right[: L - 1])
         assert np.array_equal(got["parent"], ref.parent)
+
+
+@pytest.mark.parametrize("precision,h_mode,leaf", [("split", "learned", 1), ("fp32", "newton", 16), ("fp32", "fixed", 1)])
+def test_kept_sort_same_bits_as_sorting_again(precision, h_mode, leaf):
+    """An evaluation at positions the previous evaluation sorted (the first of each KDK step, or a repeated
+    evaluation) keeps that sort and topology instead of sorting again. The trajectory is the same, bit for bit, as
+    when every evaluation sorts (forced here: a standalone gravity call builds another tree in the handle before
+    each step), and an evaluation repeated at the same state reproduces every field it writes."""
+    ic = ics.config_ic(1, n=8192)
+    bodies = np.random.default_rng(5).normal(size=(64, 3)) * 1.0e6
+
+    def run(resort):
+        with GpuSph(ic.n, leaf_size=leaf, precision=precision, h_mode=h_mode) as sph:
+            if h_mode == "learned":
+                sph.load_mlp(mlp.default_weights_path())
+            sph.upload(ic.particles48(), ic.eos, ic.softening)
+            sph.set_dt(0.0)
+            for _ in range(3):
+                if resort:
+                    sph.gravity(bodies, np.ones(64), 1.0e3)
+                sph.encode_kdk(1)
+            names = ("h", "rho", "rpc", "asph", "agrav", "ncount", "hs", "pos64")
+            sph.force_eval()
+            once = {k: sph.field(k) for k in names}
+            tree = sph.debug_tree()
+            sph.force_eval()
+            twice = {k: sph.field(k) for k in names}
+            return sph.readback(), once, twice, tree
+
+    kept, once, twice, tree = run(False)
+    again, once_r, _, tree_r = run(True)
+    for k in kept.dtype.names:
+        a, b = np.ascontiguousarray(kept[k]), np.ascontiguousarray(again[k])
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), k
+    for k in once:
+        assert np.array_equal(once[k].view(np.uint8), twice[k].view(np.uint8)), k
+        assert np.array_equal(once[k].view(np.uint8), once_r[k].view(np.uint8)), k
+    for k in tree:
+        assert np.array_equal(tree[k], tree_r[k]), k
 
 
 def test_tree_single_leaf_and_tiny():
