@@ -511,7 +511,8 @@ __device__ __forceinline__ void store_node(uint32_t x, const NodeVals& v, float4
 
 // The moment arithmetic is not contracted into FMAs, so a node gets the same bits whichever kernel merges it
 // (batched and single scenarios cut the leaves into blocks at different places). l and r are the left and right
-// children.
+// children. 1/M and the two lengths of the critical radius use the hardware reciprocal and square root (1 ulp; the
+// IEEE sequences were most of a merge's instructions on the climb's critical path).
 __device__ __forceinline__ NodeVals merge_vals(const NodeVals& l, const NodeVals& r, float inv_theta) {
 #pragma clang fp contract(off)
   const float4 cl = l.c, cr = r.c;
@@ -524,7 +525,7 @@ __device__ __forceinline__ NodeVals merge_vals(const NodeVals& l, const NodeVals
   float4 qa = make_float4(0.f, 0.f, 0.f, 0.f);
   float2 qb = make_float2(0.f, 0.f);
   if (m > 0.f) {
-    const float im = 1.0f / m;
+    const float im = __builtin_amdgcn_rcpf(m);
     c.x = (cl.x * cl.w + cr.x * cr.w) * im;
     c.y = (cl.y * cl.w + cr.y * cr.w) * im;
     c.z = (cl.z * cl.w + cr.z * cr.w) * im;
@@ -535,7 +536,8 @@ __device__ __forceinline__ NodeVals merge_vals(const NodeVals& l, const NodeVals
   float4 b1 = make_float4(fmaxf(bl1.x, br1.x), fmaxf(bl1.y, br1.y), fmaxf(bl1.z, br1.z), 0.f);
   const float sx = b1.x - b0.x, sy = b1.y - b0.y, sz = b1.z - b0.z;
   const float ex = 0.5f * (b0.x + b1.x) - c.x, ey = 0.5f * (b0.y + b1.y) - c.y, ez = 0.5f * (b0.z + b1.z) - c.z;
-  b1.w = sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta + sqrtf(ex * ex + ey * ey + ez * ez);
+  b1.w = __builtin_amdgcn_sqrtf(sx * sx + sy * sy + sz * sz) * inv_theta +
+         __builtin_amdgcn_sqrtf(ex * ex + ey * ey + ez * ez);
   return NodeVals{c, b0, b1, qa, qb};
 }
 
