@@ -399,7 +399,8 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   const float eff_margin = seeded ? h->cfg.h_margin_seeded : margin;
   const float diag_scale = newton ? h->cfg.h_margin / eff_margin : 1.0f;
   if (!sorted) launch_karras(s, n, K, h->tb, sv);
-  launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, nullptr, NBR_DIAG_HS * diag_scale);
+  // Kept sort: the node moments in memory are those of these positions (the last full upsweep's); only hs changed.
+  launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, nullptr, NBR_DIAG_HS * diag_scale, sorted);
   launch_groups(s, n, K, h->tb, sv, nullptr, NBR_DIAG_HS * diag_scale, false, true);
   if (h->tb.ggroups != h->tb.groups) launch_groups(s, n, K, h->tb, sv, nullptr, GROUP_DIAG_HS * diag_scale, true);
   prof_mark(h, IHS_STAGE_TREE);
@@ -437,7 +438,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
       // Wider-radius pass, gated on the device: every kernel returns at once unless a solve hit the tight radius.
       const uint32_t* gate = sc.flags + FLAG_H_RETRY;
       launch_retry_hs(s, n, D.h, sc.hs, h->cfg.h_margin, sv, sc.flags);
-      launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, gate, NBR_DIAG_HS);
+      launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, gate, NBR_DIAG_HS, true);  // only hs grew
       launch_groups(s, n, K, h->tb, sv, gate, NBR_DIAG_HS, false, true);
       launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags,
                        sc.counters, gate);

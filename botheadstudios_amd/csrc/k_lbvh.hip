@@ -551,6 +551,24 @@ __device__ __forceinline__ NodeVals merge_pair(bool held_left, const NodeVals& h
   return merge_vals(sel(held_left, held, sib), sel(held_left, sib, held), inv_theta);
 }
 
+// The search-radius-only upsweep (HONLY: moments, boxes and topology unchanged since the last full upsweep, only the
+// search radii hs changed) carries each node's box and largest radius, the values the walk groups are marked from,
+// and stores the radius alone. min and max are exact, so the boxes and radii come out as a full merge forms them.
+__device__ __forceinline__ NodeVals merge_box(const NodeVals& a, const NodeVals& b) {
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  return NodeVals{z,
+                  make_float4(fminf(a.b0.x, b.b0.x), fminf(a.b0.y, b.b0.y), fminf(a.b0.z, b.b0.z), fmaxf(a.b0.w, b.b0.w)),
+                  make_float4(fmaxf(a.b1.x, b.b1.x), fmaxf(a.b1.y, b.b1.y), fmaxf(a.b1.z, b.b1.z), 0.f), z,
+                  make_float2(0.f, 0.f)};
+}
+__device__ __forceinline__ NodeVals load_box(uint32_t x, const float4* nbmin, const float4* nbmax) {
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  return NodeVals{z, nbmin[x], nbmax[x], z, make_float2(0.f, 0.f)};
+}
+__device__ __forceinline__ void store_hs(uint32_t x, float hm, float4* nbmin) {
+  reinterpret_cast<float*>(nbmin + x)[3] = hm;
+}
+
 // A walk group's box bound: diagonal at most diag_hs times the node's largest search radius (none: no bound).
 __device__ __forceinline__ bool group_box_ok(const float4 b0, const float4 b1, float diag_hs) {
   if (!(b0.w > 0.f)) return true;
@@ -617,6 +635,22 @@ __device__ __forceinline__ NodeVals leaf_vals(const P& src, int64_t start, int64
                   qa, qb};
 }
 
+// The box and largest search radius of the leaf holding particles [start, end) (HONLY), as leaf_vals forms them.
+template <class P>
+__device__ __forceinline__ NodeVals leaf_box(const P& src, int64_t start, int64_t end) {
+  float hm = 0.f;
+  float lo[3] = {1.0e30f, 1.0e30f, 1.0e30f}, hi[3] = {-1.0e30f, -1.0e30f, -1.0e30f};
+  for (int64_t s = start; s < end; ++s) {
+    const float4 b = src.x(s);
+    lo[0] = fminf(lo[0], b.x); lo[1] = fminf(lo[1], b.y); lo[2] = fminf(lo[2], b.z);
+    hi[0] = fmaxf(hi[0], b.x); hi[1] = fmaxf(hi[1], b.y); hi[2] = fmaxf(hi[2], b.z);
+    hm = fmaxf(hm, src.h(s));
+  }
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  return NodeVals{z, make_float4(lo[0], lo[1], lo[2], hm), make_float4(hi[0], hi[1], hi[2], 0.f), z,
+                  make_float2(0.f, 0.f)};
+}
+
 // LDS of one phase-1 workgroup: values of the subtree's internal nodes (node x at x - f, f = the root's first leaf;
 // a subtree's internal nodes are numbered inside its leaf range), its leaves (leaf c at c - f; a one-particle leaf's
 // quadrupole is zero and not stored), the internal nodes' children and parents, their arrival counters, and the
@@ -647,8 +681,9 @@ __device__ __forceinline__ bool hc_leaf_ok(int64_t c, int K, int64_t n, int gmax
 // no wait for the global stores sits on the climb), and the second arrival merges its held values with the
 // sibling's from LDS in (left, right) order. Merging is not contracted into FMAs, so every node gets the same bits
 // whichever thread or kernel merges it. The thread that completes the root records the arrival at its parent for
-// phase 2.
-template <bool KONE>
+// phase 2. HONLY: only the search radii changed since the last full upsweep; boxes and radii are carried (merge_box)
+// and the nodes' radii stored, the moments in memory stand.
+template <bool KONE, bool HONLY>
 __global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_blocks(
     const float4* __restrict__ xm, const float* __restrict__ hs, int64_t n, int K, int64_t L,
     const int2* __restrict__ child, const int2* __restrict__ range, const uint32_t* __restrict__ parent,
@@ -696,15 +731,20 @@ __global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_bl
         S.parent[t] = parent[x];
       }
       S.arrive[t] = 0u;
-      cur = leaf_vals(GlobalParticles{xm, hs}, x * K, min((x + 1) * K, n), inv_theta);
       from = (uint32_t)(L - 1 + x);
-      store_node(from, cur, ncom, nbmin, nbmax, nqa, nqb);
+      if (HONLY) {
+        cur = leaf_box(GlobalParticles{xm, hs}, x * K, min((x + 1) * K, n));
+        store_hs(from, cur.b0.w, nbmin);
+      } else {
+        cur = leaf_vals(GlobalParticles{xm, hs}, x * K, min((x + 1) * K, n), inv_theta);
+        store_node(from, cur, ncom, nbmin, nbmax, nqa, nqb);
+      }
       p = parent[from];
       p0 = p;
-      S.lc[t] = cur.c;
+      if (!HONLY) S.lc[t] = cur.c;
       S.lb0[t] = cur.b0;
       S.lb1[t] = cur.b1;
-      if (!KONE) {
+      if (!KONE && !HONLY) {
         S.lqa[t] = cur.qa;
         S.lqb[t] = cur.qb;
       }
@@ -715,11 +755,13 @@ __global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_bl
         const int pi = (int)((int64_t)p - f);
         if (from < Lm1) {
           const int fi = (int)((int64_t)from - f);
-          S.ic[fi] = cur.c;
+          if (!HONLY) {
+            S.ic[fi] = cur.c;
+            S.iqa[fi] = cur.qa;
+            S.iqb[fi] = cur.qb;
+          }
           S.ib0[fi] = cur.b0;
           S.ib1[fi] = cur.b1;
-          S.iqa[fi] = cur.qa;
-          S.iqb[fi] = cur.qb;
           S.icount[fi] = hc;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
@@ -730,19 +772,26 @@ __global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_bl
         const bool left = (uint32_t)ch.x == from;
         const uint32_t sib = left ? (uint32_t)ch.y : (uint32_t)ch.x;
         NodeVals sv;
+        const float4 z4 = make_float4(0.f, 0.f, 0.f, 0.f);
+        const float2 z2 = make_float2(0.f, 0.f);
         if (sib >= Lm1) {
           const int64_t sc = (int64_t)(sib - Lm1);
           const int si = (int)(sc - f);
-          sv = NodeVals{S.lc[si], S.lb0[si], S.lb1[si], KONE ? make_float4(0.f, 0.f, 0.f, 0.f) : S.lqa[si],
-                        KONE ? make_float2(0.f, 0.f) : S.lqb[si]};
+          sv = NodeVals{HONLY ? z4 : S.lc[si], S.lb0[si], S.lb1[si], KONE || HONLY ? z4 : S.lqa[si],
+                        KONE || HONLY ? z2 : S.lqb[si]};
           hc += (uint32_t)(min((sc + 1) * K, n) - sc * K);
         } else {
           const int si = (int)((int64_t)sib - f);
-          sv = NodeVals{S.ic[si], S.ib0[si], S.ib1[si], S.iqa[si], S.iqb[si]};
+          sv = NodeVals{HONLY ? z4 : S.ic[si], S.ib0[si], S.ib1[si], HONLY ? z4 : S.iqa[si], HONLY ? z2 : S.iqb[si]};
           hc += S.icount[si];
         }
-        cur = merge_pair(left, cur, sv, inv_theta);
-        store_node(p, cur, ncom, nbmin, nbmax, nqa, nqb);
+        if (HONLY) {
+          cur = merge_box(cur, sv);
+          store_hs(p, cur.b0.w, nbmin);
+        } else {
+          cur = merge_pair(left, cur, sv, inv_theta);
+          store_node(p, cur, ncom, nbmin, nbmax, nqa, nqb);
+        }
         from = p;
         p = S.parent[pi];
       }
@@ -819,6 +868,7 @@ __device__ __forceinline__ bool top_arrive(uint32_t* keys, uint32_t q, uint32_t*
 
 // Each thread replays recorded arrivals; every node above the blocks is merged once, by its second arrival, found
 // in an LDS table of node ids. As in phase 1 a climb holds its merged node and prefetches the next level's topology.
+template <bool HONLY>
 __global__ __launch_bounds__(1024) void k_upsweep_top(const uint32_t* __restrict__ pend,
                                                       const uint32_t* __restrict__ npend, const int2* __restrict__ child,
                                                       const uint32_t* __restrict__ parent, uint32_t* ready,
@@ -843,12 +893,20 @@ __global__ __launch_bounds__(1024) void k_upsweep_top(const uint32_t* __restrict
         qp2 = parent[qp];
       }
       if (!top_arrive(keys, q, ready)) break;
-      if (from == IHS_NONE)
-        cur = merge_vals(load_node((uint32_t)ch.x, ncom, nbmin, nbmax, nqa, nqb),
-                         load_node((uint32_t)ch.y, ncom, nbmin, nbmax, nqa, nqb), inv_theta);
-      else
-        cur = merge_with_sibling(cur, from, ch, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
-      store_node(q, cur, ncom, nbmin, nbmax, nqa, nqb);
+      if (HONLY) {
+        if (from == IHS_NONE)
+          cur = merge_box(load_box((uint32_t)ch.x, nbmin, nbmax), load_box((uint32_t)ch.y, nbmin, nbmax));
+        else
+          cur = merge_box(cur, load_box((uint32_t)ch.x == from ? (uint32_t)ch.y : (uint32_t)ch.x, nbmin, nbmax));
+        store_hs(q, cur.b0.w, nbmin);
+      } else {
+        if (from == IHS_NONE)
+          cur = merge_vals(load_node((uint32_t)ch.x, ncom, nbmin, nbmax, nqa, nqb),
+                           load_node((uint32_t)ch.y, ncom, nbmin, nbmax, nqa, nqb), inv_theta);
+        else
+          cur = merge_with_sibling(cur, from, ch, ncom, nbmin, nbmax, nqa, nqb, inv_theta);
+        store_node(q, cur, ncom, nbmin, nbmax, nqa, nqb);
+      }
       if (qp == IHS_NONE) break;
       from = q;
       q = qp;
@@ -1099,20 +1157,28 @@ void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView&
   k_group_emit<<<nblk, GRP_BLK, 0, s>>>(tb.gstart, L, K, n, tb.range, tb.gblk, out, gate, sv.segof, sv.root);
 }
 
+template <bool HONLY>
+void upsweep_launch(hipStream_t s, int64_t n, int K, int64_t L, float inv_theta, const float4* xm, const float* hs,
+                    TreeBufs& tb, const uint32_t* gate, float group_diag_hs) {
+  const int gmax = group_diag_hs > 0.f ? GROUP_MAX : 0;
+  if (K == 1)
+    k_upsweep_blocks<true, HONLY><<<grid_for(L, UB_LEAVES_1), UB_LEAVES_1, 0, s>>>(
+        xm, hs, n, K, L, tb.child, tb.range, tb.parent, tb.bstart, tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb,
+        inv_theta, tb.pend, tb.npend, gate, tb.gstart, group_diag_hs, gmax);
+  else
+    k_upsweep_blocks<false, HONLY><<<grid_for(L, UB_LEAVES_K), UB_LEAVES_K, 0, s>>>(
+        xm, hs, n, K, L, tb.child, tb.range, tb.parent, tb.bstart, tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb,
+        inv_theta, tb.pend, tb.npend, gate, tb.gstart, group_diag_hs, gmax);
+  if (L > tb.ub_leaves)
+    k_upsweep_top<HONLY><<<1, 1024, 0, s>>>(tb.pend, tb.npend, tb.child, tb.parent, tb.ready, tb.ncom, tb.nbmin,
+                                            tb.nbmax, tb.nqa, tb.nqb, inv_theta, gate);
+}
+
 void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb,
-                    const uint32_t* gate, float group_diag_hs) {
+                    const uint32_t* gate, float group_diag_hs, bool hs_only) {
   const int64_t L = (n + K - 1) / K;
   const float inv_theta = theta > 0.f ? 1.0f / theta : 3.0e38f;
   if (gate) k_tree_rearm<<<grid_for(L, 256), 256, 0, s>>>(L, tb.ready, tb.npend, gate);
-  if (K == 1)
-    k_upsweep_blocks<true><<<grid_for(L, UB_LEAVES_1), UB_LEAVES_1, 0, s>>>(
-        xm, hs, n, K, L, tb.child, tb.range, tb.parent, tb.bstart, tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb,
-        inv_theta, tb.pend, tb.npend, gate, tb.gstart, group_diag_hs, group_diag_hs > 0.f ? GROUP_MAX : 0);
-  else
-    k_upsweep_blocks<false><<<grid_for(L, UB_LEAVES_K), UB_LEAVES_K, 0, s>>>(
-        xm, hs, n, K, L, tb.child, tb.range, tb.parent, tb.bstart, tb.ncom, tb.nbmin, tb.nbmax, tb.nqa, tb.nqb,
-        inv_theta, tb.pend, tb.npend, gate, tb.gstart, group_diag_hs, group_diag_hs > 0.f ? GROUP_MAX : 0);
-  if (L > tb.ub_leaves)
-    k_upsweep_top<<<1, 1024, 0, s>>>(tb.pend, tb.npend, tb.child, tb.parent, tb.ready, tb.ncom, tb.nbmin, tb.nbmax,
-                                     tb.nqa, tb.nqb, inv_theta, gate);
+  if (hs_only) upsweep_launch<true>(s, n, K, L, inv_theta, xm, hs, tb, gate, group_diag_hs);
+  else upsweep_launch<false>(s, n, K, L, inv_theta, xm, hs, tb, gate, group_diag_hs);
 }

@@ -170,9 +170,10 @@ void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView&
 // Leaf moments and ordered upsweep; xm = sorted (x,y,z,m), hs = sorted search radius (or null). With a gate
 // the arrival counters are re-armed first (a second upsweep over the same topology). group_diag_hs > 0: the upsweep
 // also marks the neighbour-search groups (GROUP_MAX particles, that box bound) in tb.gstart, for launch_groups
-// with marked = true.
+// with marked = true. hs_only: the positions and topology are those of the last full upsweep and only hs changed;
+// the nodes' largest search radii (nbmin.w) and the group marks are formed again, their moments stand.
 void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* xm, const float* hs, TreeBufs& tb,
-                    const uint32_t* gate = nullptr, float group_diag_hs = 0.f);
+                    const uint32_t* gate = nullptr, float group_diag_hs = 0.f, bool hs_only = false);
 
 // --- k_gravity.hip ---
 // Wave-coherent Barnes-Hut group walk over sorted particles; accg = (a/G xyz... see kernel), sorted order.
