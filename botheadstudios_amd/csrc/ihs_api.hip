@@ -430,14 +430,15 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
     const float eta3 = eta * eta * eta;
     if (record) mlp_record_features(s, h->mlp, n, D, sc.eos, sv, eta);
     launch_hsolve(s, n, sc.xm, sc.xlo, sc.hs, D.h, D.hiter, sc.nbr, sc.ncount, n, eta3, h->cfg.h_tol,
-                  h->cfg.h_max_iter, h->hw, sc.flags, sc.counters, sv, nullptr, seeded ? sc.flags + FLAG_H_RETRY : nullptr,
-                  false);  // k_bbox cleared the pending lists at the start of this evaluation
+                  h->cfg.h_max_iter, h->hw, sc.flags, sc.counters, sv, nullptr,
+                  seeded ? sc.flags + FLAG_H_RETRY : nullptr);  // the evaluation's reset cleared the pending lists
     if (record) mlp_record_targets(s, h->mlp, n, D);
     prof_mark(h, IHS_STAGE_HSOLVE);
     if (seeded) {
       // Wider-radius pass, gated on the device: every kernel returns at once unless a solve hit the tight radius.
       const uint32_t* gate = sc.flags + FLAG_H_RETRY;
-      launch_retry_hs(s, n, D.h, sc.hs, h->cfg.h_margin, sv, sc.flags);
+      launch_retry_hs(s, n, D.h, sc.hs, h->cfg.h_margin, sv, sc.flags, (n + K - 1) / K, h->tb.ready, h->tb.npend,
+                      h->hw.npend, PEND_SHARDS);
       launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, gate, NBR_DIAG_HS, true);  // only hs grew
       launch_groups(s, n, K, h->tb, sv, gate, NBR_DIAG_HS, false, true);
       launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags,

@@ -839,15 +839,6 @@ __global__ __launch_bounds__(KONE ? UB_LEAVES_1 : UB_LEAVES_K) void k_upsweep_bl
   }
 }
 
-// Re-arms the arrival counters for a second upsweep over the same topology (gated).
-__global__ __launch_bounds__(256) void k_tree_rearm(int64_t L, uint32_t* __restrict__ ready, uint32_t* __restrict__ npend,
-                                                    const uint32_t* __restrict__ gate) {
-  if (gate && *gate == 0u) return;
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *npend = 0u;
-  if (i < L - 1) ready[i] = 0u;
-}
-
 // Arrival at node q in the top phase's table: true for the second arrival. The first arrival at q inserts q with
 // one compare-and-swap per probe (linear probing from a multiplicative hash); the second finds it. Slots are never
 // freed, so a node whose probe run is full for its first arrival is full for its second too, and both count in
@@ -1178,7 +1169,6 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
                     const uint32_t* gate, float group_diag_hs, bool hs_only) {
   const int64_t L = (n + K - 1) / K;
   const float inv_theta = theta > 0.f ? 1.0f / theta : 3.0e38f;
-  if (gate) k_tree_rearm<<<grid_for(L, 256), 256, 0, s>>>(L, tb.ready, tb.npend, gate);
   if (hs_only) upsweep_launch<true>(s, n, K, L, inv_theta, xm, hs, tb, gate, group_diag_hs);
   else upsweep_launch<false>(s, n, K, L, inv_theta, xm, hs, tb, gate, group_diag_hs);
 }

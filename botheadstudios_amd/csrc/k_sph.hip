@@ -533,11 +533,6 @@ __device__ __forceinline__ void hsolve_account(bool valid, const HsolveState& st
   }
 }
 
-__global__ void k_hsolve_begin(const uint32_t* __restrict__ gate, uint32_t* __restrict__ npend) {
-  if (gate && *gate == 0u) return;
-  if (threadIdx.x < PEND_SHARDS) npend[threadIdx.x] = 0u;
-}
-
 // First pass: one density sum per particle at its seed h. Converged particles are done; the others take one
 // Newton update and join a pending list (wave-aggregated append, in particle order within a wave; block b appends to
 // shard b mod PEND_SHARDS, whose entries start at shard * region) with their bracket, so that the second pass runs
@@ -931,12 +926,11 @@ template <bool REL>
 void hsolve_launch(hipStream_t s, int64_t n, const float4* xm, const uint32_t* xlo, const float* hs, float* h,
                    uint8_t* iters, const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol,
                    int max_iter, const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters,
-                   const SegView& sv, const uint32_t* gate, uint32_t* retry, bool clear_pending) {
+                   const SegView& sv, const uint32_t* gate, uint32_t* retry) {
   static unsigned nblk2 = 0;
   if (!nblk2) nblk2 = persistent_blocks((const void*)k_hsolve2<REL>, 256);
   // Shard r holds the pending particles of blocks r, r + PEND_SHARDS, ...: at most region entries.
   const uint32_t region = (uint32_t)((g256(n) + PEND_SHARDS - 1) / PEND_SHARDS) * 256u;
-  if (clear_pending) k_hsolve_begin<<<1, 64, 0, s>>>(gate, hw.npend);
   k_hsolve1<REL><<<g256(n), 256, 0, s>>>(xm, xlo, hs, h, iters, (const uint4*)nbr, ncount, stride, n, eta3, tol,
                                          max_iter, gate, retry, sv, hw.pend, hw.bracket, hw.npend, region, flags,
                                          counters);
@@ -947,13 +941,13 @@ void hsolve_launch(hipStream_t s, int64_t n, const float4* xm, const uint32_t* x
 void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const uint32_t* xlo, const float* hs, float* h,
                    uint8_t* iters, const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol,
                    int max_iter, const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters,
-                   const SegView& sv, const uint32_t* gate, uint32_t* retry, bool clear_pending) {
+                   const SegView& sv, const uint32_t* gate, uint32_t* retry) {
   if (xlo)
     hsolve_launch<true>(s, n, xm, xlo, hs, h, iters, nbr, ncount, stride, eta3, tol, max_iter, hw, flags, counters, sv,
-                        gate, retry, clear_pending);
+                        gate, retry);
   else
     hsolve_launch<false>(s, n, xm, xlo, hs, h, iters, nbr, ncount, stride, eta3, tol, max_iter, hw, flags, counters,
-                         sv, gate, retry, clear_pending);
+                         sv, gate, retry);
 }
 
 void launch_density(hipStream_t s, int split, int feat, int64_t n, const float4* xm, const uint32_t* xlo, const float* h,

@@ -22,11 +22,20 @@ __global__ void k_fe_init(uint32_t* flags, SegView sv) {
 }
 
 // Wider-radius pass of a seeded evaluation (runs only when the tight pass stopped some solve at its radius).
+// It also re-arms the upsweep's arrival counters over the kept topology (ready for the L - 1 internal nodes, the
+// pending count) and clears the h-solve's pending lists, for the second upsweep and solve.
 __global__ __launch_bounds__(256) void k_retry_hs(int64_t n, const float* __restrict__ h, float* __restrict__ hs,
-                                                  float margin, SegView sv, uint32_t* __restrict__ flags) {
+                                                  float margin, SegView sv, uint32_t* __restrict__ flags, int64_t L,
+                                                  uint32_t* __restrict__ ready, uint32_t* __restrict__ npend,
+                                                  uint32_t* __restrict__ hs_npend, int n_hs_npend) {
   if (flags[FLAG_H_RETRY] == 0u) return;
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t == 0) atomicAdd(&flags[FLAG_RETRIES], 1u);
+  if (t == 0) {
+    atomicAdd(&flags[FLAG_RETRIES], 1u);
+    *npend = 0u;
+  }
+  if (t < n_hs_npend) hs_npend[t] = 0u;
+  if (t < L - 1) ready[t] = 0u;
   if (t >= n) return;
   const float hh = h[t];
   hs[t] = fminf(hh * margin, fmaxf(sv.hcap[sv.of(t)], hh));
@@ -324,8 +333,10 @@ __global__ __launch_bounds__(256) void k_prov_max(int64_t t0, int64_t t1, const 
 void launch_fe_init(hipStream_t s, uint32_t* flags, const SegView& sv) { k_fe_init<<<1, 256, 0, s>>>(flags, sv); }
 
 void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float margin, const SegView& sv,
-                     uint32_t* flags) {
-  k_retry_hs<<<std::max(1u, g256(n)), 256, 0, s>>>(n, h, hs, margin, sv, flags);
+                     uint32_t* flags, int64_t L, uint32_t* ready, uint32_t* npend, uint32_t* hs_npend,
+                     int n_hs_npend) {
+  k_retry_hs<<<std::max(1u, g256(n)), 256, 0, s>>>(n, h, hs, margin, sv, flags, L, ready, npend, hs_npend,
+                                                   n_hs_npend);
 }
 
 void launch_gather(hipStream_t s, int split, int64_t n, const uint32_t* perm, const MasterBufs& src,

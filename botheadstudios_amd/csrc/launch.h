@@ -167,8 +167,9 @@ void launch_karras(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView&
 // neighbour-search groups (GROUP_MAX, this diag_hs) in tb.gstart.
 void launch_groups(hipStream_t s, int64_t n, int K, TreeBufs& tb, const SegView& sv, const uint32_t* gate = nullptr,
                    float diag_hs = GROUP_DIAG_HS, bool gravity = false, bool marked = false);
-// Leaf moments and ordered upsweep; xm = sorted (x,y,z,m), hs = sorted search radius (or null). With a gate
-// the arrival counters are re-armed first (a second upsweep over the same topology). group_diag_hs > 0: the upsweep
+// Leaf moments and ordered upsweep; xm = sorted (x,y,z,m), hs = sorted search radius (or null). gate (optional): every
+// kernel returns at once while *gate is zero; a gated second upsweep over the same topology needs its arrival
+// counters re-armed first (launch_retry_hs does). group_diag_hs > 0: the upsweep
 // also marks the neighbour-search groups (GROUP_MAX particles, that box bound) in tb.gstart, for launch_groups
 // with marked = true. hs_only: the positions and topology are those of the last full upsweep and only hs changed;
 // the nodes' largest search radii (nbmin.w) and the group marks are formed again, their moments stand.
@@ -222,14 +223,19 @@ struct HsolveScratch {
 // The SPH passes below take pair displacements from xm alone (xlo == null) or, in split precision, from the hi/lo
 // positions xm + xlo (pair_d in common.h).
 // retry != null: solves stopped by a search radius below hcap increment *retry instead of FLAG_H_CAPPED.
+// The pending-list lengths hw.npend start at zero: the evaluation's reset (eval_init) clears them, and before the
+// gated second solve launch_retry_hs does.
 void launch_hsolve(hipStream_t s, int64_t n, const float4* xm, const uint32_t* xlo, const float* hs, float* h,
                    uint8_t* iters,
                    const uint32_t* nbr, const uint32_t* ncount, int64_t stride, float eta3, float tol, int max_iter,
                    const HsolveScratch& hw, uint32_t* flags, unsigned long long* counters, const SegView& sv,
-                   const uint32_t* gate = nullptr, uint32_t* retry = nullptr, bool clear_pending = true);
-// Wider-radius pass set-up (gated): hs = min(h * margin, max(hcap, h)); counts the pass in FLAG_RETRIES.
+                   const uint32_t* gate = nullptr, uint32_t* retry = nullptr);
+// Wider-radius pass set-up (gated): hs = min(h * margin, max(hcap, h)); counts the pass in FLAG_RETRIES; re-arms
+// the upsweep's arrival counters (ready over the L - 1 internal nodes, npend) and clears the h-solve's n_hs_npend
+// pending-list lengths, for the second upsweep and solve over the same topology.
 void launch_retry_hs(hipStream_t s, int64_t n, const float* h, float* hs, float margin, const SegView& sv,
-                     uint32_t* flags);
+                     uint32_t* flags, int64_t L, uint32_t* ready, uint32_t* npend, uint32_t* hs_npend,
+                     int n_hs_npend);
 // Also reduces each segment's Courant signal into sv.clk[s].signal. vh enters as (v, h_prev) and leaves as (v, h).
 // mask (optional, lists of at most NMASK_WORDS * 32 entries): receives each list's support mask, bit k of word
 // k / 32 at (k / 32) * stride + t set when entry k lies inside the kernel support r < h_ij.

@@ -182,3 +182,32 @@ def test_network_inputs_match_brute_force(fused):
     assert np.all(np.abs(feat[:, 2] - np.sqrt((grad ** 2).sum(1))) <= 2e-5 * gscale + 1e-30)
     assert np.all(feat[:, 3] == 0.0)
     assert np.abs(divv).mean() > 1e-3 * vscale.mean() and np.sqrt((grad ** 2).sum(1)).mean() > 0.0
+
+
+def test_tight_seeded_radius_takes_the_wider_pass():
+    """A seeded evaluation searches h_margin_seeded x seed; a solve that reaches that radius raises the gate of the
+    wider-radius pass (h_margin x h for every particle: the counters re-armed, a search-radius-only upsweep, new
+    groups and lists, a second solve). With the tight radius equal to the seed itself, solves whose root lies above
+    their seed hit it, so the pass runs on every step: the lists then come from the wider radius (far more entries
+    than the default margin leaves), and the solved h is the default run's to the solve's tolerance, each h meeting
+    |f|/rho <= 1e-4 in the oracle's FP64 residual."""
+    ic = ics.config_ic(1, n=16384)
+    out = {}
+    for margin in (1.03, 1.0):
+        with GpuSph(ic.n, leaf_size=1, precision="split", h_mode="learned", h_margin_seeded=margin) as sph:
+            sph.load_mlp(mlp.default_weights_path())
+            sph.upload(ic.particles48(), ic.eos, ic.softening)
+            sph.set_dt(2.0)
+            sph.stats(reset=True)
+            sph.encode_kdk(4)
+            out[margin] = (sph.stats(), sph.field("h").astype(np.float64), sph.field("ncount").astype(np.float64),
+                           sph.field("pos64"))
+    (st_d, h_d, nc_d, p_d), (st_t, h_t, nc_t, p_t) = out[1.03], out[1.0]
+    for st in (st_d, st_t):
+        assert st["h_unconverged"] == 0 and st["h_capped"] == 0 and st["nonfinite"] == 0, st
+        assert st["nbr_overflow"] == 0 and st["walk_overflow"] == 0, st
+    assert nc_t.mean() > 1.4 * nc_d.mean(), (nc_t.mean(), nc_d.mean())
+    rel = np.abs(h_t - h_d) / h_d
+    assert np.median(rel) < 1e-4 and np.max(rel) < 2e-3, (np.median(rel), np.max(rel))
+    resid, _ = oracle.hresidual(p_t, ic.m, h_t, ics.ETA_48)
+    assert np.max(resid) <= 1.2e-4
