@@ -57,6 +57,12 @@ def parse():
     ap.add_argument("--theta", type=float, default=None, help="default: the autotuned value for this N")
     ap.add_argument("--group", type=int, default=None, help="default: the autotuned value for this N")
     ap.add_argument("--quadrupole", type=int, default=None, help="default: the autotuned value for this N")
+    ap.add_argument("--theta-mono", type=float, default=None,
+                    help="cells passing the opening test at this angle act as monopoles (0: off); default: the "
+                         "autotuned value for this N")
+    ap.add_argument("--quad-tol", type=float, default=None,
+                    help="cells whose quadrupole term is below this fraction of their monopole act as monopoles "
+                         "(0: off); default: the autotuned value for this N")
     ap.add_argument("--profile-steps", type=int, default=3)
     ap.add_argument("--variants", default="plain,learned,reuse,continuity,literal,contact",
                     help="plain: Newton FP32; learned: learned seed + split precision; reuse: both again with the "
@@ -110,6 +116,8 @@ def measured_traffic(n, args, variant):
     rec = json.loads(line[len("traffic: "):])
     same = (rec.get("n") == n and rec.get("leaf") == args.leaf and rec.get("group") == args.group and
             abs(rec.get("theta", -1) - args.theta) < 1e-6 and rec.get("quadrupole") == bool(args.quadrupole) and
+            abs(rec.get("theta_mono", 0.0) - args.theta_mono) < 1e-6 and
+            abs(rec.get("quad_tol", 0.0) - args.quad_tol) < 1e-9 and
             rec.get("variant") == variant and rec.get("fused_walk", False) == FUSED_WALK)
     return rec["bytes_per_launch"] if same else None
 
@@ -148,10 +156,12 @@ def run_variant(ic, precision, h_mode, args, weights=None, reuse=False, steps=No
     timed KDK steps (hipGraph replays), then a profiled pass with per-stage HIP events and the force error."""
     from botheadstudios_amd.gpu_sph import GpuSph
 
-    w = {**dict(leaf=args.leaf, theta=args.theta, group=args.group, quadrupole=args.quadrupole), **(walk or {})}
+    w = {**dict(leaf=args.leaf, theta=args.theta, group=args.group, quadrupole=args.quadrupole,
+                theta_mono=args.theta_mono, quad_tol=args.quad_tol), **(walk or {})}
     steps = args.steps if steps is None else steps
     sph = GpuSph(ic.n, leaf_size=w["leaf"], precision=precision, h_mode=h_mode, theta=w["theta"],
-                 group_max=w["group"], quadrupole=w["quadrupole"], reuse_positions=reuse)
+                 group_max=w["group"], quadrupole=w["quadrupole"], reuse_positions=reuse,
+                 theta_mono=w.get("theta_mono", 0.0), quad_tol=w.get("quad_tol", 0.0))
     try:
         if weights is not None:
             sph.load_mlp(weights)
@@ -425,7 +435,10 @@ def main():
     args.theta = tuned["theta"] if args.theta is None else args.theta
     args.group = tuned["group"] if args.group is None else args.group
     args.quadrupole = tuned["quadrupole"] if args.quadrupole is None else bool(args.quadrupole)
-    log(f"walk parameters: leaf {args.leaf}, group {args.group}, theta {args.theta}, quadrupole {args.quadrupole}")
+    args.theta_mono = tuned["theta_mono"] if args.theta_mono is None else args.theta_mono
+    args.quad_tol = tuned["quad_tol"] if args.quad_tol is None else args.quad_tol
+    log(f"walk parameters: leaf {args.leaf}, group {args.group}, theta {args.theta}, quadrupole {args.quadrupole}, "
+        f"theta_mono {args.theta_mono}, quad_tol {args.quad_tol}")
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
     variants = {}
@@ -459,7 +472,7 @@ def main():
     if "continuity" in wanted and weights is not None:
         add("continuity_split", "split", "learned", args, weights=weights, zero_seed=True, steps=vsteps)
     if "literal" in wanted:
-        lit = dict(leaf=16, theta=0.5, group=64, quadrupole=False)
+        lit = dict(leaf=16, theta=0.5, group=64, quadrupole=False, theta_mono=0.0, quad_tol=0.0)
         add("plain_fp32_literal", "fp32", "newton", args, walk=lit, steps=vsteps)
         if weights is not None:
             add("learned_split_literal", "split", "learned", args, weights=weights, walk=lit, steps=vsteps)
@@ -488,7 +501,8 @@ def main():
         "data": "synthetic (seeded jittered-lattice two-body impact)",
         "config": {"workload": f"two-body giant impact, N={ic.n}, 0.9+0.1 Earth masses, 45 deg, 1.1 v_esc; "
                                f"KDK step = 2 force evaluations; variant={head}",
-                   "n": ic.n, "leaf": args.leaf, "group": args.group, "theta": args.theta, "quadrupole": args.quadrupole, "n_ngb": 48, "h_tol": 1e-4, "variant": head,
+                   "n": ic.n, "leaf": args.leaf, "group": args.group, "theta": args.theta, "quadrupole": args.quadrupole,
+                   "theta_mono": args.theta_mono, "quad_tol": args.quad_tol, "n_ngb": 48, "h_tol": 1e-4, "variant": head,
                    "fused_walk": FUSED_WALK},
         "roofline": roofline(res, ic.n, args, head, walk_work, equal_work),
         "passes": passes(res, ic.n, walk_work["flops"]),

@@ -4,7 +4,10 @@ The step's cost is dominated by the Barnes-Hut walk. Its cost and its force erro
 - the leaf bucket K (exact particle-particle sums inside opened leaves vs monopoles of small subtrees);
 - the group size (how many particles share one walk);
 - theta;
-- the multipole order of accepted cells (monopole, as the reference, or monopole + quadrupole).
+- the multipole order of accepted cells (monopole, as the reference, or monopole + quadrupole);
+- with quadrupoles, which accepted cells may act as monopoles: those that also pass the opening test at a smaller
+  angle theta_mono, or whose quadrupole term is below quad_tol times their monopole term (`tune_monopole`, a second
+  stage at the chosen leaf, group and theta).
 
 The reference fixes theta = 0.5 with K = 1 (ie/crates/engine/src/gpu_gravity.rs:55,61) at an RMS force error of
 0.70% (gpu-bh-verify:552-569). Here the force error is a bound, RMS relative error <= 1e-3 against the FP64
@@ -19,6 +22,7 @@ Results are stored per particle-count bucket in botheadstudios_amd/data/tuned.js
 read through `tuned_params(n)`.
 
     python -m botheadstudios_amd.autotune --cfg 4          # tune for the 1M-particle impact
+    python -m botheadstudios_amd.autotune --cfg 4 --mono   # only the monopole-cell stage, at the tuned walk
     python -m botheadstudios_amd.autotune --knee           # the standalone gravity's direct-vs-tree knee
 """
 from __future__ import annotations
@@ -30,18 +34,21 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(HERE, "data", "tuned.json")
-DEFAULT = {"leaf": 16, "group": 64, "theta": 0.5, "quadrupole": True}
+DEFAULT = {"leaf": 16, "group": 64, "theta": 0.5, "quadrupole": True, "theta_mono": 0.0, "quad_tol": 0.0}
 QUAD_THETAS = (1.2, 1.1, 1.0, 0.95, 0.9, 0.875, 0.85, 0.825, 0.8, 0.75, 0.7, 0.6, 0.5, 0.45, 0.4)
 MONO_THETAS = (0.5, 0.45, 0.42, 0.4, 0.375, 0.35)
 
 
+MONO_GRID = ((0.0, 0.0), (0.0, 0.03), (0.5, 0.03), (0.55, 0.03), (0.6, 0.03), (0.5, 0.05), (0.6, 0.0))
+
+
 def measure(ic, leaf: int, theta: float, group: int = 64, quadrupole: bool = True, steps: int = 3, warmup: int = 1,
-            n_targets: int = 8192, device: int = 0) -> dict:
+            n_targets: int = 8192, device: int = 0, theta_mono: float = 0.0, quad_tol: float = 0.0) -> dict:
     """ms/step (plain Newton, FP32) and the walk's force error for one setting on an impact IC."""
     from .gpu_sph import GpuSph
 
     with GpuSph(ic.n, device=device, leaf_size=leaf, theta=theta, group_max=group, quadrupole=quadrupole,
-                h_mode="newton", use_graph=True) as sph:
+                h_mode="newton", use_graph=True, theta_mono=theta_mono, quad_tol=quad_tol) as sph:
         sph.upload(ic.particles48(), ic.eos, ic.softening)
         sph.set_dt(0.0)
         sph.encode_kdk(warmup)
@@ -52,8 +59,8 @@ def measure(ic, leaf: int, theta: float, group: int = 64, quadrupole: bool = Tru
         ms = 1e3 * (time.perf_counter() - t0) / steps
         st = sph.stats()
         err = sph.gravity_error(n_targets)
-    return {"leaf": leaf, "group": group, "theta": theta, "quadrupole": bool(quadrupole), "ms_per_step": ms,
-            "err_rms": err["rms"],
+    return {"leaf": leaf, "group": group, "theta": theta, "quadrupole": bool(quadrupole), "theta_mono": theta_mono,
+            "quad_tol": quad_tol, "ms_per_step": ms, "err_rms": err["rms"],
             "err_max": err["max"], "ok": st["nbr_overflow"] == 0 and st["walk_overflow"] == 0 and st["nonfinite"] == 0}
 
 
@@ -81,6 +88,27 @@ def tune(ic, err_target: float = 1e-3, margin: float = 0.92, leaves=(1, 2, 4, 8,
     return {"n": ic.n, "err_target": err_target, "margin": margin, "best": best, "table": table}
 
 
+def tune_monopole(ic, base: dict, err_target: float = 1e-3, margin: float = 0.92, grid=MONO_GRID, steps: int = 3,
+                  reps: int = 2, log=print) -> dict:
+    """Second stage at a quadrupole walk's (leaf, group, theta): the fastest (theta_mono, quad_tol) whose error stays
+    within margin * err_target. Each setting is timed reps times, interleaved, and its best time kept."""
+    table = {}
+    for _ in range(reps):
+        for tm, qt in grid:
+            if tm > base["theta"]:
+                continue
+            m = measure(ic, base["leaf"], base["theta"], base["group"], True, steps=steps, theta_mono=tm, quad_tol=qt)
+            k = (tm, qt)
+            if k not in table or m["ms_per_step"] < table[k]["ms_per_step"]:
+                table[k] = m
+            log(f"  theta {base['theta']:.3f} theta_mono {tm:.2f} quad_tol {qt:.3f}: {m['ms_per_step']:.3f} ms/step, "
+                f"err rms {m['err_rms']:.2e} max {m['err_max']:.2e}")
+    rows = list(table.values())
+    feasible = [m for m in rows if m["ok"] and m["err_rms"] <= margin * err_target]
+    best = min(feasible, key=lambda m: m["ms_per_step"]) if feasible else None
+    return {"best": best, "table": rows}
+
+
 def bucket(n: int) -> str:
     """Particle-count bucket key (powers of two)."""
     return str(1 << max(0, int(round(math.log2(max(n, 1))))))
@@ -98,7 +126,8 @@ def save(result: dict, path: str = TUNED):
 
 
 def tuned_params(n: int, path: str = TUNED) -> dict:
-    """The tuned {leaf, group, theta} for n particles (nearest tuned bucket), or the defaults."""
+    """The tuned {leaf, group, theta, quadrupole, theta_mono, quad_tol} for n particles (nearest tuned bucket), or
+    the defaults."""
     if not os.path.exists(path):
         return dict(DEFAULT)
     with open(path) as f:
@@ -108,7 +137,17 @@ def tuned_params(n: int, path: str = TUNED) -> dict:
         return dict(DEFAULT)
     best = min(cands, key=lambda c: c[0])[1]["best"]
     return {"leaf": int(best["leaf"]), "group": int(best["group"]), "theta": float(best["theta"]),
-            "quadrupole": bool(best.get("quadrupole", False))}
+            "quadrupole": bool(best.get("quadrupole", False)), "theta_mono": float(best.get("theta_mono", 0.0)),
+            "quad_tol": float(best.get("quad_tol", 0.0))}
+
+
+def walk_kwargs(params: dict, leaf: bool = True) -> dict:
+    """GpuSph keyword arguments of a tuned walk setting (leaf_size left out with leaf=False)."""
+    kw = dict(group_max=params["group"], theta=params["theta"], quadrupole=params["quadrupole"],
+              theta_mono=params.get("theta_mono", 0.0), quad_tol=params.get("quad_tol", 0.0))
+    if leaf:
+        kw["leaf_size"] = params["leaf"]
+    return kw
 
 
 KNEE_NS = (1500, 3000, 6000, 12000, 24000, 48000, 96000, 192000, 384000)
@@ -189,6 +228,7 @@ def main(argv=None):
     ap.add_argument("--err", type=float, default=1e-3)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--out", default=TUNED)
+    ap.add_argument("--mono", action="store_true", help="only the monopole-cell stage, at the stored tuned walk")
     a = ap.parse_args(argv)
     if a.knee:
         res = gravity_knee()
@@ -197,7 +237,16 @@ def main(argv=None):
         return
     ic = ics.config_ic(a.cfg, n=a.n)
     t0 = time.time()
-    res = tune(ic, err_target=a.err, steps=a.steps)
+    if a.mono:
+        with open(a.out) as f:
+            res = json.load(f)[bucket(ic.n)]
+    else:
+        res = tune(ic, err_target=a.err, steps=a.steps)
+    if res["best"] and res["best"].get("quadrupole"):
+        mono = tune_monopole(ic, res["best"], err_target=a.err, margin=res.get("margin", 0.92), steps=a.steps)
+        res["monopole_table"] = mono["table"]
+        if mono["best"]:
+            res["best"] = mono["best"]
     save(res, a.out)
     print(json.dumps({"best": res["best"], "n": ic.n, "seconds": round(time.time() - t0, 1)}), flush=True)
 

@@ -16,11 +16,11 @@ import numpy as np
 
 
 def run(ic, steps: int, every: int, precision: str, h_mode: str = "newton", params=None, weights=None) -> dict:
+    from . import autotune
     from .gpu_sph import GpuSph
 
     params = params or {"leaf": 16, "group": 64, "theta": 0.5, "quadrupole": True}
-    with GpuSph(ic.n, leaf_size=params["leaf"], theta=params["theta"], group_max=params["group"],
-                quadrupole=params["quadrupole"], precision=precision, h_mode=h_mode) as sph:
+    with GpuSph(ic.n, precision=precision, h_mode=h_mode, **autotune.walk_kwargs(params)) as sph:
         if weights is not None and h_mode == "learned":
             sph.load_mlp(weights)
         sph.upload(ic.particles48(), ic.eos, ic.softening)

@@ -237,6 +237,9 @@ int alloc_all(ihs_handle* h) {
   HIP_TRY(h, dalloc(h, &t.nqa, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.nqb, 2 * Lc));
   t.quadrupole = h->cfg.quadrupole;
+  t.theta_mono = h->cfg.theta_mono;
+  t.quad_tol = h->cfg.quad_tol;
+  if (t.quadrupole && (t.theta_mono > 0.f || t.quad_tol > 0.f)) HIP_TRY(h, dalloc(h, &t.nfar, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.gstart, Lc));
   HIP_TRY(h, dalloc(h, &t.gblk, Lc / 1024 + 2));
   HIP_TRY(h, dalloc(h, &t.groups, C + 1));
@@ -572,6 +575,8 @@ void ihs_default_config(ihs_config* c) {
   c->quadrupole = 1;
   c->reuse_positions = 0;
   c->fused_walk = 1;
+  c->theta_mono = 0.0f;
+  c->quad_tol = 0.0f;
 }
 
 int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
@@ -584,7 +589,8 @@ int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
       c.precision < 0 || c.precision > 1 || !(c.theta >= 0.f) || !(c.h_margin >= 1.0f) ||
       !(c.h_margin_seeded >= 1.0f) || c.group_max < c.leaf_size || c.group_max > 2 * GROUP_MAX ||
       c.quadrupole < 0 || c.quadrupole > 1 || c.reuse_positions < 0 || c.reuse_positions > 1 || c.fused_walk < 0 ||
-      c.fused_walk > 1)
+      c.fused_walk > 1 || !(c.theta_mono >= 0.f) || (c.theta_mono > 0.f && !(c.theta_mono <= c.theta)) ||
+      !(c.quad_tol >= 0.f))
     return IHS_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IHS_ERR_HIP;

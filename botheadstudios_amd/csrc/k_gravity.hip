@@ -8,7 +8,10 @@
 // (ie/shaders/bh_gravity.wgsl:375-386) evaluated at the group's nearest point, so every particle of the
 // group satisfies the per-particle test and the walk is never less accurate than the reference's
 // per-particle walk (bh_gravity.wgsl:337-394). Leaves are never accepted: their particles are summed
-// exactly (bh_gravity.wgsl:356-369). Accepted nodes and opened leaves are staged in LDS and evaluated by
+// exactly (bh_gravity.wgsl:356-369). With quadrupoles, an accepted cell that also passes the test at a smaller
+// angle theta_mono acts as a monopole (13 instead of 31 VALU instructions per interaction): its quadrupole term,
+// of order theta_mono^2, is below the truncation error of the cells accepted at theta, of order theta^3.
+// Accepted nodes and opened leaves are staged in LDS and evaluated by
 // all 64 lanes against broadcast sources, so there is no divergence in the force loop.
 #include "common.h"
 #include "launch.h"
@@ -25,7 +28,7 @@ struct WalkLds {
   float4 cell[GW_RING];
   float4 qa[QUAD ? GW_RING : 1];  // accepted cells' quadrupoles (xx, yy, zz, xy), (xz, yz)
   float2 qb[QUAD ? GW_RING : 1];
-  float4 mono[QUAD ? GW_RING : 1];  // accepted one-particle leaves: monopoles, evaluated without the quadrupole
+  float4 mono[QUAD ? GW_RING : 1];  // accepted one-particle leaves and far cells: monopoles, without the quadrupole
   uint32_t leaf[GW_RING];
   float4 part[64];
   uint32_t pidx[64];
@@ -141,7 +144,7 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
                                            const float4* __restrict__ nqa, const float2* __restrict__ nqb,
                                            float soft2, float4* __restrict__ accg, uint32_t* __restrict__ flags,
                                            uint64_t& n_cell, uint64_t& n_pp, uint64_t& n_nodes, const NbrIo& io,
-                                           uint64_t& ncand, uint64_t& npairs) {
+                                           uint64_t& ncand, uint64_t& npairs, const float* __restrict__ nfar) {
   static_assert(!NBR || T == 1, "neighbour lists are written by walks with one target per lane");
   const int64_t t0 = grp.x;
   int64_t t[T];
@@ -322,11 +325,12 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
     // Every node, leaves included, is tested with the MAC (the reference always sums leaves exactly,
     // bh_gravity.wgsl:356-369; with buckets of K particles a far leaf is the same monopole test as any node).
     const bool live = node != IHS_NONE;
-    bool acc = false;
+    bool acc = false, far = false;
     float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
     if (live) {
       c = ncom[node];
       const float rc = nbmax[node].w;
+      const float dfar = nfar ? nfar[node] : 3.0e38f;
       const float dx = fmaxf(fmaxf(bx0 - c.x, c.x - bx1), 0.f);
       const float dy = fmaxf(fmaxf(by0 - c.y, c.y - by1), 0.f);
       const float dz = fmaxf(fmaxf(bz0 - c.z, c.z - bz1), 0.f);
@@ -334,6 +338,7 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
       // Without softening a leaf touching the group box may hold a target itself: it takes the
       // particle-particle path, which skips the self pair.
       acc = d2 >= rc * rc && (!SELF || node < leaf0 || d2 > 0.f);
+      far = d2 >= dfar;
       if (NBR && acc) {
         // The neighbour build's box test (group box against the node box grown by the larger search radius): a
         // node that passes is opened, and its leaves come to the leaf path.
@@ -347,7 +352,9 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
     }
     const bool isleaf = live && !acc && node >= leaf0;
     const bool open = live && !acc && node < leaf0;
-    const bool accm = acc && mono_leaves && node >= leaf0;
+    // Monopole ring: one-particle leaves (zero quadrupole) and cells far enough that their quadrupole term is below
+    // the error of the cells accepted at the edge of the test.
+    const bool accm = acc && QUAD && ((mono_leaves && node >= leaf0) || far);
     acc = acc && !accm;
     const uint64_t ma = __ballot(acc), mo = __ballot(open), ml = __ballot(isleaf), mm = __ballot(accm);
     if (accm) S.mono[(mtail + lanes_below(mm)) & (GW_RING - 1)] = c;
@@ -446,7 +453,8 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
                                                                  const uint32_t* __restrict__ ngroups,
                                                                  uint32_t* __restrict__ wq, SegView sv,
                                                                  float4* __restrict__ accg, uint32_t* __restrict__ flags,
-                                                                 unsigned long long* __restrict__ counters, NbrIo io) {
+                                                                 unsigned long long* __restrict__ counters, NbrIo io,
+                                                                 const float* __restrict__ nfar) {
   __shared__ WalkLds<QUAD> lds[GW_WAVES];
   const int w = threadIdx.x >> 6;
   const uint32_t lane = threadIdx.x & 63;
@@ -465,7 +473,7 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
       const int4 grp = groups[g];
       const float soft2 = sv.soft2[sv.of(grp.x)];
       walk_group<KAHAN, SELF, T, POT, QUAD, NBR>(S, grp, lane, xm, n, K, L, child, ncom, nbmax, nqa, nqb, soft2,
-                                                 accg, flags, n_cell, n_pp, n_nodes, io, ncand, npairs);
+                                                 accg, flags, n_cell, n_pp, n_nodes, io, ncand, npairs, nfar);
     }
   }
   if (lane == 0) {
@@ -618,12 +626,49 @@ void launch_acc_out(hipStream_t s, const uint32_t* perm, int64_t n, const float4
   k_acc_out<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(perm, n, acc4, acc3, pot);
 }
 
+// Smallest squared distance (from a walk group's box) at which an accepted cell acts as a monopole: the opening test
+// at theta_mono, d >= rc theta / theta_mono (far2 = (theta / theta_mono)^2; rc = b / theta + delta, so this is at
+// least b / theta_mono + delta), or a quadrupole term below tol times the monopole term. With d = com - x and
+// r >= |d|, |a_quad| = M |-q d / r^5 + 5/2 (d.q.d) d / r^7| <= 3.5 |q|_F M / r^4, a fraction 3.5 |q|_F / r^2 of the
+// monopole's M / r^2: d^2 >= 3.5 |q|_F / tol. One-particle leaves (K = 1) store no quadrupole and are monopoles.
+__global__ __launch_bounds__(256) void k_far_radius(int64_t nodes, int64_t leaf0, int K, const float4* __restrict__ nbmax,
+                                                    const float4* __restrict__ nqa, const float2* __restrict__ nqb,
+                                                    float far2, float qscale, float* __restrict__ nfar) {
+  const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (x >= nodes) return;
+  if (K == 1 && x >= leaf0) {
+    nfar[x] = 0.f;
+    return;
+  }
+  const float rc = nbmax[x].w;
+  float d2 = far2 > 0.f ? rc * rc * far2 : 3.0e38f;
+  if (qscale > 0.f) {
+    const float4 qa = nqa[x];
+    const float2 qb = nqb[x];
+    const float qf = __builtin_sqrtf(qa.x * qa.x + qa.y * qa.y + qa.z * qa.z +
+                                     2.f * (qa.w * qa.w + qb.x * qb.x + qb.y * qb.y));
+    d2 = fminf(d2, qf * qscale);
+  }
+  nfar[x] = d2;
+}
+
+void launch_far_radius(hipStream_t s, int64_t n, int K, float theta, const TreeBufs& tb) {
+  if (!tb.nfar || !tb.quadrupole) return;
+  const int64_t L = (n + K - 1) / K;
+  const float r = tb.theta_mono > 0.f && theta > 0.f ? theta / tb.theta_mono : 0.f;
+  const float qscale = tb.quad_tol > 0.f ? 3.5f / tb.quad_tol : 0.f;
+  const int64_t nodes = 2 * L - 1;
+  k_far_radius<<<(unsigned)((nodes + 255) / 256), 256, 0, s>>>(nodes, L - 1, K, tb.nbmax, tb.nqa, tb.nqb, r * r,
+                                                                qscale, tb.nfar);
+}
+
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
                          const int4* groups, const uint32_t* ngroups, const SegView& sv, float4* accg, uint32_t* flags,
                          unsigned long long* counters, bool pot, bool self, const NbrIo* nbr) {
   const int64_t L = (n + K - 1) / K;
   const bool two = tb.group_max > GROUP_MAX;
   const bool quad = tb.quadrupole != 0;
+  const float* nfar = quad ? tb.nfar : nullptr;
   const bool lists = nbr != nullptr && !two;
   const NbrIo io = lists ? *nbr : NbrIo{};
   static unsigned nblk[64] = {0};
@@ -634,7 +679,7 @@ void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float
       nblk[variant] = persistent_blocks((const void*)k_gravity_walk<KA, SE, T, PO, QU, NB>, GW_WAVES * 64);     \
     k_gravity_walk<KA, SE, T, PO, QU, NB><<<nblk[variant], GW_WAVES * 64, 0, s>>>(                              \
         xm, n, K, L, tb.child, tb.ncom, tb.nbmax, tb.nqa, tb.nqb, groups, ngroups, tb.wq, sv, accg,               \
-        flags, counters, io);                                                                                   \
+        flags, counters, io, nfar);                                                                             \
   } while (0)
 #define IHS_WALK_P(KA, SE, T, QU, NB)           \
   do {                                          \

@@ -1169,6 +1169,10 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
                     const uint32_t* gate, float group_diag_hs, bool hs_only) {
   const int64_t L = (n + K - 1) / K;
   const float inv_theta = theta > 0.f ? 1.0f / theta : 3.0e38f;
-  if (hs_only) upsweep_launch<true>(s, n, K, L, inv_theta, xm, hs, tb, gate, group_diag_hs);
-  else upsweep_launch<false>(s, n, K, L, inv_theta, xm, hs, tb, gate, group_diag_hs);
+  if (hs_only) {
+    upsweep_launch<true>(s, n, K, L, inv_theta, xm, hs, tb, gate, group_diag_hs);
+  } else {
+    upsweep_launch<false>(s, n, K, L, inv_theta, xm, hs, tb, gate, group_diag_hs);
+    launch_far_radius(s, n, K, theta, tb);
+  }
 }

@@ -52,7 +52,8 @@ class GpuSph:
                  h_tol: float = 1e-4, h_max_iter: int = 40, h_margin: float = 1.25, h_cap_factor: float = 4.0,
                  nbr_capacity: int = 192, use_graph: bool = True, h_margin_seeded: float = 1.03,
                  group_max: int = 64, quadrupole: bool = True, reuse_positions: bool = False,
-                 fused_walk: bool = True):
+                 fused_walk: bool = True, theta_mono: float = 0.0,
+                 quad_tol: float = 0.0):
         self._L = _lib.load()
         cfg = default_config()
         cfg.capacity = int(capacity)
@@ -66,6 +67,8 @@ class GpuSph:
         cfg.quadrupole = int(bool(quadrupole))
         cfg.reuse_positions = int(bool(reuse_positions))
         cfg.fused_walk = int(bool(fused_walk))
+        cfg.theta_mono = float(theta_mono)
+        cfg.quad_tol = float(quad_tol)
         self.config = cfg
         h = C.c_void_p()
         rc = self._L.ihs_create(int(device), C.byref(cfg), C.byref(h))

@@ -123,7 +123,8 @@ pub struct ihs_config {
     pub quadrupole: i32,
     pub reuse_positions: i32,
     pub fused_walk: i32,
-    pub reserved: [i32; 2],
+    pub theta_mono: f32,
+    pub quad_tol: f32,
 }
 
 /// Opaque library handle.

@@ -24,10 +24,11 @@ def build(count: int = 64, seed: int | None = None):
 
 
 def _handle(capacity, params, h_mode, precision, weights):
+    from . import autotune
     from .gpu_sph import GpuSph
 
-    sph = GpuSph(capacity, leaf_size=1, precision=precision, h_mode=h_mode, theta=params["theta"],
-                 group_max=params["group"], quadrupole=params["quadrupole"])
+    sph = GpuSph(capacity, leaf_size=1, precision=precision, h_mode=h_mode,
+                 **autotune.walk_kwargs(params, leaf=False))
     if weights is not None and h_mode == "learned":
         sph.load_mlp(weights)
     return sph

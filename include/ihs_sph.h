@@ -78,7 +78,11 @@ typedef struct {
   int32_t fused_walk;    /* 1: the gravity walk also writes the neighbour lists (one traversal of the tree per
                             evaluation instead of two; the same neighbour sets, listed in walk order), when walk and
                             neighbour groups coincide (group_max 64); 0: a separate neighbour build (default 1) */
-  int32_t reserved[2];
+  float theta_mono;      /* with quadrupole: a cell that also passes the opening test at this smaller angle acts as
+                            a monopole (its quadrupole term is below the error of the cells accepted at theta);
+                            0: every accepted cell takes its quadrupole (default 0) */
+  float quad_tol;        /* with quadrupole: a cell whose quadrupole term is bounded below quad_tol times its
+                            monopole term (3.5 |q|_F / r^2 at the group's nearest point) acts as a monopole; 0: off */
 } ihs_config;
 
 typedef struct ihs_handle ihs_handle;

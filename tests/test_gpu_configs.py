@@ -81,7 +81,7 @@ def test_cfg4_learned_split_at_1m_meets_tolerances_and_saves_work():
     ic = ics.config_ic(4)
     assert ic.n == 1048576
     p = autotune.tuned_params(ic.n)
-    kw = dict(leaf_size=p["leaf"], group_max=p["group"], theta=p["theta"], quadrupole=p["quadrupole"])
+    kw = autotune.walk_kwargs(p)
     res = {}
     for name, mode, prec in (("learned", "learned", "split"), ("plain", "newton", "fp32")):
         with GpuSph(ic.n, precision=prec, h_mode=mode, **kw) as sph:
@@ -111,8 +111,7 @@ def test_cfg4_learned_split_at_1m_meets_tolerances_and_saves_work():
 
 
 def _sweep_single(ic, params, weights, steps):
-    with GpuSph(ic.n, leaf_size=params["leaf"], group_max=params["group"], theta=params["theta"],
-                quadrupole=params["quadrupole"], h_mode="learned") as sph:
+    with GpuSph(ic.n, h_mode="learned", **autotune.walk_kwargs(params)) as sph:
         sph.load_mlp(weights)
         sph.upload(ic.particles48(), ic.eos, ic.softening)
         sph.set_dt(0.0)
@@ -128,8 +127,7 @@ def test_cfg5_64_scenarios_learned_batch_equals_one_at_a_time():
     assert params["leaf"] == 1  # any scenario size batches with one-particle leaves
     steps = 3
     ntot = sum(ic.n for ic in scen)
-    with GpuSph(ntot, leaf_size=1, group_max=params["group"], theta=params["theta"],
-                quadrupole=params["quadrupole"], h_mode="learned") as sph:
+    with GpuSph(ntot, leaf_size=1, h_mode="learned", **autotune.walk_kwargs(params, leaf=False)) as sph:
         sph.load_mlp(weights)
         sph.upload_batch([ic.particles48() for ic in scen], scen[0].eos, [ic.softening for ic in scen])
         btree = sph.debug_tree()

@@ -154,6 +154,53 @@ def test_quadrupole_never_worse_on_the_cloud(cloud):
         assert r[True] < r[False], (theta, r)
 
 
+@pytest.mark.parametrize("leaf", [1, 8])
+def test_monopole_cells_at_their_limits_reproduce_both_walks(cloud, leaf):
+    """theta_mono = theta, or a quad_tol no quadrupole term stays under, makes every accepted cell a monopole: the
+    accelerations are the monopole walk's bit for bit. A vanishing theta_mono and quad_tol make none: the
+    quadrupole walk's bit for bit. Between the two (theta 0.7, theta_mono 0.4, quad_tol 0.03 on the 1500-body
+    cloud) the error stays within twice the quadrupole walk's, far below the monopole walk's."""
+    pos, mass, soft, ref = cloud
+
+    def run(**kw):
+        with GpuSph(2048, leaf_size=leaf, theta=0.7, **kw) as sph:
+            return sph.gravity(pos, mass, soft, theta=0.7, mode="tree")
+
+    mono, quad = run(quadrupole=False), run(quadrupole=True)
+    assert not np.array_equal(mono, quad)
+    assert np.array_equal(run(quadrupole=True, theta_mono=0.7), mono)
+    assert np.array_equal(run(quadrupole=True, quad_tol=1e30), mono)
+    assert np.array_equal(run(quadrupole=True, theta_mono=1e-6, quad_tol=1e-30), quad)
+    r_mono, _ = rms_rel(mono, ref)
+    r_quad, _ = rms_rel(quad, ref)
+    r_mid, _ = rms_rel(run(quadrupole=True, theta_mono=0.4, quad_tol=0.03), ref)
+    assert r_mid < 0.5 * (r_mono + r_quad) and r_mid < 2.0 * r_quad, (r_mono, r_mid, r_quad)
+
+
+def test_monopole_cells_keep_the_impact_force_error():
+    """The 262,144-particle impact at the 1M tuning's walk (leaf 1, group 64, theta 0.9, quadrupole): with far and
+    weak-quadrupole cells as monopoles (theta_mono 0.55, quad_tol 0.03) the walk sums fewer quadrupole cells, its
+    RMS force error against the FP64 direct sum stays within 5% of the all-quadrupole walk's and within the 1e-3
+    bound, and the neighbour lists (written by the same walk) are unchanged."""
+    ic = ics.config_ic(2)
+    out = {}
+    for tm, qt in ((0.0, 0.0), (0.55, 0.03)):
+        with GpuSph(ic.n, leaf_size=1, group_max=64, theta=0.9, quadrupole=True, theta_mono=tm, quad_tol=qt,
+                    h_mode="newton") as sph:
+            sph.upload(ic.particles48(), ic.eos, ic.softening)
+            sph.stats(reset=True)
+            sph.force_eval()
+            st = sph.stats(reset=True)
+            err = sph.gravity_error(4096)
+            out[tm] = (st, err, sph.field("ncount").copy())
+    (st0, e0, n0), (st1, e1, n1) = out[0.0], out[0.55]
+    print(f"cells {st0['grav_cell_interactions'] / ic.n:.1f} -> {st1['grav_cell_interactions'] / ic.n:.1f} per "
+          f"particle, RMS {e0['rms']:.3e} -> {e1['rms']:.3e}")
+    assert st1["grav_cell_interactions"] < 0.85 * st0["grav_cell_interactions"]
+    assert e1["rms"] < 1e-3 and e1["rms"] < 1.05 * e0["rms"], (e0["rms"], e1["rms"])
+    assert np.array_equal(n0, n1)
+
+
 def test_tree_top_phase_beyond_its_lds_table():
     """8,388,608 bodies, one per leaf (bh_cloud seed 7): 12,222 tree nodes hold more than the upsweep's 1,024-leaf
     block roots (counted on the oracle's tree), more than the top phase's 8,192-entry LDS table holds, so part of the

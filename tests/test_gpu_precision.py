@@ -17,8 +17,7 @@ pytestmark = pytest.mark.gpu
 
 
 def _state(ic, precision, params):
-    with GpuSph(ic.n, leaf_size=params["leaf"], theta=params["theta"], group_max=params["group"],
-                quadrupole=params["quadrupole"], precision=precision, h_mode="newton") as sph:
+    with GpuSph(ic.n, precision=precision, h_mode="newton", **autotune.walk_kwargs(params)) as sph:
         sph.upload(ic.particles48(), ic.eos, ic.softening)
         sph.set_dt(0.0)
         sph.encode_kdk(30)   # the bodies start in rigid motion (du/dt = 0): let self-gravity compress them first

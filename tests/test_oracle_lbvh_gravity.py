@@ -206,3 +206,43 @@ def test_walk_counts_are_consistent():
     for c in (c5, c9):
         own = 1  # the target's own one-particle leaf contributes no interaction
         assert np.all(c[:, 0] + c[:, 1] + own + c[:, 2] == 1 + 2 * c[:, 2])
+
+
+def test_monopole_cell_thresholds():
+    """The walk lets an accepted quadrupole cell act as a monopole on two tests, both resting on facts checked here
+    in FP64 on random cells:
+    (1) the quadrupole part of a cell's acceleration, differentiated numerically from the expansion's potential
+        phi = -M [1/r + (d.q.d) / (2 r^5)] (q traceless, mass-normalised; d = com - x), is at most 3.5 |q|_F M / r^4,
+        a fraction 3.5 |q|_F / r^2 of the monopole's M / r^2 (so d^2 >= 3.5 |q|_F / tol keeps it below tol); the
+        largest ratio met is about 0.35 (an eigenvector of q: 3 |lambda| against 3.5 sqrt(6) |lambda|);
+    (2) a critical radius rc = b / theta + delta scaled by theta / theta_mono is at least b / theta_mono + delta, so
+        the test d >= rc theta / theta_mono is the opening test at theta_mono or a stricter one."""
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for _ in range(400):
+        a = rng.normal(size=(3, 3)) * rng.uniform(0.1, 10.0)
+        q = a + a.T
+        q -= np.trace(q) / 3.0 * np.eye(3)
+        qf = np.sqrt((q * q).sum())
+        d = rng.normal(size=3) * rng.uniform(5.0, 50.0)
+        M = rng.uniform(0.5, 2.0)
+
+        def phi(x, quad):
+            dd = d - x
+            r = np.linalg.norm(dd)
+            return -M * (1.0 / r + (quad * (dd @ q @ dd) / (2.0 * r ** 5)))
+
+        def grad(quad):
+            h = 1e-4 * np.linalg.norm(d)
+            return np.array([(phi(h * e, quad) - phi(-h * e, quad)) / (2 * h) for e in np.eye(3)])
+
+        a_quad = -(grad(1.0) - grad(0.0))  # a = -grad_x phi, quadrupole part only
+        r = np.linalg.norm(d)
+        bound = 3.5 * qf * M / r ** 4
+        ratio = np.linalg.norm(a_quad) / bound
+        assert ratio <= 1.0 + 1e-6, ratio
+        worst = max(worst, ratio)
+    assert worst > 0.3, worst
+    for b, delta, th, tm in rng.uniform([1.0, 0.0, 0.5, 0.1], [10.0, 5.0, 1.2, 0.5], size=(1000, 4)):
+        rc = b / th + delta
+        assert rc * th / tm >= b / tm + delta - 1e-12
