@@ -239,7 +239,7 @@ int alloc_all(ihs_handle* h) {
   t.quadrupole = h->cfg.quadrupole;
   t.theta_mono = h->cfg.theta_mono;
   t.quad_tol = h->cfg.quad_tol;
-  if (t.quadrupole && (t.theta_mono > 0.f || t.quad_tol > 0.f)) HIP_TRY(h, dalloc(h, &t.nfar, 2 * Lc));
+  if (t.quadrupole && (t.theta_mono > 0.f || t.quad_tol > 0.f)) HIP_TRY(h, dalloc(h, &t.nmac, 2 * Lc));
   HIP_TRY(h, dalloc(h, &t.gstart, Lc));
   HIP_TRY(h, dalloc(h, &t.gblk, Lc / 1024 + 2));
   HIP_TRY(h, dalloc(h, &t.groups, C + 1));

@@ -144,7 +144,7 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
                                            const float4* __restrict__ nqa, const float2* __restrict__ nqb,
                                            float soft2, float4* __restrict__ accg, uint32_t* __restrict__ flags,
                                            uint64_t& n_cell, uint64_t& n_pp, uint64_t& n_nodes, const NbrIo& io,
-                                           uint64_t& ncand, uint64_t& npairs, const float* __restrict__ nfar) {
+                                           uint64_t& ncand, uint64_t& npairs, const float2* __restrict__ nmac) {
   static_assert(!NBR || T == 1, "neighbour lists are written by walks with one target per lane");
   const int64_t t0 = grp.x;
   int64_t t[T];
@@ -329,8 +329,14 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
     float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
     if (live) {
       c = ncom[node];
-      const float rc = nbmax[node].w;
-      const float dfar = nfar ? nfar[node] : 3.0e38f;
+      float rc, dfar = 3.0e38f;
+      if (nmac) {
+        const float2 m = nmac[node];
+        rc = m.x;
+        dfar = m.y;
+      } else {
+        rc = nbmax[node].w;
+      }
       const float dx = fmaxf(fmaxf(bx0 - c.x, c.x - bx1), 0.f);
       const float dy = fmaxf(fmaxf(by0 - c.y, c.y - by1), 0.f);
       const float dz = fmaxf(fmaxf(bz0 - c.z, c.z - bz1), 0.f);
@@ -454,7 +460,7 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
                                                                  uint32_t* __restrict__ wq, SegView sv,
                                                                  float4* __restrict__ accg, uint32_t* __restrict__ flags,
                                                                  unsigned long long* __restrict__ counters, NbrIo io,
-                                                                 const float* __restrict__ nfar) {
+                                                                 const float2* __restrict__ nmac) {
   __shared__ WalkLds<QUAD> lds[GW_WAVES];
   const int w = threadIdx.x >> 6;
   const uint32_t lane = threadIdx.x & 63;
@@ -473,7 +479,7 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
       const int4 grp = groups[g];
       const float soft2 = sv.soft2[sv.of(grp.x)];
       walk_group<KAHAN, SELF, T, POT, QUAD, NBR>(S, grp, lane, xm, n, K, L, child, ncom, nbmax, nqa, nqb, soft2,
-                                                 accg, flags, n_cell, n_pp, n_nodes, io, ncand, npairs, nfar);
+                                                 accg, flags, n_cell, n_pp, n_nodes, io, ncand, npairs, nmac);
     }
   }
   if (lane == 0) {
@@ -626,40 +632,41 @@ void launch_acc_out(hipStream_t s, const uint32_t* perm, int64_t n, const float4
   k_acc_out<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(perm, n, acc4, acc3, pot);
 }
 
-// Smallest squared distance (from a walk group's box) at which an accepted cell acts as a monopole: the opening test
-// at theta_mono, d >= rc theta / theta_mono (far2 = (theta / theta_mono)^2; rc = b / theta + delta, so this is at
-// least b / theta_mono + delta), or a quadrupole term below tol times the monopole term. With d = com - x and
-// r >= |d|, |a_quad| = M |-q d / r^5 + 5/2 (d.q.d) d / r^7| <= 3.5 |q|_F M / r^4, a fraction 3.5 |q|_F / r^2 of the
-// monopole's M / r^2: d^2 >= 3.5 |q|_F / tol. One-particle leaves (K = 1) store no quadrupole and are monopoles.
-__global__ __launch_bounds__(256) void k_far_radius(int64_t nodes, int64_t leaf0, int K, const float4* __restrict__ nbmax,
+// Each node's opening record for walks that let far or weak-quadrupole cells act as monopoles: (critical radius,
+// smallest squared distance from a walk group's box at which the accepted cell is a monopole), one 8-byte load at
+// the opening test in place of the critical radius alone. The distance: the opening test at theta_mono,
+// d >= rc theta / theta_mono (far2 = (theta / theta_mono)^2; rc = b / theta + delta, so this is at least
+// b / theta_mono + delta), or a quadrupole term below tol times the monopole term: with d = com - x and r >= |d|,
+// |a_quad| = M |-q d / r^5 + 5/2 (d.q.d) d / r^7| <= 3.5 |q|_F M / r^4, a fraction 3.5 |q|_F / r^2 of the monopole's
+// M / r^2, so d^2 >= 3.5 |q|_F / tol. One-particle leaves (K = 1) store no quadrupole and are monopoles.
+__global__ __launch_bounds__(256) void k_mac_record(int64_t nodes, int64_t leaf0, int K, const float4* __restrict__ nbmax,
                                                     const float4* __restrict__ nqa, const float2* __restrict__ nqb,
-                                                    float far2, float qscale, float* __restrict__ nfar) {
+                                                    float far2, float qscale, float2* __restrict__ nmac) {
   const int64_t x = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (x >= nodes) return;
-  if (K == 1 && x >= leaf0) {
-    nfar[x] = 0.f;
-    return;
-  }
   const float rc = nbmax[x].w;
-  float d2 = far2 > 0.f ? rc * rc * far2 : 3.0e38f;
-  if (qscale > 0.f) {
-    const float4 qa = nqa[x];
-    const float2 qb = nqb[x];
-    const float qf = __builtin_sqrtf(qa.x * qa.x + qa.y * qa.y + qa.z * qa.z +
-                                     2.f * (qa.w * qa.w + qb.x * qb.x + qb.y * qb.y));
-    d2 = fminf(d2, qf * qscale);
+  float d2 = 0.f;
+  if (K != 1 || x < leaf0) {
+    d2 = far2 > 0.f ? rc * rc * far2 : 3.0e38f;
+    if (qscale > 0.f) {
+      const float4 qa = nqa[x];
+      const float2 qb = nqb[x];
+      const float qf = __builtin_sqrtf(qa.x * qa.x + qa.y * qa.y + qa.z * qa.z +
+                                       2.f * (qa.w * qa.w + qb.x * qb.x + qb.y * qb.y));
+      d2 = fminf(d2, qf * qscale);
+    }
   }
-  nfar[x] = d2;
+  nmac[x] = make_float2(rc, d2);
 }
 
-void launch_far_radius(hipStream_t s, int64_t n, int K, float theta, const TreeBufs& tb) {
-  if (!tb.nfar || !tb.quadrupole) return;
+void launch_mac_record(hipStream_t s, int64_t n, int K, float theta, const TreeBufs& tb) {
+  if (!tb.nmac || !tb.quadrupole) return;
   const int64_t L = (n + K - 1) / K;
   const float r = tb.theta_mono > 0.f && theta > 0.f ? theta / tb.theta_mono : 0.f;
   const float qscale = tb.quad_tol > 0.f ? 3.5f / tb.quad_tol : 0.f;
   const int64_t nodes = 2 * L - 1;
-  k_far_radius<<<(unsigned)((nodes + 255) / 256), 256, 0, s>>>(nodes, L - 1, K, tb.nbmax, tb.nqa, tb.nqb, r * r,
-                                                                qscale, tb.nfar);
+  k_mac_record<<<(unsigned)((nodes + 255) / 256), 256, 0, s>>>(nodes, L - 1, K, tb.nbmax, tb.nqa, tb.nqb, r * r,
+                                                                qscale, tb.nmac);
 }
 
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
@@ -668,7 +675,7 @@ void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float
   const int64_t L = (n + K - 1) / K;
   const bool two = tb.group_max > GROUP_MAX;
   const bool quad = tb.quadrupole != 0;
-  const float* nfar = quad ? tb.nfar : nullptr;
+  const float2* nmac = quad ? tb.nmac : nullptr;
   const bool lists = nbr != nullptr && !two;
   const NbrIo io = lists ? *nbr : NbrIo{};
   static unsigned nblk[64] = {0};
@@ -679,7 +686,7 @@ void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float
       nblk[variant] = persistent_blocks((const void*)k_gravity_walk<KA, SE, T, PO, QU, NB>, GW_WAVES * 64);     \
     k_gravity_walk<KA, SE, T, PO, QU, NB><<<nblk[variant], GW_WAVES * 64, 0, s>>>(                              \
         xm, n, K, L, tb.child, tb.ncom, tb.nbmax, tb.nqa, tb.nqb, groups, ngroups, tb.wq, sv, accg,               \
-        flags, counters, io, nfar);                                                                             \
+        flags, counters, io, nmac);                                                                             \
   } while (0)
 #define IHS_WALK_P(KA, SE, T, QU, NB)           \
   do {                                          \

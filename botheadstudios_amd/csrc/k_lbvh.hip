@@ -1173,6 +1173,6 @@ void launch_upsweep(hipStream_t s, int64_t n, int K, float theta, const float4* 
     upsweep_launch<true>(s, n, K, L, inv_theta, xm, hs, tb, gate, group_diag_hs);
   } else {
     upsweep_launch<false>(s, n, K, L, inv_theta, xm, hs, tb, gate, group_diag_hs);
-    launch_far_radius(s, n, K, theta, tb);
+    launch_mac_record(s, n, K, theta, tb);
   }
 }

@@ -112,7 +112,8 @@ struct TreeBufs {
   int quadrupole = 1;       // the walk adds the cells' quadrupole terms
   float theta_mono = 0.f;   // > 0: cells that pass the opening test at this angle act as monopoles
   float quad_tol = 0.f;     // > 0: cells whose quadrupole term is below quad_tol x their monopole's act as monopoles
-  float* nfar = nullptr;    // 2L-1 (theta_mono or quad_tol set): squared distance beyond which a cell is a monopole
+  float2* nmac = nullptr;   // 2L-1 (theta_mono or quad_tol set): critical radius, squared distance beyond which an
+                            // accepted cell is a monopole
   // walk groups: contiguous Morton ranges of at most group_max particles with compact boxes, in leaf order
   uint32_t* gstart = nullptr;   // L: group root node if leaf c is the first leaf of a group; GROUP_SPLIT if
                                 // leaf c is split into single-particle groups; none otherwise
@@ -195,9 +196,9 @@ struct NbrIo {
   int cap = 0;
   int64_t stride = 0;
 };
-// After a full upsweep (the moments changed): each node's monopole distance tb.nfar from theta_mono and quad_tol
-// (nothing when neither is set).
-void launch_far_radius(hipStream_t s, int64_t n, int K, float theta, const TreeBufs& tb);
+// After a full upsweep (the moments changed): each node's opening record tb.nmac (critical radius, monopole
+// distance from theta_mono and quad_tol; nothing when neither is set).
+void launch_mac_record(hipStream_t s, int64_t n, int K, float theta, const TreeBufs& tb);
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
                          const int4* groups, const uint32_t* ngroups, const SegView& sv, float4* accg, uint32_t* flags,
                          unsigned long long* counters, bool pot, bool self,
