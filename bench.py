@@ -60,6 +60,9 @@ def parse():
     ap.add_argument("--theta-mono", type=float, default=None,
                     help="cells passing the opening test at this angle act as monopoles (0: off); default: the "
                          "autotuned value for this N")
+    ap.add_argument("--theta-settled", type=float, default=None,
+                    help="opening angle of the walk that keeps the neighbour lists at unchanged positions (0: every "
+                         "evaluation rebuilds them); default: the autotuned value for this N")
     ap.add_argument("--quad-tol", type=float, default=None,
                     help="cells whose quadrupole term is below this fraction of their monopole act as monopoles "
                          "(0: off); default: the autotuned value for this N")
@@ -118,6 +121,7 @@ def measured_traffic(n, args, variant):
             abs(rec.get("theta", -1) - args.theta) < 1e-6 and rec.get("quadrupole") == bool(args.quadrupole) and
             abs(rec.get("theta_mono", 0.0) - args.theta_mono) < 1e-6 and
             abs(rec.get("quad_tol", 0.0) - args.quad_tol) < 1e-9 and
+            abs(rec.get("theta_settled", 0.0) - args.theta_settled) < 1e-9 and
             rec.get("variant") == variant and rec.get("fused_walk", False) == FUSED_WALK)
     return rec["bytes_per_launch"] if same else None
 
@@ -157,11 +161,13 @@ def run_variant(ic, precision, h_mode, args, weights=None, reuse=False, steps=No
     from botheadstudios_amd.gpu_sph import GpuSph
 
     w = {**dict(leaf=args.leaf, theta=args.theta, group=args.group, quadrupole=args.quadrupole,
-                theta_mono=args.theta_mono, quad_tol=args.quad_tol), **(walk or {})}
+                theta_mono=args.theta_mono, quad_tol=args.quad_tol, theta_settled=args.theta_settled),
+         **(walk or {})}
     steps = args.steps if steps is None else steps
     sph = GpuSph(ic.n, leaf_size=w["leaf"], precision=precision, h_mode=h_mode, theta=w["theta"],
                  group_max=w["group"], quadrupole=w["quadrupole"], reuse_positions=reuse,
-                 theta_mono=w.get("theta_mono", 0.0), quad_tol=w.get("quad_tol", 0.0))
+                 theta_mono=w.get("theta_mono", 0.0), quad_tol=w.get("quad_tol", 0.0),
+                 theta_settled=w.get("theta_settled", 0.0))
     try:
         if weights is not None:
             sph.load_mlp(weights)
@@ -192,12 +198,18 @@ def run_variant(ic, precision, h_mode, args, weights=None, reuse=False, steps=No
         sph.profile(False)
         # Force error of the last evaluation's tree walk against the FP64 direct sum on 8192 sampled targets.
         gerr = sph.gravity_error(8192)
+        # The first evaluation of each step keeps the lists and walks at theta_settled: its own force error, from an
+        # evaluation at the same (unchanged) positions.
+        gerr_settled = None
+        if w.get("theta_settled", 0.0) > 0:
+            sph.force_eval()
+            gerr_settled = sph.gravity_error(8192)
         for k in ("nbr_overflow", "walk_overflow", "nonfinite"):
             if st[k] or pst[k]:
                 raise RuntimeError(f"{precision}/{h_mode}: {k} = {st[k]}/{pst[k]}")
         sec = t1 - t0
         out = dict(sec=sec, steps=steps, ms_per_step=1e3 * sec / steps, value=ic.n * steps / sec, stats=st,
-                   profile=prof, profile_stats=pst, gravity_error=gerr, walk=w, t_start=t_start,
+                   profile=prof, profile_stats=pst, gravity_error=gerr, gravity_error_settled=gerr_settled, walk=w, t_start=t_start,
                    t_end=sph.segment_clock(0)["t"])
         if keep_state:
             back = sph.readback()
@@ -437,8 +449,9 @@ def main():
     args.quadrupole = tuned["quadrupole"] if args.quadrupole is None else bool(args.quadrupole)
     args.theta_mono = tuned["theta_mono"] if args.theta_mono is None else args.theta_mono
     args.quad_tol = tuned["quad_tol"] if args.quad_tol is None else args.quad_tol
+    args.theta_settled = tuned["theta_settled"] if args.theta_settled is None else args.theta_settled
     log(f"walk parameters: leaf {args.leaf}, group {args.group}, theta {args.theta}, quadrupole {args.quadrupole}, "
-        f"theta_mono {args.theta_mono}, quad_tol {args.quad_tol}")
+        f"theta_mono {args.theta_mono}, quad_tol {args.quad_tol}, theta_settled {args.theta_settled}")
     if args.copy_probe > 0:
         copy_probe(args.copy_probe)
     variants = {}
@@ -472,7 +485,7 @@ def main():
     if "continuity" in wanted and weights is not None:
         add("continuity_split", "split", "learned", args, weights=weights, zero_seed=True, steps=vsteps)
     if "literal" in wanted:
-        lit = dict(leaf=16, theta=0.5, group=64, quadrupole=False, theta_mono=0.0, quad_tol=0.0)
+        lit = dict(leaf=16, theta=0.5, group=64, quadrupole=False, theta_mono=0.0, quad_tol=0.0, theta_settled=0.0)
         add("plain_fp32_literal", "fp32", "newton", args, walk=lit, steps=vsteps)
         if weights is not None:
             add("learned_split_literal", "split", "learned", args, weights=weights, walk=lit, steps=vsteps)
@@ -502,7 +515,8 @@ def main():
         "config": {"workload": f"two-body giant impact, N={ic.n}, 0.9+0.1 Earth masses, 45 deg, 1.1 v_esc; "
                                f"KDK step = 2 force evaluations; variant={head}",
                    "n": ic.n, "leaf": args.leaf, "group": args.group, "theta": args.theta, "quadrupole": args.quadrupole,
-                   "theta_mono": args.theta_mono, "quad_tol": args.quad_tol, "n_ngb": 48, "h_tol": 1e-4, "variant": head,
+                   "theta_mono": args.theta_mono, "quad_tol": args.quad_tol, "theta_settled": args.theta_settled,
+                   "n_ngb": 48, "h_tol": 1e-4, "variant": head,
                    "fused_walk": FUSED_WALK},
         "roofline": roofline(res, ic.n, args, head, walk_work, equal_work),
         "passes": passes(res, ic.n, walk_work["flops"]),
@@ -513,6 +527,8 @@ def main():
                          "gravity_rms_rel_err": v["gravity_error"]["rms"],
                          "gravity_rms_ref_norm": v["gravity_error"]["rms_ref_norm"],
                          "gravity_max_rel_err": v["gravity_error"]["max"],
+                         "gravity_rms_rel_err_kept_lists": (v["gravity_error_settled"] or {}).get("rms"),
+                         "gravity_max_rel_err_kept_lists": (v["gravity_error_settled"] or {}).get("max"),
                          "h_capped": v["stats"]["h_capped"], "h_unconverged": v["stats"]["h_unconverged"],
                          "nbr_candidates_per_particle": v["stats"]["nbr_candidates"] / max(1, v["stats"]["force_evals"]) / ic.n,
                          "walk_nodes_per_group_eval": v["stats"]["walk_nodes"] / max(1, v["stats"]["force_evals"]),

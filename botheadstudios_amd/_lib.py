@@ -18,6 +18,7 @@ class IhsConfig(C.Structure):
         ("nbr_capacity", C.c_int32), ("use_graph", C.c_int32), ("h_margin_seeded", C.c_float),
         ("group_max", C.c_int32), ("quadrupole", C.c_int32), ("reuse_positions", C.c_int32),
         ("fused_walk", C.c_int32), ("theta_mono", C.c_float), ("quad_tol", C.c_float),
+        ("theta_settled", C.c_float), ("reserved", C.c_int32 * 3),
     ]
 
 

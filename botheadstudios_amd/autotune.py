@@ -34,7 +34,8 @@ import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 TUNED = os.path.join(HERE, "data", "tuned.json")
-DEFAULT = {"leaf": 16, "group": 64, "theta": 0.5, "quadrupole": True, "theta_mono": 0.0, "quad_tol": 0.0}
+DEFAULT = {"leaf": 16, "group": 64, "theta": 0.5, "quadrupole": True, "theta_mono": 0.0, "quad_tol": 0.0,
+           "theta_settled": 0.0}
 QUAD_THETAS = (1.2, 1.1, 1.0, 0.95, 0.9, 0.875, 0.85, 0.825, 0.8, 0.75, 0.7, 0.6, 0.5, 0.45, 0.4)
 MONO_THETAS = (0.5, 0.45, 0.42, 0.4, 0.375, 0.35)
 
@@ -43,12 +44,16 @@ MONO_GRID = ((0.0, 0.0), (0.0, 0.03), (0.5, 0.03), (0.55, 0.03), (0.6, 0.03), (0
 
 
 def measure(ic, leaf: int, theta: float, group: int = 64, quadrupole: bool = True, steps: int = 3, warmup: int = 1,
-            n_targets: int = 8192, device: int = 0, theta_mono: float = 0.0, quad_tol: float = 0.0) -> dict:
-    """ms/step (plain Newton, FP32) and the walk's force error for one setting on an impact IC."""
+            n_targets: int = 8192, device: int = 0, theta_mono: float = 0.0, quad_tol: float = 0.0,
+            theta_settled: float = 0.0) -> dict:
+    """ms/step (plain Newton, FP32) and the walk's force error for one setting on an impact IC. With theta_settled,
+    also the error of the walk that keeps the lists (an evaluation at the unchanged positions after the steps);
+    err_rms is then the larger of the two."""
     from .gpu_sph import GpuSph
 
     with GpuSph(ic.n, device=device, leaf_size=leaf, theta=theta, group_max=group, quadrupole=quadrupole,
-                h_mode="newton", use_graph=True, theta_mono=theta_mono, quad_tol=quad_tol) as sph:
+                h_mode="newton", use_graph=True, theta_mono=theta_mono, quad_tol=quad_tol,
+                theta_settled=theta_settled) as sph:
         sph.upload(ic.particles48(), ic.eos, ic.softening)
         sph.set_dt(0.0)
         sph.encode_kdk(warmup)
@@ -59,8 +64,13 @@ def measure(ic, leaf: int, theta: float, group: int = 64, quadrupole: bool = Tru
         ms = 1e3 * (time.perf_counter() - t0) / steps
         st = sph.stats()
         err = sph.gravity_error(n_targets)
+        if theta_settled > 0:
+            sph.force_eval()  # positions and h unchanged: the lists are kept and the walk runs at theta_settled
+            e2 = sph.gravity_error(n_targets)
+            err = {"rms": max(err["rms"], e2["rms"]), "max": max(err["max"], e2["max"]), "rms_settled": e2["rms"]}
     return {"leaf": leaf, "group": group, "theta": theta, "quadrupole": bool(quadrupole), "theta_mono": theta_mono,
-            "quad_tol": quad_tol, "ms_per_step": ms, "err_rms": err["rms"],
+            "quad_tol": quad_tol, "theta_settled": theta_settled, "ms_per_step": ms, "err_rms": err["rms"],
+            "err_rms_settled": err.get("rms_settled"),
             "err_max": err["max"], "ok": st["nbr_overflow"] == 0 and st["walk_overflow"] == 0 and st["nonfinite"] == 0}
 
 
@@ -109,6 +119,33 @@ def tune_monopole(ic, base: dict, err_target: float = 1e-3, margin: float = 0.92
     return {"best": best, "table": rows}
 
 
+SETTLED_THETAS = (0.9, 0.85, 0.8, 0.75, 0.7, 0.65, 0.6)
+
+
+def tune_settled(ic, base: dict, err_target: float = 1e-3, margin: float = 0.92, thetas=SETTLED_THETAS,
+                 steps: int = 3, reps: int = 2, log=print) -> dict:
+    """Third stage: the opening angle of the walk that keeps the lists (the first evaluation of a KDK step), at the
+    tuned walk. Without the neighbour search's opened nodes that walk is less accurate at the same angle; the
+    fastest theta_settled whose error (and the list-building walk's) stays within margin * err_target wins."""
+    table = {}
+    for _ in range(reps):
+        for th in thetas:
+            if th > base["theta"]:
+                continue
+            m = measure(ic, base["leaf"], base["theta"], base["group"], base["quadrupole"], steps=steps,
+                        theta_mono=base.get("theta_mono", 0.0), quad_tol=base.get("quad_tol", 0.0), theta_settled=th)
+            if th not in table or m["ms_per_step"] < table[th]["ms_per_step"]:
+                table[th] = m
+            log(f"  theta_settled {th:.3f}: {m['ms_per_step']:.3f} ms/step, err rms {m['err_rms']:.2e} "
+                f"(kept lists {m['err_rms_settled']:.2e})")
+    rows = list(table.values())
+    feasible = [m for m in rows if m["ok"] and m["err_rms"] <= margin * err_target]
+    best = min(feasible, key=lambda m: m["ms_per_step"]) if feasible else None
+    if best is not None and best["ms_per_step"] >= base.get("ms_per_step", float("inf")):
+        best = None  # keeping the lists must pay for itself
+    return {"best": best, "table": rows}
+
+
 def bucket(n: int) -> str:
     """Particle-count bucket key (powers of two)."""
     return str(1 << max(0, int(round(math.log2(max(n, 1))))))
@@ -138,13 +175,14 @@ def tuned_params(n: int, path: str = TUNED) -> dict:
     best = min(cands, key=lambda c: c[0])[1]["best"]
     return {"leaf": int(best["leaf"]), "group": int(best["group"]), "theta": float(best["theta"]),
             "quadrupole": bool(best.get("quadrupole", False)), "theta_mono": float(best.get("theta_mono", 0.0)),
-            "quad_tol": float(best.get("quad_tol", 0.0))}
+            "quad_tol": float(best.get("quad_tol", 0.0)), "theta_settled": float(best.get("theta_settled", 0.0))}
 
 
 def walk_kwargs(params: dict, leaf: bool = True) -> dict:
     """GpuSph keyword arguments of a tuned walk setting (leaf_size left out with leaf=False)."""
     kw = dict(group_max=params["group"], theta=params["theta"], quadrupole=params["quadrupole"],
-              theta_mono=params.get("theta_mono", 0.0), quad_tol=params.get("quad_tol", 0.0))
+              theta_mono=params.get("theta_mono", 0.0), quad_tol=params.get("quad_tol", 0.0),
+              theta_settled=params.get("theta_settled", 0.0))
     if leaf:
         kw["leaf_size"] = params["leaf"]
     return kw
@@ -229,6 +267,8 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--out", default=TUNED)
     ap.add_argument("--mono", action="store_true", help="only the monopole-cell stage, at the stored tuned walk")
+    ap.add_argument("--settled", action="store_true",
+                    help="only the kept-list walk's opening angle, at the stored tuned walk")
     a = ap.parse_args(argv)
     if a.knee:
         res = gravity_knee()
@@ -237,16 +277,24 @@ def main(argv=None):
         return
     ic = ics.config_ic(a.cfg, n=a.n)
     t0 = time.time()
-    if a.mono:
+    if a.mono or a.settled:
         with open(a.out) as f:
             res = json.load(f)[bucket(ic.n)]
     else:
         res = tune(ic, err_target=a.err, steps=a.steps)
-    if res["best"] and res["best"].get("quadrupole"):
+    if not a.settled and res["best"] and res["best"].get("quadrupole"):
         mono = tune_monopole(ic, res["best"], err_target=a.err, margin=res.get("margin", 0.92), steps=a.steps)
         res["monopole_table"] = mono["table"]
         if mono["best"]:
             res["best"] = mono["best"]
+    if res["best"]:
+        base = dict(res["best"], theta_settled=0.0)
+        if a.settled:  # the list-building step's time at the same moment, for the comparison
+            base.update(measure(ic, base["leaf"], base["theta"], base["group"], base["quadrupole"], steps=a.steps,
+                                theta_mono=base.get("theta_mono", 0.0), quad_tol=base.get("quad_tol", 0.0)))
+        st = tune_settled(ic, base, err_target=a.err, margin=res.get("margin", 0.92), steps=a.steps)
+        res["settled_table"] = st["table"]
+        res["best"] = st["best"] or base
     save(res, a.out)
     print(json.dumps({"best": res["best"], "n": ic.n, "seconds": round(time.time() - t0, 1)}), flush=True)
 

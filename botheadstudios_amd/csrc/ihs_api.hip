@@ -115,6 +115,7 @@ struct ihs_handle {
   std::vector<int64_t> seg_start_host;  // nseg + 1
   bool zero_soft = false;               // some scenario has no softening: the walk skips self pairs by index
   int64_t force_evals = 0;
+  int64_t list_builds = 0;  // evaluations that built neighbour lists (the mean neighbour count's denominator)
   bool have_fe = false;
   bool pot_fresh = false;  // accg.w holds the potential of the last force evaluation
   // The last force evaluation solved h at the current positions (nothing has drifted since), so the next
@@ -131,6 +132,7 @@ struct ihs_handle {
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   int graph_next[2] = {0, 1};  // the buffer a captured step leaves the state in
+  int graph_builds[2] = {2, 2};  // list-building evaluations in a captured step
   // stage profiling
   bool prof_on = false;
   std::vector<hipEvent_t> prof_ev;
@@ -364,6 +366,11 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   const int64_t n = h->n;
   const int K = h->cfg.leaf_size;
   const bool sorted = h->tree_fresh && !after_drift;
+  const bool newton_mode = h->cfg.h_mode != IHS_H_FIXED;
+  // At positions unchanged since the last full evaluation and with no solve to run, the neighbour lists and walk
+  // groups that evaluation left are the ones a rebuild would find (the same positions and h): with theta_settled set
+  // they are kept, and gravity takes a walk that builds no lists, at theta_settled.
+  const bool keep_lists = sorted && h->cfg.theta_settled > 0.f && !(newton_mode && !h->h_settled);
   MasterBufs& S = h->st[h->cur];
   MasterBufs& D = sorted ? S : h->st[h->cur ^ 1];
   Scratch& sc = h->sc;
@@ -379,6 +386,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   fe.clk = sv.clk;
   fe.hs_npend = h->hw.npend;
   fe.n_hs_npend = PEND_SHARDS;
+  if (keep_lists) fe.walk_q = h->tb.wq;
   float margin = 1.0f;
   if (newton) margin = (!after_drift && h->h_settled) ? kMarginSettled : h->cfg.h_margin;
   if (sorted) {
@@ -402,15 +410,22 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   const float eff_margin = seeded ? h->cfg.h_margin_seeded : margin;
   const float diag_scale = newton ? h->cfg.h_margin / eff_margin : 1.0f;
   if (!sorted) launch_karras(s, n, K, h->tb, sv);
-  // Kept sort: the node moments in memory are those of these positions (the last full upsweep's); only hs changed.
-  launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, nullptr, NBR_DIAG_HS * diag_scale, sorted);
-  launch_groups(s, n, K, h->tb, sv, nullptr, NBR_DIAG_HS * diag_scale, false, true);
-  if (h->tb.ggroups != h->tb.groups) launch_groups(s, n, K, h->tb, sv, nullptr, GROUP_DIAG_HS * diag_scale, true);
+  if (!keep_lists) {
+    // Kept sort: the node moments in memory are those of these positions (the last full upsweep's); only hs changed.
+    launch_upsweep(s, n, K, h->cfg.theta, sc.xm, sc.hs, h->tb, nullptr, NBR_DIAG_HS * diag_scale, sorted);
+    launch_groups(s, n, K, h->tb, sv, nullptr, NBR_DIAG_HS * diag_scale, false, true);
+    if (h->tb.ggroups != h->tb.groups) launch_groups(s, n, K, h->tb, sv, nullptr, GROUP_DIAG_HS * diag_scale, true);
+  }
   prof_mark(h, IHS_STAGE_TREE);
   // One traversal for gravity and the neighbour lists when the walk and neighbour groups coincide: the walk opens
   // the nodes the neighbour search needs and writes the lists (gravity does not depend on h, so it runs first).
-  const bool fused = h->cfg.fused_walk && h->tb.ggroups == h->tb.groups && h->tb.group_max <= GROUP_MAX;
-  if (fused) {
+  const bool fused = !keep_lists && h->cfg.fused_walk && h->tb.ggroups == h->tb.groups && h->tb.group_max <= GROUP_MAX;
+  if (keep_lists) {
+    const float r = h->cfg.theta / h->cfg.theta_settled;
+    launch_gravity_walk(s, n, K, kahan, sc.xm, h->tb, h->tb.ggroups, h->tb.nggroups, sv, sc.accg, sc.flags,
+                        sc.counters, need_pot, h->zero_soft, nullptr, r * r);
+    prof_mark(h, IHS_STAGE_GRAVITY);
+  } else if (fused) {
     NbrIo io;
     io.hs = sc.hs;
     io.nbmin = h->tb.nbmin;
@@ -425,10 +440,11 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
     launch_nbr_build(s, n, K, sc.xm, sc.hs, h->tb, sc.nbr, sc.ncount, h->cfg.nbr_capacity, n, sc.flags, sc.counters);
     prof_mark(h, IHS_STAGE_NBR);
   }
+  if (!keep_lists) h->list_builds++;
   // At positions unchanged since the last solve (the first evaluation of a KDK step: the kick changed only v and
   // u) every h already meets the tolerance, and a solve would only confirm it: it is skipped. A particle the last
   // solve left capped or unconverged is solved again at the next evaluation after a drift.
-  const bool solve = newton && !(h->h_settled && !after_drift);
+  const bool solve = newton && !(h->h_settled && !after_drift);  // never with keep_lists
   if (solve) {
     const float eta3 = eta * eta * eta;
     if (record) mlp_record_features(s, h->mlp, n, D, sc.eos, sv, eta);
@@ -461,7 +477,7 @@ void force_eval(ihs_handle* h, bool after_drift, bool need_pot = true) {
   launch_forces(s, kahan, n, sc.xm, sc.xlo, sc.vh, sc.rpc, sc.nbr, sc.ncount, sc.nmask, n,
                 (const float*)((const char*)sc.ctl + offsetof(StepCtl, av)), sc.asph, feat ? D.feat : nullptr);
   prof_mark(h, IHS_STAGE_FORCES);
-  if (!fused) {
+  if (!fused && !keep_lists) {
     launch_gravity_walk(s, n, K, kahan, sc.xm, h->tb, h->tb.ggroups, h->tb.nggroups, sv, sc.accg, sc.flags,
                         sc.counters, need_pot, h->zero_soft);
     prof_mark(h, IHS_STAGE_GRAVITY);
@@ -577,6 +593,7 @@ void ihs_default_config(ihs_config* c) {
   c->fused_walk = 1;
   c->theta_mono = 0.0f;
   c->quad_tol = 0.0f;
+  c->theta_settled = 0.0f;
 }
 
 int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
@@ -590,7 +607,7 @@ int ihs_create(int device, const ihs_config* cfg, ihs_handle** out) {
       !(c.h_margin_seeded >= 1.0f) || c.group_max < c.leaf_size || c.group_max > 2 * GROUP_MAX ||
       c.quadrupole < 0 || c.quadrupole > 1 || c.reuse_positions < 0 || c.reuse_positions > 1 || c.fused_walk < 0 ||
       c.fused_walk > 1 || !(c.theta_mono >= 0.f) || (c.theta_mono > 0.f && !(c.theta_mono <= c.theta)) ||
-      !(c.quad_tol >= 0.f))
+      !(c.quad_tol >= 0.f) || !(c.theta_settled >= 0.f) || c.theta_settled > c.theta)
     return IHS_ERR_ARG;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return IHS_ERR_HIP;
@@ -727,6 +744,7 @@ int ihs_upload_batch(ihs_handle* h, const ihs_particle* p, const int64_t* counts
     HIP_TRY(h, hipMemcpy(sc.flags, fl, sizeof fl, hipMemcpyHostToDevice));
   }
   h->force_evals = 0;
+  h->list_builds = 0;
   return IHS_OK;
 }
 
@@ -820,11 +838,13 @@ int ihs_step_kdk(ihs_handle* h, int32_t substeps) {
   for (int i = 0; i < substeps; ++i) {
     const int par = h->cur;
     if (!h->graph_exec[par]) {
-      const int64_t fe0 = h->force_evals;
+      const int64_t fe0 = h->force_evals, lb0 = h->list_builds;
       HIP_TRY(h, hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
       kdk_step(h);
       hipError_t e = hipStreamEndCapture(h->stream, &h->graph[par]);
       h->force_evals = fe0;
+      h->graph_builds[par] = (int)(h->list_builds - lb0);
+      h->list_builds = lb0;
       h->graph_next[par] = h->cur;
       h->cur = par;
       if (e != hipSuccess) return fail(h, IHS_ERR_HIP, "graph capture failed: %s", hipGetErrorString(e));
@@ -833,6 +853,7 @@ int ihs_step_kdk(ihs_handle* h, int32_t substeps) {
     HIP_TRY(h, hipGraphLaunch(h->graph_exec[par], h->stream));
     h->cur = h->graph_next[par];
     h->force_evals += 2;
+    h->list_builds += h->graph_builds[par];
     h->have_fe = true;
     h->pot_fresh = false;
   }
@@ -957,7 +978,7 @@ int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset) {
   out->mean_h_iters = solved > 0 ? (double)cn[CNT_H_ITERS] / solved : 0.0;
   out->mean_h_evals = solved > 0 ? (double)cn[CNT_H_EVALS] / solved : 0.0;
   out->mean_neighbours =
-      h->force_evals > 0 && h->n > 0 ? (double)cn[CNT_PAIRS] / ((double)h->force_evals * (double)h->n) : 0.0;
+      h->list_builds > 0 && h->n > 0 ? (double)cn[CNT_PAIRS] / ((double)h->list_builds * (double)h->n) : 0.0;
   out->nbr_overflow = fl[FLAG_NBR_OVERFLOW];
   out->walk_overflow = fl[FLAG_WALK_OVERFLOW];
   out->h_capped = fl[FLAG_H_CAPPED];
@@ -973,6 +994,7 @@ int ihs_stats_get(ihs_handle* h, ihs_stats* out, int32_t reset) {
     fl[FLAG_NBR_OVERFLOW] = fl[FLAG_WALK_OVERFLOW] = fl[FLAG_H_CAPPED] = fl[FLAG_H_UNCONVERGED] = fl[FLAG_NONFINITE] = 0;
     HIP_TRY(h, hipMemcpy(h->sc.flags, fl, sizeof fl, hipMemcpyHostToDevice));
     h->force_evals = 0;
+    h->list_builds = 0;
   }
   return IHS_OK;
 }
@@ -1839,6 +1861,7 @@ int ihs_checkpoint_load(ihs_handle* h, const void* buf, int64_t bytes) {
   h->tree_fresh = false;
   h->have_fe = false;
   h->force_evals = hd.force_evals;
+  h->list_builds = 0;
   MlpDev& m = h->mlp;
   m.trained = hd.mlp_trained != 0;
   if (m.trained) {

@@ -137,6 +137,8 @@ __device__ __forceinline__ void interact_quad(const float4 s, const float4 qa, c
 // r_ij^2 < max(hs_i^2, hs_j^2), j != i, in 16-byte chunks nbr4[chunk * stride + t]). A node passing the
 // neighbour search's box test is opened instead of accepted, so every candidate leaf reaches the leaf path, where
 // the candidate's distance is the one its gravity term computes.
+// nmac (optional): each node's (critical radius, monopole distance^2) record (k_mac_record); mac2: the opening test's
+// scale, (theta / theta')^2 for a walk at theta' below the tree's theta (1: the tree's theta).
 template <bool KAHAN, bool SELF, int T, bool POT, bool QUAD, bool NBR>
 __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uint32_t lane, const float4* __restrict__ xm,
                                            int64_t n, int K, int64_t L, const int2* __restrict__ child,
@@ -144,7 +146,8 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
                                            const float4* __restrict__ nqa, const float2* __restrict__ nqb,
                                            float soft2, float4* __restrict__ accg, uint32_t* __restrict__ flags,
                                            uint64_t& n_cell, uint64_t& n_pp, uint64_t& n_nodes, const NbrIo& io,
-                                           uint64_t& ncand, uint64_t& npairs, const float2* __restrict__ nmac) {
+                                           uint64_t& ncand, uint64_t& npairs, const float2* __restrict__ nmac,
+                                           float mac2) {
   static_assert(!NBR || T == 1, "neighbour lists are written by walks with one target per lane");
   const int64_t t0 = grp.x;
   int64_t t[T];
@@ -343,8 +346,10 @@ __device__ __forceinline__ void walk_group(WalkLds<QUAD>& S, const int4 grp, uin
       const float d2 = dx * dx + dy * dy + dz * dz + soft2;
       // Without softening a leaf touching the group box may hold a target itself: it takes the
       // particle-particle path, which skips the self pair.
-      acc = d2 >= rc * rc && (!SELF || node < leaf0 || d2 > 0.f);
-      far = d2 >= dfar;
+      acc = d2 >= rc * rc * mac2 && (!SELF || node < leaf0 || d2 > 0.f);
+      // The monopole distance scales with the opening test: a walk at theta' (mac2 = (theta / theta')^2) takes its
+      // monopole cells at theta_mono theta' / theta.
+      far = d2 >= dfar * mac2;
       if (NBR && acc) {
         // The neighbour build's box test (group box against the node box grown by the larger search radius): a
         // node that passes is opened, and its leaves come to the leaf path.
@@ -460,7 +465,7 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
                                                                  uint32_t* __restrict__ wq, SegView sv,
                                                                  float4* __restrict__ accg, uint32_t* __restrict__ flags,
                                                                  unsigned long long* __restrict__ counters, NbrIo io,
-                                                                 const float2* __restrict__ nmac) {
+                                                                 const float2* __restrict__ nmac, float mac2) {
   __shared__ WalkLds<QUAD> lds[GW_WAVES];
   const int w = threadIdx.x >> 6;
   const uint32_t lane = threadIdx.x & 63;
@@ -479,7 +484,7 @@ __global__ __launch_bounds__(GW_WAVES * 64) void k_gravity_walk(const float4* __
       const int4 grp = groups[g];
       const float soft2 = sv.soft2[sv.of(grp.x)];
       walk_group<KAHAN, SELF, T, POT, QUAD, NBR>(S, grp, lane, xm, n, K, L, child, ncom, nbmax, nqa, nqb, soft2,
-                                                 accg, flags, n_cell, n_pp, n_nodes, io, ncand, npairs, nmac);
+                                                 accg, flags, n_cell, n_pp, n_nodes, io, ncand, npairs, nmac, mac2);
     }
   }
   if (lane == 0) {
@@ -671,7 +676,7 @@ void launch_mac_record(hipStream_t s, int64_t n, int K, float theta, const TreeB
 
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
                          const int4* groups, const uint32_t* ngroups, const SegView& sv, float4* accg, uint32_t* flags,
-                         unsigned long long* counters, bool pot, bool self, const NbrIo* nbr) {
+                         unsigned long long* counters, bool pot, bool self, const NbrIo* nbr, float mac2) {
   const int64_t L = (n + K - 1) / K;
   const bool two = tb.group_max > GROUP_MAX;
   const bool quad = tb.quadrupole != 0;
@@ -686,7 +691,7 @@ void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float
       nblk[variant] = persistent_blocks((const void*)k_gravity_walk<KA, SE, T, PO, QU, NB>, GW_WAVES * 64);     \
     k_gravity_walk<KA, SE, T, PO, QU, NB><<<nblk[variant], GW_WAVES * 64, 0, s>>>(                              \
         xm, n, K, L, tb.child, tb.ncom, tb.nbmax, tb.nqa, tb.nqb, groups, ngroups, tb.wq, sv, accg,               \
-        flags, counters, io, nmac);                                                                             \
+        flags, counters, io, nmac, mac2);                                                                       \
   } while (0)
 #define IHS_WALK_P(KA, SE, T, QU, NB)           \
   do {                                          \

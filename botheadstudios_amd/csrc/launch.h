@@ -58,6 +58,8 @@ struct EvalInit {
   SegClock* clk = nullptr;       // every scenario's Courant signal is re-armed
   uint32_t* hs_npend = nullptr;  // the h-solve's pending-list lengths are cleared (n_hs_npend of them)
   int n_hs_npend = 0;
+  uint32_t* walk_q = nullptr;    // the gravity walk's 8 per-XCD queue counters are cleared (an evaluation that keeps
+                                 // the walk groups, whose build clears them otherwise)
 };
 
 // The reset, by the threads of block 0 of the evaluation's first kernel (k_bbox, or k_stage when the sort is kept).
@@ -69,6 +71,7 @@ __device__ __forceinline__ void eval_init(const EvalInit& fe, int nseg) {
   }
   for (int q = threadIdx.x; q < nseg; q += blockDim.x) fe.clk[q].signal = 0xffffffffu;
   if (fe.hs_npend && (int)threadIdx.x < fe.n_hs_npend) fe.hs_npend[threadIdx.x] = 0u;
+  if (fe.walk_q && threadIdx.x < 8) fe.walk_q[threadIdx.x] = 0u;
 }
 
 struct SegView {
@@ -202,7 +205,7 @@ void launch_mac_record(hipStream_t s, int64_t n, int K, float theta, const TreeB
 void launch_gravity_walk(hipStream_t s, int64_t n, int K, int kahan, const float4* xm, const TreeBufs& tb,
                          const int4* groups, const uint32_t* ngroups, const SegView& sv, float4* accg, uint32_t* flags,
                          unsigned long long* counters, bool pot, bool self,
-                         const NbrIo* nbr = nullptr);
+                         const NbrIo* nbr = nullptr, float mac2 = 1.0f);
 // FP64 direct-sum accelerations (G included) at the particles with the given ids, from the state's positions
 // in storage order (accuracy probe of the walk), each over its own scenario with its softening. inv needs n
 // entries, tpos nt.

@@ -53,7 +53,7 @@ class GpuSph:
                  nbr_capacity: int = 192, use_graph: bool = True, h_margin_seeded: float = 1.03,
                  group_max: int = 64, quadrupole: bool = True, reuse_positions: bool = False,
                  fused_walk: bool = True, theta_mono: float = 0.0,
-                 quad_tol: float = 0.0):
+                 quad_tol: float = 0.0, theta_settled: float = 0.0):
         self._L = _lib.load()
         cfg = default_config()
         cfg.capacity = int(capacity)
@@ -69,6 +69,7 @@ class GpuSph:
         cfg.fused_walk = int(bool(fused_walk))
         cfg.theta_mono = float(theta_mono)
         cfg.quad_tol = float(quad_tol)
+        cfg.theta_settled = float(theta_settled)
         self.config = cfg
         h = C.c_void_p()
         rc = self._L.ihs_create(int(device), C.byref(cfg), C.byref(h))

@@ -125,6 +125,8 @@ pub struct ihs_config {
     pub fused_walk: i32,
     pub theta_mono: f32,
     pub quad_tol: f32,
+    pub theta_settled: f32,
+    pub reserved: [i32; 3],
 }
 
 /// Opaque library handle.
@@ -229,7 +231,7 @@ pub struct ihs_disk_summary {
 // Layouts of the header's structs (checked against the C compiler's by the repository's tests).
 const _: () = assert!(std::mem::size_of::<ihs_particle>() == 48);
 const _: () = assert!(std::mem::size_of::<ihs_eos>() == 48);
-const _: () = assert!(std::mem::size_of::<ihs_config>() == 80);
+const _: () = assert!(std::mem::size_of::<ihs_config>() == 96);
 const _: () = assert!(std::mem::size_of::<ihs_stats>() == 184);
 const _: () = assert!(std::mem::size_of::<ihs_clump>() == 120);
 const _: () = assert!(std::mem::size_of::<ihs_measure>() == 192);

@@ -83,6 +83,13 @@ typedef struct {
                             0: every accepted cell takes its quadrupole (default 0) */
   float quad_tol;        /* with quadrupole: a cell whose quadrupole term is bounded below quad_tol times its
                             monopole term (3.5 |q|_F / r^2 at the group's nearest point) acts as a monopole; 0: off */
+  float theta_settled;   /* > 0: an evaluation at positions and h unchanged since the previous evaluation (the first
+                            of a KDK step, after a kick) keeps that evaluation's neighbour lists and walk groups,
+                            which the unchanged positions and h reproduce, and sums gravity with a walk that builds
+                            no lists at this opening angle (<= theta; without the neighbour search's opened nodes it
+                            needs a smaller angle for the same force error); 0: every evaluation rebuilds the lists
+                            (default 0) */
+  int32_t reserved[3];
 } ihs_config;
 
 typedef struct ihs_handle ihs_handle;

@@ -91,3 +91,35 @@ def test_fused_walk_kdk_matches_separate_build():
     assert a["t"] == pytest.approx(b["t"], rel=1e-4)
     assert a["etot"] == pytest.approx(b["etot"], rel=5e-5)
     assert np.allclose(a["p"], b["p"], rtol=0, atol=1e-6 * ic.m.sum() * ic.meta["v0"])
+
+
+def test_kept_lists_at_unchanged_positions():
+    """theta_settled: an evaluation at positions and h unchanged since the last one keeps that evaluation's neighbour
+    lists and walk groups and walks gravity without building lists. On the 262,144-particle impact after the upload's
+    solve: the kept-list walk runs (its interactions are counted), its force error against the FP64 direct sum stays
+    within the 1e-3 bound, and the density and SPH accelerations equal those of an evaluation that rebuilds the lists
+    (the same pairs inside the kernel support; only the order of the list entries may differ)."""
+    from botheadstudios_amd import ics
+
+    ic = ics.config_ic(2)
+    out = {}
+    for ts in (0.0, 0.8):
+        with GpuSph(ic.n, leaf_size=1, group_max=64, theta=0.9, quadrupole=True, theta_mono=0.6, quad_tol=0.03,
+                    theta_settled=ts, h_mode="newton") as sph:
+            sph.upload(ic.particles48(), ic.eos, ic.softening)
+            sph.stats(reset=True)
+            sph.force_eval()  # positions and h as the upload's last solve left them
+            st = sph.stats(reset=True)
+            err = sph.gravity_error(4096)
+            out[ts] = dict(st=st, err=err, rho=sph.field("rho").copy(), asph=sph.field("asph").copy(),
+                           ncount=sph.field("ncount").copy())
+    a, b = out[0.0], out[0.8]
+    print(f"rebuilt: {a['st']['grav_cell_interactions'] / ic.n:.0f} cells, RMS {a['err']['rms']:.2e}; kept lists: "
+          f"{b['st']['grav_cell_interactions'] / ic.n:.0f} cells, RMS {b['err']['rms']:.2e}")
+    assert b["st"]["grav_cell_interactions"] > 0.5 * a["st"]["grav_cell_interactions"]
+    assert b["st"]["nbr_candidates"] == 0 and a["st"]["nbr_candidates"] > 0
+    assert b["err"]["rms"] < 1e-3 and a["err"]["rms"] < 1e-3
+    assert not np.array_equal(a["ncount"], b["ncount"])  # the kept lists were searched at the solve's wider radius
+    np.testing.assert_allclose(b["rho"], a["rho"], rtol=2e-6)
+    scale = np.abs(a["asph"][:, :3]).max(1, keepdims=True)
+    assert np.max(np.abs(b["asph"][:, :3] - a["asph"][:, :3]) / scale) < 1e-4
