@@ -324,8 +324,9 @@ def roofline(res, n, args, variant, walk_work, equal_work=None):
                 executed=dict(flops_per_launch=exe, tflops=exe_tf, frac=exe_tf / FP32_PEAK_TFLOPS,
                               cell_per_particle=pst["grav_cell_interactions"] / max(calls, 1) / n,
                               pp_per_particle=pst["grav_pp_interactions"] / max(calls, 1) / n,
-                              definition=f"the walk's own interactions: {cell_flops:.0f} flops per accepted cell, 20 "
-                                         "per leaf particle"))
+                              definition=f"the walk's own interactions: {cell_flops:.0f} flops per accepted "
+                                         "quadrupole cell, 20 per monopole (leaf particle, or cell that acts as a "
+                                         "monopole)"))
 
 
 def _drift(e0, e1):
